@@ -1,0 +1,202 @@
+"""CPU oracle for the CKF path -- TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker / the timed CPU baseline; the product never imports it.
+
+liboracle_ckf.so is this project's restatement of traccc's host CKF
+(core/include/traccc/finding/details/combinatorial_kalman_filter.hpp:59-718),
+templated on the scalar: FP64 = oracle, FP32 = timed CPU baseline.
+
+Pinning status of each function (what fixes it independently of the
+restatement itself) is listed in tests/test_oracle_pins.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_ckf.so")
+
+from traccc_amd import _ffi as F  # noqa: E402  (struct layouts only)
+
+
+class OracleTracks(C.Structure):
+    _fields_ = [
+        ("n_tracks", C.c_uint32),
+        ("n_constituents", C.c_uint32),
+        ("params_surface_link", F.c_u64_p),
+        ("params_vector", F.c_float_p),
+        ("params_covariance", F.c_float_p),
+        ("ndf", C.POINTER(C.c_double)),
+        ("chi2", C.POINTER(C.c_double)),
+        ("pval", C.POINTER(C.c_double)),
+        ("nholes", F.c_u32_p),
+        ("constituent_offsets", F.c_u32_p),
+        ("constituent_index", F.c_u32_p),
+        ("n_steps", C.c_uint32),
+        ("n_links", C.c_uint64),
+        ("n_propagations", C.c_uint64),
+        ("n_tips", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} not built (make -C oracle)")
+        L = C.CDLL(LIB)
+        dptr = C.POINTER(C.c_double)
+        L.oracle_ckf_run.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
+                                     C.POINTER(F.BField), C.POINTER(F.Measurements),
+                                     C.POINTER(F.BoundParams), C.c_int, C.POINTER(OracleTracks)]
+        L.oracle_ckf_run.restype = C.c_int
+        L.oracle_tracks_free.argtypes = [C.POINTER(OracleTracks)]
+        L.oracle_ckf_run_many.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
+                                          C.POINTER(F.BField), C.POINTER(F.Measurements),
+                                          C.POINTER(F.BoundParams), C.c_uint32, C.c_int,
+                                          C.c_uint32, C.POINTER(C.c_uint64), dptr]
+        L.oracle_ckf_run_many.restype = C.c_int
+        L.oracle_prob.argtypes = [C.c_double, C.c_double]
+        L.oracle_prob.restype = C.c_double
+        L.oracle_prob_f.argtypes = [C.c_float, C.c_float]
+        L.oracle_prob_f.restype = C.c_float
+        L.oracle_kf_update.argtypes = [dptr, dptr, F.c_float_p, F.c_float_p, C.c_uint32,
+                                       F.c_u8_p, dptr, dptr, dptr]
+        L.oracle_kf_update.restype = C.c_int
+        L.oracle_propagate.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
+                                       C.POINTER(F.BField), C.c_uint32, dptr, dptr, C.c_int,
+                                       F.c_u32_p, dptr, dptr, dptr, F.c_u32_p]
+        L.oracle_propagate.restype = C.c_int
+        L.oracle_rk_step.argtypes = [C.POINTER(F.BField), C.c_double, C.c_double, C.c_double,
+                                     dptr, dptr, dptr]
+        _lib = L
+    return _lib
+
+
+@dataclass
+class Tracks:
+    """Track candidates (traccc track_container fields) from the oracle."""
+    params_surface_link: np.ndarray
+    params_vector: np.ndarray
+    ndf: np.ndarray
+    chi2: np.ndarray
+    pval: np.ndarray
+    nholes: np.ndarray
+    offsets: np.ndarray
+    constituents: np.ndarray
+    n_steps: int
+    n_links: int
+    n_propagations: int
+    n_tips: int
+
+    def __len__(self):
+        return len(self.ndf)
+
+    def track(self, i: int) -> np.ndarray:
+        return self.constituents[self.offsets[i]:self.offsets[i + 1]]
+
+
+def run(cfg, det, fld, event, precision: int = 64) -> Tracks:
+    L = lib()
+    out = OracleTracks()
+    m = event.measurements_struct()
+    s = event.seeds_struct()
+    rc = L.oracle_ckf_run(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), C.byref(m),
+                          C.byref(s), precision, C.byref(out))
+    if rc != 0:
+        raise ValueError(f"oracle_ckf_run failed ({rc})")
+    try:
+        n, nc = out.n_tracks, out.n_constituents
+        t = Tracks(
+            params_surface_link=F.np_from(out.params_surface_link, n, np.uint64),
+            params_vector=F.np_from(out.params_vector, 6 * n, np.float32, (n, 6)),
+            ndf=F.np_from(out.ndf, n, np.float64),
+            chi2=F.np_from(out.chi2, n, np.float64),
+            pval=F.np_from(out.pval, n, np.float64),
+            nholes=F.np_from(out.nholes, n, np.uint32),
+            offsets=F.np_from(out.constituent_offsets, n + 1, np.uint32),
+            constituents=F.np_from(out.constituent_index, nc, np.uint32),
+            n_steps=out.n_steps, n_links=out.n_links, n_propagations=out.n_propagations,
+            n_tips=out.n_tips)
+    finally:
+        L.oracle_tracks_free(C.byref(out))
+    return t
+
+
+def run_many(cfg, det, fld, events, precision: int = 32, threads: int = 1):
+    """Time the CPU CKF over events on `threads` host threads.
+    Returns (seconds, n_tracks_total)."""
+    L = lib()
+    n = len(events)
+    ms = (F.Measurements * n)(*[e.measurements_struct() for e in events])
+    ss = (F.BoundParams * n)(*[e.seeds_struct() for e in events])
+    tot = C.c_uint64()
+    sec = C.c_double()
+    rc = L.oracle_ckf_run_many(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), ms, ss,
+                               n, precision, threads, C.byref(tot), C.byref(sec))
+    if rc != 0:
+        raise ValueError("oracle_ckf_run_many failed")
+    return sec.value, tot.value
+
+
+def prob(chi2: float, ndf: float) -> float:
+    return lib().oracle_prob(chi2, ndf)
+
+
+def prob_f(chi2: float, ndf: float) -> float:
+    return lib().oracle_prob_f(chi2, ndf)
+
+
+def kf_update(vec, cov, meas, var, dims=2, subspace=(0, 1)):
+    """Returns (status, chi2, filtered_vec, filtered_cov) in FP64."""
+    L = lib()
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    c = np.ascontiguousarray(cov, dtype=np.float64).reshape(36)
+    m = np.ascontiguousarray(meas, dtype=np.float32)
+    va = np.ascontiguousarray(var, dtype=np.float32)
+    su = np.ascontiguousarray(subspace, dtype=np.uint8)
+    chi2 = C.c_double()
+    ov = np.zeros(6)
+    oc = np.zeros(36)
+    dp = C.POINTER(C.c_double)
+    st = L.oracle_kf_update(v.ctypes.data_as(dp), c.ctypes.data_as(dp), F.ptr(m, C.c_float),
+                            F.ptr(va, C.c_float), dims, F.ptr(su, C.c_uint8), C.byref(chi2),
+                            ov.ctypes.data_as(dp), oc.ctypes.data_as(dp))
+    return st, chi2.value, ov, oc.reshape(6, 6)
+
+
+def propagate(cfg, det, fld, start: int, vec, cov, apply_start_material=False):
+    """Returns (status, end_surface, vec, cov, path, n_steps) in FP64."""
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    c = np.ascontiguousarray(cov, dtype=np.float64).reshape(36)
+    es = C.c_uint32()
+    ov = np.zeros(6)
+    oc = np.zeros(36)
+    path = C.c_double()
+    ns = C.c_uint32()
+    st = L.oracle_propagate(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), start,
+                            v.ctypes.data_as(dp), c.ctypes.data_as(dp), int(apply_start_material),
+                            C.byref(es), ov.ctypes.data_as(dp), oc.ctypes.data_as(dp),
+                            C.byref(path), C.byref(ns))
+    return st, es.value, ov, oc.reshape(6, 6), path.value, ns.value
+
+
+def rk_step(fld, mass, abs_q, h, free_in):
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    fi = np.ascontiguousarray(free_in, dtype=np.float64)
+    fo = np.zeros(8)
+    D = np.zeros(64)
+    L.oracle_rk_step(C.byref(fld.struct), mass, abs_q, h, fi.ctypes.data_as(dp),
+                     fo.ctypes.data_as(dp), D.ctypes.data_as(dp))
+    return fo, D.reshape(8, 8)

@@ -1,0 +1,1278 @@
+// CPU oracle for the CKF track-finding path -- TEST INFRASTRUCTURE ONLY.
+//
+// Restates traccc's host CKF (core/include/traccc/finding/details/
+// combinatorial_kalman_filter.hpp:59-718) step for step, with dense-matrix
+// Kalman algebra as in the reference (measurement_selector.hpp:152-212,
+// gain_matrix_updater.hpp:69-181) and this project's propagation model
+// (RK4-Nystroem + semi-analytic transport + layer-window navigation) written
+// independently of the product's kernels, in dense 8x8 free-parameter form.
+// Templated on the scalar: double = oracle, float = timed CPU baseline.
+//
+// Deliberate deviation from the reference, applied identically in the
+// product: the per-surface branch sort breaks chi2 ties by the lower
+// measurement index (traccc host uses std::sort, whose tie order is
+// unspecified; the traccc device kernel uses the same lower-index rule,
+// device/common/include/traccc/finding/device/impl/find_tracks.ipp:474-477).
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace oracle {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kUnitT = 0.000299792458;
+constexpr double kC = 299.792458;
+
+// ---------------------------------------------------------------------------
+// Dense matrices
+// ---------------------------------------------------------------------------
+template <typename S, int R, int C>
+struct mat {
+    S a[R][C];
+    static mat zero() {
+        mat m;
+        for (int i = 0; i < R; ++i)
+            for (int j = 0; j < C; ++j) m.a[i][j] = S(0);
+        return m;
+    }
+    static mat identity() {
+        mat m = zero();
+        for (int i = 0; i < (R < C ? R : C); ++i) m.a[i][i] = S(1);
+        return m;
+    }
+    S& operator()(int i, int j) { return a[i][j]; }
+    const S& operator()(int i, int j) const { return a[i][j]; }
+};
+
+template <typename S, int R, int K, int C>
+mat<S, R, C> operator*(const mat<S, R, K>& x, const mat<S, K, C>& y) {
+    mat<S, R, C> m;
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < C; ++j) {
+            S acc = S(0);
+            for (int k = 0; k < K; ++k) acc += x.a[i][k] * y.a[k][j];
+            m.a[i][j] = acc;
+        }
+    return m;
+}
+template <typename S, int R, int C>
+mat<S, R, C> operator+(const mat<S, R, C>& x, const mat<S, R, C>& y) {
+    mat<S, R, C> m;
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < C; ++j) m.a[i][j] = x.a[i][j] + y.a[i][j];
+    return m;
+}
+template <typename S, int R, int C>
+mat<S, R, C> operator-(const mat<S, R, C>& x, const mat<S, R, C>& y) {
+    mat<S, R, C> m;
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < C; ++j) m.a[i][j] = x.a[i][j] - y.a[i][j];
+    return m;
+}
+template <typename S, int R, int C>
+mat<S, C, R> transpose(const mat<S, R, C>& x) {
+    mat<S, C, R> m;
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < C; ++j) m.a[j][i] = x.a[i][j];
+    return m;
+}
+
+template <typename S>
+using v3 = mat<S, 3, 1>;
+
+template <typename S>
+v3<S> mk3(S x, S y, S z) {
+    v3<S> v;
+    v.a[0][0] = x;
+    v.a[1][0] = y;
+    v.a[2][0] = z;
+    return v;
+}
+template <typename S>
+S dot3(const v3<S>& a, const v3<S>& b) {
+    return a.a[0][0] * b.a[0][0] + a.a[1][0] * b.a[1][0] + a.a[2][0] * b.a[2][0];
+}
+template <typename S>
+v3<S> cross3(const v3<S>& a, const v3<S>& b) {
+    return mk3<S>(a.a[1][0] * b.a[2][0] - a.a[2][0] * b.a[1][0],
+                  a.a[2][0] * b.a[0][0] - a.a[0][0] * b.a[2][0],
+                  a.a[0][0] * b.a[1][0] - a.a[1][0] * b.a[0][0]);
+}
+template <typename S>
+v3<S> scale3(S s, const v3<S>& a) {
+    return mk3<S>(s * a.a[0][0], s * a.a[1][0], s * a.a[2][0]);
+}
+template <typename S>
+v3<S> fl3(const float* f) {
+    return mk3<S>(static_cast<S>(f[0]), static_cast<S>(f[1]), static_cast<S>(f[2]));
+}
+
+// ---------------------------------------------------------------------------
+// Bound / free parameter containers
+// ---------------------------------------------------------------------------
+template <typename S>
+struct bound_state {
+    uint32_t surface;
+    mat<S, 6, 1> vec;
+    mat<S, 6, 6> cov;
+};
+
+// ---------------------------------------------------------------------------
+// Field: constant or bilinear (r, z) table, clamped at the grid edge.
+// ---------------------------------------------------------------------------
+template <typename S>
+v3<S> bfield(const ckf_bfield& f, const v3<S>& p) {
+    if (f.kind == CKF_BFIELD_CONSTANT) {
+        return mk3<S>(static_cast<S>(f.b[0] * kUnitT), static_cast<S>(f.b[1] * kUnitT),
+                      static_cast<S>(f.b[2] * kUnitT));
+    }
+    const S x = p.a[0][0], y = p.a[1][0], z = p.a[2][0];
+    const S r = std::sqrt(x * x + y * y);
+    S gr = (r - static_cast<S>(f.r_min)) * static_cast<S>(1.0 / f.dr);
+    S gz = (z - static_cast<S>(f.z_min)) * static_cast<S>(1.0 / f.dz);
+    gr = std::min(std::max(gr, S(0)), static_cast<S>(f.n_r - 1));
+    gz = std::min(std::max(gz, S(0)), static_cast<S>(f.n_z - 1));
+    const int ir = std::min(static_cast<int>(gr), static_cast<int>(f.n_r) - 2);
+    const int iz = std::min(static_cast<int>(gz), static_cast<int>(f.n_z) - 2);
+    const S fr = gr - static_cast<S>(ir);
+    const S fz = gz - static_cast<S>(iz);
+    auto at = [&](int jz, int jr, int c) {
+        return static_cast<S>(f.map[2 * (jz * static_cast<int>(f.n_r) + jr) + c]);
+    };
+    S b[2];
+    for (int c = 0; c < 2; ++c) {
+        b[c] = (S(1) - fr) * (S(1) - fz) * at(iz, ir, c) + fr * (S(1) - fz) * at(iz, ir + 1, c) +
+               (S(1) - fr) * fz * at(iz + 1, ir, c) + fr * fz * at(iz + 1, ir + 1, c);
+        b[c] *= static_cast<S>(kUnitT);
+    }
+    if (r > S(0)) {
+        return mk3<S>(b[0] * x / r, b[0] * y / r, b[1]);
+    }
+    return mk3<S>(S(0), S(0), b[1]);
+}
+
+// ---------------------------------------------------------------------------
+// RK4-Nystroem step on the free vector F = [x, y, z, t, Tx, Ty, Tz, q/p] and
+// its 8x8 transport matrix D = dF'/dF (field gradient neglected).
+// ---------------------------------------------------------------------------
+template <typename S>
+S dt_ds(S qop, S mass, S abs_q) {
+    const S mq = mass * qop / abs_q;
+    return std::sqrt(S(1) + mq * mq) / static_cast<S>(kC);
+}
+
+template <typename S>
+void rk_step(const ckf_bfield& field, S mass, S abs_q, S h, mat<S, 8, 1>& F, mat<S, 8, 8>* D) {
+    const v3<S> r0 = mk3<S>(F.a[0][0], F.a[1][0], F.a[2][0]);
+    const v3<S> T0 = mk3<S>(F.a[4][0], F.a[5][0], F.a[6][0]);
+    const S q = F.a[7][0];
+    const S half = h * S(0.5);
+
+    const v3<S> B1 = bfield<S>(field, r0);
+    const v3<S> k1 = scale3(q, cross3(T0, B1));
+    const v3<S> r2 = r0 + scale3(half, T0) + scale3(h * h * S(0.125), k1);
+    const v3<S> B2 = bfield<S>(field, r2);
+    const v3<S> T2 = T0 + scale3(half, k1);
+    const v3<S> k2 = scale3(q, cross3(T2, B2));
+    const v3<S> T3 = T0 + scale3(half, k2);
+    const v3<S> k3 = scale3(q, cross3(T3, B2));
+    const v3<S> r4 = r0 + scale3(h, T0) + scale3(h * h * S(0.5), k3);
+    const v3<S> B4 = bfield<S>(field, r4);
+    const v3<S> T4 = T0 + scale3(h, k3);
+    const v3<S> k4 = scale3(q, cross3(T4, B4));
+
+    if (D != nullptr) {
+        // dk_i/dF as 3x8 blocks.  k = q T_i x B_i; dT_i/dF from the previous stage.
+        auto xB = [&](const mat<S, 3, 8>& dT, const v3<S>& B) {
+            // d(q * T x B)/dF restricted to the T dependence
+            mat<S, 3, 8> m;
+            for (int j = 0; j < 8; ++j) {
+                const v3<S> col = mk3<S>(dT.a[0][j], dT.a[1][j], dT.a[2][j]);
+                const v3<S> c = scale3(q, cross3(col, B));
+                m.a[0][j] = c.a[0][0];
+                m.a[1][j] = c.a[1][0];
+                m.a[2][j] = c.a[2][0];
+            }
+            return m;
+        };
+        mat<S, 3, 8> dT1 = mat<S, 3, 8>::zero();
+        dT1.a[0][4] = dT1.a[1][5] = dT1.a[2][6] = S(1);
+        mat<S, 3, 8> dk1 = xB(dT1, B1);
+        {
+            const v3<S> c = cross3(T0, B1);
+            for (int i = 0; i < 3; ++i) dk1.a[i][7] += c.a[i][0];
+        }
+        mat<S, 3, 8> dT2 = dT1;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 8; ++j) dT2.a[i][j] += half * dk1.a[i][j];
+        mat<S, 3, 8> dk2 = xB(dT2, B2);
+        {
+            const v3<S> c = cross3(T2, B2);
+            for (int i = 0; i < 3; ++i) dk2.a[i][7] += c.a[i][0];
+        }
+        mat<S, 3, 8> dT3 = dT1;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 8; ++j) dT3.a[i][j] += half * dk2.a[i][j];
+        mat<S, 3, 8> dk3 = xB(dT3, B2);
+        {
+            const v3<S> c = cross3(T3, B2);
+            for (int i = 0; i < 3; ++i) dk3.a[i][7] += c.a[i][0];
+        }
+        mat<S, 3, 8> dT4 = dT1;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 8; ++j) dT4.a[i][j] += h * dk3.a[i][j];
+        mat<S, 3, 8> dk4 = xB(dT4, B4);
+        {
+            const v3<S> c = cross3(T4, B4);
+            for (int i = 0; i < 3; ++i) dk4.a[i][7] += c.a[i][0];
+        }
+        mat<S, 8, 8> M = mat<S, 8, 8>::identity();
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 8; ++j) {
+                // position rows: r' = r + h T + h^2/6 (k1 + k2 + k3)
+                M.a[i][j] += (j == 4 + i ? h : S(0)) +
+                             h * h / S(6) * (dk1.a[i][j] + dk2.a[i][j] + dk3.a[i][j]);
+                // direction rows: T' = T + h/6 (k1 + 2 k2 + 2 k3 + k4)
+                M.a[4 + i][j] += h / S(6) *
+                                 (dk1.a[i][j] + S(2) * dk2.a[i][j] + S(2) * dk3.a[i][j] + dk4.a[i][j]);
+            }
+        }
+        // time row: t' = t + h / (beta c)
+        const S mq = mass / abs_q;
+        M.a[3][7] += h * mq * mq * q / (static_cast<S>(kC) * static_cast<S>(kC) * dt_ds(q, mass, abs_q));
+        *D = M * (*D);
+    }
+
+    const v3<S> rn = r0 + scale3(h, T0) + scale3(h * h / S(6), k1 + k2 + k3);
+    v3<S> Tn = T0 + scale3(h / S(6), k1 + scale3(S(2), k2) + scale3(S(2), k3) + k4);
+    Tn = scale3(S(1) / std::sqrt(dot3(Tn, Tn)), Tn);
+    F.a[0][0] = rn.a[0][0];
+    F.a[1][0] = rn.a[1][0];
+    F.a[2][0] = rn.a[2][0];
+    F.a[3][0] += h * dt_ds(q, mass, abs_q);
+    F.a[4][0] = Tn.a[0][0];
+    F.a[5][0] = Tn.a[1][0];
+    F.a[6][0] = Tn.a[2][0];
+}
+
+// ---------------------------------------------------------------------------
+// Geometry queries
+// ---------------------------------------------------------------------------
+template <typename S>
+bool in_mask(const ckf_surface& s, S l0, S l1, S tol) {
+    if (s.bound_kind == CKF_BOUND_RECTANGLE) {
+        return std::fabs(l0) <= static_cast<S>(s.bound[0]) + tol &&
+               std::fabs(l1) <= static_cast<S>(s.bound[1]) + tol;
+    }
+    const S hh = static_cast<S>(s.bound[2]);
+    if (std::fabs(l1) > hh + tol) return false;
+    const S lim = static_cast<S>(s.bound[0]) +
+                  (static_cast<S>(s.bound[1]) - static_cast<S>(s.bound[0])) * (l1 + hh) / (S(2) * hh);
+    return std::fabs(l0) <= lim + tol;
+}
+
+// Straight-line entry distance into a layer envelope (0 inside, +inf never).
+template <typename S>
+S envelope_entry(const ckf_layer& L, const v3<S>& p, const v3<S>& d, S margin) {
+    const S inf = static_cast<S>(1e30);
+    if (L.kind == CKF_LAYER_PLANE) {
+        const v3<S> n = fl3<S>(L.normal);
+        const S lo = static_cast<S>(L.env_r[0]), hi = static_cast<S>(L.env_r[1]);
+        const S here = dot3(n, p);
+        if (here >= lo && here <= hi) return S(0);
+        const S rate = dot3(n, d);
+        if (std::fabs(rate) < static_cast<S>(1e-12)) return inf;
+        S best = inf;
+        for (S face : {lo, hi}) {
+            const S s = (face - here) / rate;
+            if (s > S(0) && s < best) best = s;
+        }
+        return best;
+    }
+    const S rlo = static_cast<S>(L.env_r[0]), rhi = static_cast<S>(L.env_r[1]);
+    const S zlo = static_cast<S>(L.env_z[0]), zhi = static_cast<S>(L.env_z[1]);
+    const S px = p.a[0][0], py = p.a[1][0], pz = p.a[2][0];
+    const S dx = d.a[0][0], dy = d.a[1][0], dz = d.a[2][0];
+    const S rho2 = px * px + py * py;
+    if (rho2 >= rlo * rlo && rho2 <= rhi * rhi && pz >= zlo && pz <= zhi) return S(0);
+    S best = inf;
+    const S A = dx * dx + dy * dy;
+    if (A > static_cast<S>(1e-12)) {
+        const S Bh = px * dx + py * dy;
+        for (S rad : {rlo, rhi}) {
+            const S disc = Bh * Bh - A * (rho2 - rad * rad);
+            if (disc < S(0)) continue;
+            const S sq = std::sqrt(disc);
+            for (S s : {(-Bh - sq) / A, (-Bh + sq) / A}) {
+                if (s > S(0) && s < best) {
+                    const S zc = pz + s * dz;
+                    if (zc >= zlo - margin && zc <= zhi + margin) best = s;
+                }
+            }
+        }
+    }
+    if (std::fabs(dz) > static_cast<S>(1e-12)) {
+        for (S face : {zlo, zhi}) {
+            const S s = (face - pz) / dz;
+            if (s > S(0) && s < best) {
+                const S xc = px + s * dx, yc = py + s * dy;
+                const S rr = xc * xc + yc * yc;
+                const S lo = std::max(rlo - margin, S(0)), hi = rhi + margin;
+                if (rr >= lo * lo && rr <= hi * hi) best = s;
+            }
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Propagation engine for one track
+// ---------------------------------------------------------------------------
+template <typename S>
+struct propagation {
+    const ckf_finding_config& cfg;
+    const ckf_detector& det;
+    const ckf_bfield& field;
+    bool with_jacobian;
+    S mass, abs_q;
+
+    mat<S, 8, 1> F;
+    mat<S, 8, 8> D;  // accumulated transport
+    S path = S(0);
+    uint32_t steps = 0;
+
+    S step_scale() const {
+        return static_cast<S>(std::pow(120.0 * static_cast<double>(cfg.propagation.rk_error_tol), 0.2));
+    }
+
+    v3<S> pos() const { return mk3<S>(F.a[0][0], F.a[1][0], F.a[2][0]); }
+    v3<S> dir() const { return mk3<S>(F.a[4][0], F.a[5][0], F.a[6][0]); }
+
+    S accuracy_step() const {
+        const v3<S> B = bfield<S>(field, pos());
+        const v3<S> tb = cross3(dir(), B);
+        const S kappa = std::fabs(F.a[7][0]) * std::sqrt(dot3(tb, tb));
+        S h = static_cast<S>(cfg.propagation.step_constraint);
+        if (kappa > static_cast<S>(1e-12)) {
+            h = std::min(h, step_scale() * std::pow(kappa, static_cast<S>(-0.8)));
+        }
+        return h;
+    }
+
+    bool step(S h) {
+        rk_step<S>(field, mass, abs_q, h, F, with_jacobian ? &D : nullptr);
+        path += h;
+        ++steps;
+        return steps <= cfg.max_step_counts_for_next_surface &&
+               std::fabs(path) <= static_cast<S>(cfg.propagation.path_limit);
+    }
+
+    // 0 on plane, 1 abort, 2 parallel
+    int onto_plane(const ckf_surface& s) {
+        const v3<S> c = fl3<S>(s.center);
+        const v3<S> w = fl3<S>(s.normal);
+        S dist = S(0);
+        while (true) {
+            const S cosw = dot3(dir(), w);
+            if (std::fabs(cosw) < static_cast<S>(1e-8)) return 2;
+            dist = dot3(c - pos(), w) / cosw;
+            if (std::fabs(dist) <= static_cast<S>(cfg.propagation.on_surface_tolerance)) break;
+            const S lim = accuracy_step();
+            const S h = dist > S(0) ? std::min(dist, lim) : std::max(dist, -lim);
+            if (!step(h)) return 1;
+        }
+        for (int i = 0; i < 3; ++i) F.a[i][0] += dist * F.a[4 + i][0];
+        path += dist;
+        F.a[3][0] += dist * dt_ds(F.a[7][0], mass, abs_q);
+        return 0;
+    }
+
+    // 0 found, 1 none, 2 abort
+    int scan_layer(uint32_t li, int skip, uint32_t& found) {
+        const ckf_layer& L = det.layers[li];
+        const S margin = static_cast<S>(cfg.propagation.max_mask_tolerance);
+        S se = envelope_entry<S>(L, pos(), dir(), margin);
+        if (!(se < static_cast<S>(1e29))) se = S(0);
+        const v3<S> at = pos() + scale3(se, dir());
+        const int nphi = static_cast<int>(L.n_phi), nlong = static_cast<int>(L.n_long);
+        int cphi = 0, clong = 0;
+        int wphi = static_cast<int>(cfg.propagation.search_window[0]);
+        int wlong = L.kind == CKF_LAYER_BARREL ? static_cast<int>(cfg.propagation.search_window[1]) : 0;
+        if (L.kind == CKF_LAYER_PLANE) {
+            wphi = (nphi - 1) / 2;
+            wlong = (nlong - 1) / 2;
+            cphi = wphi;
+            clong = wlong;
+        } else {
+            const S ph = std::atan2(at.a[1][0], at.a[0][0]);
+            const S pitch = static_cast<S>(2.0 * kPi) / static_cast<S>(nphi);
+            cphi = static_cast<int>(std::floor((ph - static_cast<S>(L.phi0)) / pitch + S(0.5))) % nphi;
+            if (cphi < 0) cphi += nphi;
+            if (L.kind == CKF_LAYER_BARREL && nlong > 1) {
+                clong = static_cast<int>(std::floor((at.a[2][0] - static_cast<S>(L.long0)) /
+                                                        static_cast<S>(L.long_pitch) + S(0.5)));
+                clong = std::min(std::max(clong, 0), nlong - 1);
+            }
+        }
+        if (2 * wphi + 1 > nphi) wphi = (nphi - 1) / 2;
+        struct cand {
+            S s;
+            uint32_t idx;
+        };
+        cand cbuf[9];
+        size_t nc = 0;
+        for (int a = -wphi; a <= wphi; ++a) {
+            const int ip = ((cphi + a) % nphi + nphi) % nphi;
+            for (int b = -wlong; b <= wlong; ++b) {
+                const int il = clong + b;
+                if (il < 0 || il >= nlong) continue;
+                const uint32_t m = L.first_surface + static_cast<uint32_t>(ip * nlong + il);
+                if (static_cast<int>(m) == skip) continue;
+                const ckf_surface& sf = det.surfaces[m];
+                const v3<S> w = fl3<S>(sf.normal);
+                const S cosw = dot3(dir(), w);
+                if (std::fabs(cosw) < static_cast<S>(1e-6)) continue;
+                const S s = dot3(fl3<S>(sf.center) - pos(), w) / cosw;
+                if (s < static_cast<S>(cfg.propagation.overstep_tolerance)) continue;
+                const v3<S> rel = pos() + scale3(s, dir()) - fl3<S>(sf.center);
+                const S tol = std::min(static_cast<S>(cfg.propagation.min_mask_tolerance) +
+                                           static_cast<S>(cfg.propagation.mask_tolerance_scalor) * std::fabs(s),
+                                       static_cast<S>(cfg.propagation.max_mask_tolerance));
+                if (!in_mask<S>(sf, dot3(rel, fl3<S>(sf.axis_u)), dot3(rel, fl3<S>(sf.axis_v)), tol)) continue;
+                if (nc < 9u) cbuf[nc++] = {s, m};
+            }
+        }
+        std::sort(cbuf, cbuf + nc, [](const cand& x, const cand& y) {
+            return x.s < y.s || (x.s == y.s && x.idx < y.idx);
+        });
+        for (size_t ci = 0; ci < nc; ++ci) {
+            const cand& c = cbuf[ci];
+            const ckf_surface& sf = det.surfaces[c.idx];
+            const v3<S> w = fl3<S>(sf.normal);
+            const S cosw = dot3(dir(), w);
+            if (std::fabs(cosw) < static_cast<S>(1e-8)) continue;
+            if (dot3(fl3<S>(sf.center) - pos(), w) / cosw < static_cast<S>(cfg.propagation.overstep_tolerance)) continue;
+            const int r = onto_plane(sf);
+            if (r == 1) return 2;
+            if (r == 2) continue;
+            const v3<S> rel = pos() - fl3<S>(sf.center);
+            if (in_mask<S>(sf, dot3(rel, fl3<S>(sf.axis_u)), dot3(rel, fl3<S>(sf.axis_v)), S(0)) &&
+                path > static_cast<S>(cfg.min_step_length_for_next_surface)) {
+                found = c.idx;
+                return 0;
+            }
+        }
+        return 1;
+    }
+
+    // 0 found, 1 exit, 2 abort
+    int run(int start, uint32_t& found) {
+        path = S(0);
+        steps = 0u;
+        const S p = abs_q / std::fabs(F.a[7][0]);
+        const S pT = p * std::sqrt(F.a[4][0] * F.a[4][0] + F.a[5][0] * F.a[5][0]);
+        if (p < static_cast<S>(cfg.min_p) || pT < static_cast<S>(cfg.min_pT)) return 2;
+        uint64_t done = 0ull;
+        if (start >= 0) {
+            const uint32_t l0 = det.surfaces[start].layer;
+            const int r = scan_layer(l0, start, found);
+            if (r == 0) return 0;
+            if (r == 2) return 2;
+            done |= 1ull << l0;
+        }
+        const S margin = static_cast<S>(cfg.propagation.max_mask_tolerance);
+        const S approach = S(25);
+        for (uint32_t round = 0; round <= det.n_layers; ++round) {
+            int next = -1;
+            S dist = static_cast<S>(1e30);
+            for (uint32_t li = 0; li < det.n_layers; ++li) {
+                if (done & (1ull << li)) continue;
+                const S s = envelope_entry<S>(det.layers[li], pos(), dir(), margin);
+                if (s < dist) {
+                    dist = s;
+                    next = static_cast<int>(li);
+                }
+            }
+            if (next < 0 || !(dist < static_cast<S>(1e29))) return 1;
+            while (dist > approach && dist < static_cast<S>(1e29)) {
+                if (!step(std::min(dist, accuracy_step()))) return 2;
+                dist = envelope_entry<S>(det.layers[next], pos(), dir(), margin);
+            }
+            if (dist < static_cast<S>(1e29)) {
+                const int r = scan_layer(static_cast<uint32_t>(next), -1, found);
+                if (r == 0) return 0;
+                if (r == 2) return 2;
+            }
+            done |= 1ull << next;
+        }
+        return 1;
+    }
+};
+
+template <typename S>
+mat<S, 8, 6> bound_to_free_jac(const ckf_surface& sf, const mat<S, 6, 1>& v) {
+    const S phi = v.a[2][0], th = v.a[3][0];
+    mat<S, 8, 6> J = mat<S, 8, 6>::zero();
+    for (int i = 0; i < 3; ++i) {
+        J.a[i][0] = static_cast<S>(sf.axis_u[i]);
+        J.a[i][1] = static_cast<S>(sf.axis_v[i]);
+    }
+    J.a[3][5] = S(1);
+    J.a[4][2] = -std::sin(phi) * std::sin(th);
+    J.a[5][2] = std::cos(phi) * std::sin(th);
+    J.a[6][2] = S(0);
+    J.a[4][3] = std::cos(phi) * std::cos(th);
+    J.a[5][3] = std::sin(phi) * std::cos(th);
+    J.a[6][3] = -std::sin(th);
+    J.a[7][4] = S(1);
+    return J;
+}
+
+// Highland width (PDG eq. 34.15)
+template <typename S>
+S highland(S x, S qop, S mass, S abs_q) {
+    if (!(x > S(0))) return S(0);
+    const S p = abs_q / std::fabs(qop);
+    const S beta = p / std::sqrt(p * p + mass * mass);
+    return S(0.0136) / (beta * p) * abs_q * std::sqrt(x) *
+           (S(1) + S(0.038) * std::log(x * abs_q * abs_q / (beta * beta)));
+}
+
+// Multiple-scattering noise of the module the state sits on
+// (pointwise material interaction, covariance part).
+template <typename S>
+void add_material(const ckf_surface& sf, bound_state<S>& st, S mass, S abs_q) {
+    if (!(sf.thickness_x0 > 0.f)) return;
+    const S phi = st.vec.a[2][0], th = st.vec.a[3][0];
+    const v3<S> d = mk3<S>(std::cos(phi) * std::sin(th), std::sin(phi) * std::sin(th), std::cos(th));
+    const S cosi = std::fabs(dot3(d, fl3<S>(sf.normal)));
+    const S t0 = highland<S>(static_cast<S>(sf.thickness_x0) / cosi, st.vec.a[4][0], mass, abs_q);
+    const S sth = std::sin(th);
+    st.cov.a[2][2] += t0 * t0 / (sth * sth);
+    st.cov.a[3][3] += t0 * t0;
+}
+
+// Propagate a bound state to the next sensitive surface.
+// 0 found (out filled), 1 exit, 2 abort.
+template <typename S>
+int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, const ckf_bfield& field,
+                    const bound_state<S>& in, S mass, S abs_q, bound_state<S>& out, S* path_out,
+                    uint32_t* steps_out) {
+    const ckf_surface& s0 = det.surfaces[in.surface];
+    propagation<S> P{cfg, det, field, true, mass, abs_q, mat<S, 8, 1>::zero(), mat<S, 8, 8>::identity(),
+                     S(0), 0u};
+    const S l0 = in.vec.a[0][0], l1 = in.vec.a[1][0], phi = in.vec.a[2][0], th = in.vec.a[3][0];
+    for (int i = 0; i < 3; ++i) {
+        P.F.a[i][0] = static_cast<S>(s0.center[i]) + l0 * static_cast<S>(s0.axis_u[i]) +
+                      l1 * static_cast<S>(s0.axis_v[i]);
+    }
+    P.F.a[3][0] = in.vec.a[5][0];
+    P.F.a[4][0] = std::cos(phi) * std::sin(th);
+    P.F.a[5][0] = std::sin(phi) * std::sin(th);
+    P.F.a[6][0] = std::cos(th);
+    P.F.a[7][0] = in.vec.a[4][0];
+    P.D = mat<S, 8, 8>::identity();
+    uint32_t found = 0u;
+    const int r = P.run(static_cast<int>(in.surface), found);
+    if (path_out) *path_out = P.path;
+    if (steps_out) *steps_out = P.steps;
+    if (r != 0) return r;
+
+    const ckf_surface& s1 = det.surfaces[found];
+    const v3<S> rel = P.pos() - fl3<S>(s1.center);
+    const v3<S> T = P.dir();
+    out.surface = found;
+    out.vec.a[0][0] = dot3(rel, fl3<S>(s1.axis_u));
+    out.vec.a[1][0] = dot3(rel, fl3<S>(s1.axis_v));
+    out.vec.a[2][0] = std::atan2(T.a[1][0], T.a[0][0]);
+    out.vec.a[3][0] = std::acos(std::min(std::max(T.a[2][0], S(-1)), S(1)));
+    out.vec.a[4][0] = P.F.a[7][0];
+    out.vec.a[5][0] = P.F.a[3][0];
+
+    // path correction onto the plane: dF -> dF - (dF/ds) (w . dr)/(w . T)
+    const v3<S> w = fl3<S>(s1.normal);
+    const S wt = dot3(w, T);
+    const v3<S> B = bfield<S>(field, P.pos());
+    const v3<S> dTds = scale3(P.F.a[7][0], cross3(T, B));
+    mat<S, 8, 1> dFds = mat<S, 8, 1>::zero();
+    for (int i = 0; i < 3; ++i) {
+        dFds.a[i][0] = T.a[i][0];
+        dFds.a[4 + i][0] = dTds.a[i][0];
+    }
+    dFds.a[3][0] = dt_ds(P.F.a[7][0], mass, abs_q);
+    mat<S, 1, 8> dsdF = mat<S, 1, 8>::zero();
+    for (int i = 0; i < 3; ++i) dsdF.a[0][i] = -w.a[i][0] / wt;
+    const mat<S, 8, 8> corr = mat<S, 8, 8>::identity() + dFds * dsdF;
+
+    // free -> bound projection
+    mat<S, 6, 8> Jfb = mat<S, 6, 8>::zero();
+    const S sin2 = T.a[0][0] * T.a[0][0] + T.a[1][0] * T.a[1][0];
+    const S sinT = std::sqrt(sin2);
+    for (int i = 0; i < 3; ++i) {
+        Jfb.a[0][i] = static_cast<S>(s1.axis_u[i]);
+        Jfb.a[1][i] = static_cast<S>(s1.axis_v[i]);
+    }
+    Jfb.a[2][4] = -T.a[1][0] / sin2;
+    Jfb.a[2][5] = T.a[0][0] / sin2;
+    Jfb.a[3][4] = T.a[0][0] * T.a[2][0] / sinT;
+    Jfb.a[3][5] = T.a[1][0] * T.a[2][0] / sinT;
+    Jfb.a[3][6] = -sinT;
+    Jfb.a[4][7] = S(1);
+    Jfb.a[5][3] = S(1);
+
+    const mat<S, 6, 6> J = Jfb * corr * P.D * bound_to_free_jac<S>(s0, in.vec);
+    out.cov = J * in.cov * transpose(J);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// chi2 p-value: upper regularised incomplete gamma Q(ndf/2, chi2/2), the
+// series / continued-fraction split of the reference's Cephes-derived code
+// (core/include/traccc/utils/prob.hpp:148-288), log-gamma from the C library.
+// ---------------------------------------------------------------------------
+template <typename S>
+S igam_series(S a, S x) {
+    S ax = a * std::log(x) - x - static_cast<S>(std::lgamma(static_cast<double>(a)));
+    if (ax < static_cast<S>(-709.78)) return S(0);
+    ax = std::exp(ax);
+    S r = a, c = S(1), ans = S(1);
+    do {
+        r += S(1);
+        c *= x / r;
+        ans += c;
+    } while (c / ans > static_cast<S>(1.11022302462515654042e-16));
+    return ans * ax / a;
+}
+
+template <typename S>
+S igamc_fraction(S a, S x) {
+    S ax = a * std::log(x) - x - static_cast<S>(std::lgamma(static_cast<double>(a)));
+    if (ax < static_cast<S>(-709.78)) return S(0);
+    ax = std::exp(ax);
+    S y = S(1) - a, z = x + y + S(1), c = S(0);
+    S pkm2 = S(1), qkm2 = x, pkm1 = x + S(1), qkm1 = z * x;
+    S ans = pkm1 / qkm1, t;
+    const S big = static_cast<S>(4.503599627370496e15);
+    const S biginv = static_cast<S>(2.22044604925031308085e-16);
+    int guard = 0;
+    do {
+        c += S(1);
+        y += S(1);
+        z += S(2);
+        const S yc = y * c;
+        const S pk = pkm1 * z - pkm2 * yc;
+        const S qk = qkm1 * z - qkm2 * yc;
+        if (qk != S(0)) {
+            const S r = pk / qk;
+            t = std::fabs((ans - r) / r);
+            ans = r;
+        } else {
+            t = S(1);
+        }
+        pkm2 = pkm1;
+        pkm1 = pk;
+        qkm2 = qkm1;
+        qkm1 = qk;
+        if (std::fabs(pk) > big) {
+            pkm2 *= biginv;
+            pkm1 *= biginv;
+            qkm2 *= biginv;
+            qkm1 *= biginv;
+        }
+    } while (t > static_cast<S>(1.11022302462515654042e-16) && ++guard < 2000);
+    return ans * ax;
+}
+
+template <typename S>
+S prob(S chi2, S ndf) {
+    if (ndf <= S(0)) return S(0);
+    if (chi2 <= S(0)) return chi2 < S(0) ? S(0) : S(1);
+    const S a = S(0.5) * ndf, x = S(0.5) * chi2;
+    if (x < S(1) || x < a) return S(1) - igam_series<S>(a, x);
+    return igamc_fraction<S>(a, x);
+}
+
+// ---------------------------------------------------------------------------
+// Kalman filter pieces (measurement_selector.hpp:53-212, gain_matrix_updater.hpp:69-181)
+// ---------------------------------------------------------------------------
+template <typename S>
+struct meas_view {
+    S pos[2];  // already reordered by the subspace (get_measurement_local)
+    S var[2];
+    uint32_t dims;
+    uint8_t subs[2];
+};
+
+template <typename S>
+meas_view<S> load_meas(const ckf_measurements& m, uint32_t i) {
+    meas_view<S> v;
+    v.dims = m.dimensions[i];
+    v.subs[0] = m.subspace[2 * i];
+    v.subs[1] = m.subspace[2 * i + 1];
+    const S a = static_cast<S>(m.local_position[2 * i]), b = static_cast<S>(m.local_position[2 * i + 1]);
+    const S va = static_cast<S>(m.local_variance[2 * i]), vb = static_cast<S>(m.local_variance[2 * i + 1]);
+    // measurement_helpers.ipp: subspace[0] == loc1 swaps the two coordinates
+    if (v.subs[0] == 1u) {
+        v.pos[0] = b;
+        v.pos[1] = a;
+        v.var[0] = vb;
+        v.var[1] = va;
+    } else {
+        v.pos[0] = a;
+        v.pos[1] = b;
+        v.var[0] = va;
+        v.var[1] = vb;
+    }
+    if (v.dims == 1u) v.var[1] = static_cast<S>(std::numeric_limits<float>::max());
+    return v;
+}
+
+template <typename S>
+mat<S, 2, 6> projector(const meas_view<S>& m) {
+    mat<S, 2, 6> H = mat<S, 2, 6>::zero();
+    H.a[0][m.subs[0]] = S(1);
+    if (m.dims == 2u) H.a[1][m.subs[1]] = S(1);
+    return H;
+}
+
+template <typename S>
+mat<S, 2, 2> masked_inv(const mat<S, 2, 2>& M, uint32_t dims) {
+    mat<S, 2, 2> R = mat<S, 2, 2>::zero();
+    if (dims == 1u) {
+        R.a[0][0] = S(1) / M.a[0][0];
+        return R;
+    }
+    const S det = M.a[0][0] * M.a[1][1] - M.a[0][1] * M.a[1][0];
+    R.a[0][0] = M.a[1][1] / det;
+    R.a[1][1] = M.a[0][0] / det;
+    R.a[0][1] = -M.a[0][1] / det;
+    R.a[1][0] = -M.a[1][0] / det;
+    return R;
+}
+
+template <typename S>
+S predicted_chi2(const meas_view<S>& m, const bound_state<S>& p) {
+    const mat<S, 2, 6> H = projector(m);
+    mat<S, 2, 2> V = mat<S, 2, 2>::zero();
+    V.a[0][0] = m.var[0];
+    V.a[1][1] = m.var[1];
+    const mat<S, 2, 2> R = H * (p.cov * transpose(H)) + V;
+    const mat<S, 2, 2> Ri = masked_inv(R, m.dims);
+    mat<S, 2, 1> ml;
+    ml.a[0][0] = m.pos[0];
+    ml.a[1][0] = m.dims == 2u ? m.pos[1] : S(0);
+    const mat<S, 2, 1> res = ml - H * p.vec;
+    return (transpose(res) * Ri * res).a[0][0];
+}
+
+template <typename S>
+bool wrap_angles(mat<S, 6, 1>& v) {
+    const S twopi = static_cast<S>(2.0 * kPi), pi = static_cast<S>(kPi);
+    const S th_in = v.a[3][0];
+    if (std::fmod(th_in, twopi) == S(0) || th_in == S(0)) return false;
+    S phi = v.a[2][0];
+    S th = th_in;
+    // wrap_periodic(theta, 0, 2pi)
+    if (!(th >= S(0) && th < twopi)) th = th - twopi * std::floor(th / twopi);
+    if (pi < th) {
+        phi += pi;
+        th = twopi - th;
+    }
+    const S d = phi + pi;
+    if (!(d >= S(0) && d < twopi)) phi = phi - twopi * std::floor(d / twopi);
+    if (th <= S(0) || th >= twopi) return false;
+    v.a[2][0] = phi;
+    v.a[3][0] = th;
+    return true;
+}
+
+template <typename S>
+int gain_update(const meas_view<S>& m, const bound_state<S>& pred, bound_state<S>& filt) {
+    const mat<S, 2, 6> H = projector(m);
+    mat<S, 2, 2> V = mat<S, 2, 2>::zero();
+    V.a[0][0] = m.var[0];
+    V.a[1][1] = m.var[1];
+    const mat<S, 6, 2> PHt = pred.cov * transpose(H);
+    const mat<S, 2, 2> Minv = masked_inv(H * PHt + V, m.dims);
+    const mat<S, 6, 2> K = PHt * Minv;
+    mat<S, 2, 1> ml;
+    ml.a[0][0] = m.pos[0];
+    ml.a[1][0] = m.dims == 2u ? m.pos[1] : S(0);
+    mat<S, 6, 1> x = pred.vec + K * (ml - H * pred.vec);
+    if (!std::isfinite(x.a[3][0]) || !std::isfinite(x.a[2][0])) return CKF_FIT_ERROR_INVERSION;
+    if (std::fabs(x.a[4][0]) == S(0)) return CKF_FIT_ERROR_QOP_ZERO;
+    if (!wrap_angles(x)) return CKF_FIT_ERROR_THETA_POLE;
+    const mat<S, 6, 6> IKH = mat<S, 6, 6>::identity() - K * H;
+    mat<S, 6, 6> C = IKH * pred.cov * transpose(IKH) + K * V * transpose(K);
+    const S min_var = S(-0.01);
+    for (int i = 0; i < 6; ++i)
+        if (C.a[i][i] < min_var) return CKF_FIT_ERROR_UPDATER_INVALID_COVARIANCE;
+    for (int i = 0; i < 6; ++i) C.a[i][i] = std::fabs(C.a[i][i]);
+    filt.surface = pred.surface;
+    filt.vec = x;
+    filt.cov = C;
+    return CKF_FIT_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// The CKF (host semantics)
+// ---------------------------------------------------------------------------
+struct link {
+    uint32_t step;
+    uint32_t previous_candidate_idx;
+    uint32_t meas_idx;
+    uint32_t seed_idx;
+    uint32_t n_skipped;
+    uint32_t n_consecutive_skipped;
+    double chi2;
+    double chi2_sum;
+    uint32_t ndf_sum;
+};
+
+struct track_out {
+    uint32_t seed;
+    double ndf, chi2, pval;
+    uint32_t nholes;
+    std::vector<uint32_t> meas;
+};
+
+struct run_stats {
+    uint32_t steps = 0;
+    uint64_t links = 0, props = 0, tips = 0;
+};
+
+// surface identifier -> index
+inline uint32_t surface_index(const ckf_detector& det, uint64_t id) {
+    uint32_t lo = 0, hi = det.n_surfaces;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (det.surfaces[mid].identifier < id) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename S>
+std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& det,
+                           const ckf_bfield& field, const ckf_measurements& meas,
+                           const ckf_bound_params& seeds, run_stats& stats) {
+    const uint32_t n_meas = meas.size;
+    const S mass = static_cast<S>(cfg.ptc_mass);
+    const S abs_q = static_cast<S>(std::fabs(cfg.ptc_charge));
+    const S chi2_max = static_cast<S>(cfg.chi2_max);
+
+    // measurement ranges per surface (upper bounds), reference :97-120
+    std::vector<uint32_t> meas_ranges(det.n_surfaces);
+    for (uint32_t s = 0; s < det.n_surfaces; ++s) {
+        if (!det.surfaces[s].sensitive) {
+            meas_ranges[s] = s == 0 ? 0u : meas_ranges[s - 1];
+            continue;
+        }
+        const uint64_t id = det.surfaces[s].identifier;
+        meas_ranges[s] = static_cast<uint32_t>(
+            std::upper_bound(meas.surface_link, meas.surface_link + n_meas, id) - meas.surface_link);
+    }
+
+    const uint32_t n_steps_max = cfg.max_track_candidates_per_track;
+    std::vector<std::vector<link>> links(n_steps_max);
+    std::vector<std::vector<size_t>> param_to_link(n_steps_max);
+    std::vector<std::pair<uint32_t, uint32_t>> tips;
+
+    std::vector<bound_state<S>> in_params(seeds.size);
+    for (uint32_t i = 0; i < seeds.size; ++i) {
+        in_params[i].surface = surface_index(det, seeds.surface_link[i]);
+        for (int k = 0; k < 6; ++k) in_params[i].vec.a[k][0] = static_cast<S>(seeds.vector[6 * i + k]);
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) in_params[i].cov.a[r][c] = static_cast<S>(seeds.covariance[36 * i + 6 * r + c]);
+    }
+    std::vector<uint32_t> n_trks_per_seed(seeds.size);
+    std::vector<bound_state<S>> out_params;
+    // the link a link continues (param_to_link maps an input index to its link)
+    auto back = [&](const link& L) {
+        const size_t pos = param_to_link[L.step - 1][L.previous_candidate_idx];
+        return links[L.step - 1][pos];
+    };
+
+    for (uint32_t step = 0; step < n_steps_max; ++step) {
+        const size_t n_in = in_params.size();
+        if (n_in == 0) break;
+        stats.steps = step + 1;
+        std::fill(n_trks_per_seed.begin(), n_trks_per_seed.end(), 0u);
+        std::vector<bound_state<S>> updated;
+
+        for (uint32_t ip = 0; ip < n_in; ++ip) {
+            const bound_state<S>& in = in_params[ip];
+            const link* prev = step == 0 ? nullptr : &links[step - 1][param_to_link[step - 1][ip]];
+            const uint32_t orig = prev ? prev->seed_idx : ip;
+            const uint32_t skip = prev ? prev->n_skipped : 0u;
+            const uint32_t consec = prev ? prev->n_consecutive_skipped : 0u;
+            const double chi2_prev = prev ? prev->chi2_sum : 0.0;
+            const uint32_t ndf_prev = prev ? prev->ndf_sum : 0u;
+
+            const ckf_surface& sf = det.surfaces[in.surface];
+            const uint32_t lo = in.surface == 0 ? 0u : meas_ranges[in.surface - 1];
+            const uint32_t up = meas_ranges[in.surface];
+
+            struct cand {
+                link l;
+                bound_state<S> p;
+                S chi2;
+            };
+            std::vector<cand> best;
+            for (uint32_t mi = lo; mi < up; ++mi) {
+                const meas_view<S> mv = load_meas<S>(meas, mi);
+                const S chi2 = predicted_chi2<S>(mv, in);
+                if (chi2 > chi2_max || chi2 < S(0)) continue;
+                bound_state<S> filt;
+                int res = CKF_FIT_ERROR_OTHER;
+                if (step == 0 && !(sf.thickness_x0 > 0.f)) {
+                    if (chi2 == S(0)) {
+                        res = CKF_FIT_SUCCESS;
+                        filt = in;
+                        filt.cov.a[0][0] = mv.var[0];
+                        filt.cov.a[1][1] = mv.var[1];
+                    }
+                } else {
+                    res = gain_update<S>(mv, in, filt);
+                }
+                if (res == CKF_FIT_SUCCESS) {
+                    link l{step, ip, mi, orig, skip, 0u, static_cast<double>(chi2),
+                           chi2_prev + static_cast<double>(chi2), ndf_prev + mv.dims};
+                    if (!std::is_same<S, double>::value) {
+                        // FP32 build accumulates the chi2 sum in FP32, as traccc's scalar does
+                        l.chi2_sum = static_cast<double>(static_cast<S>(chi2_prev) + chi2);
+                    }
+                    best.push_back({l, filt, chi2});
+                }
+            }
+            std::sort(best.begin(), best.end(), [](const cand& a, const cand& b) {
+                return a.chi2 < b.chi2 || (a.chi2 == b.chi2 && a.l.meas_idx < b.l.meas_idx);
+            });
+            const uint32_t nb = std::min<uint32_t>(cfg.max_num_branches_per_surface,
+                                                   static_cast<uint32_t>(best.size()));
+            for (uint32_t i = 0; i < nb; ++i) {
+                links[step].push_back(best[i].l);
+                updated.push_back(best[i].p);
+            }
+            if (nb == 0) {
+                links[step].push_back({step, ip, std::numeric_limits<uint32_t>::max(), orig, skip + 1,
+                                       consec + 1, static_cast<double>(std::numeric_limits<float>::max()),
+                                       chi2_prev, ndf_prev});
+                updated.push_back(in);
+            }
+        }
+
+        // deduplication, reference :371-488
+        const size_t n_links = links[step].size();
+        stats.links += n_links;
+        std::vector<uint8_t> alive(n_links, 1u);
+        if (step >= cfg.duplicate_removal_minimum_length) {
+            std::map<uint32_t, std::vector<size_t>> groups;
+            for (size_t i = 0; i < n_links; ++i) {
+                link L = links[step][i];
+                while (L.meas_idx >= n_meas && L.step != 0u) L = back(L);
+                groups[L.meas_idx].push_back(i);
+            }
+            for (const auto& g : groups) {
+                const std::vector<size_t>& tr = g.second;
+                for (size_t i = 0; i < tr.size(); ++i) {
+                    const link& base = links[step][tr[i]];
+                    // ndf_sum - 5 in unsigned arithmetic, as the reference writes it
+                    const S p_this = prob<S>(static_cast<S>(base.chi2_sum),
+                                             static_cast<S>(static_cast<uint32_t>(base.ndf_sum - 5u)));
+                    if (step + 1 - base.n_skipped <= cfg.duplicate_removal_minimum_length ||
+                        base.ndf_sum <= 5u)
+                        continue;
+                    for (size_t j = 0; j < tr.size(); ++j) {
+                        if (i == j) continue;
+                        link a = base;
+                        link b = links[step][tr[j]];
+                        if (step + 1 - b.n_skipped <= cfg.duplicate_removal_minimum_length ||
+                            base.ndf_sum <= 5u)
+                            continue;
+                        bool dominated = true;
+                        const S p_that = prob<S>(static_cast<S>(b.chi2_sum),
+                                                 static_cast<S>(static_cast<uint32_t>(b.ndf_sum - 5u)));
+                        while (true) {
+                            while (a.meas_idx >= n_meas && a.step != 0u) a = back(a);
+                            while (b.meas_idx >= n_meas && b.step != 0u) b = back(b);
+                            if (a.meas_idx == b.meas_idx) {
+                                if (a.step == 0u) break;
+                                if (b.step == 0u) {
+                                    dominated = false;
+                                    break;
+                                }
+                                a = back(a);
+                                b = back(b);
+                            } else {
+                                dominated = false;
+                                break;
+                            }
+                        }
+                        if (p_this != p_that) dominated = dominated && (p_that >= p_this);
+                        else dominated = dominated && (tr[j] < tr[i]);
+                        if (dominated) {
+                            alive[tr[i]] = 0u;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+
+        // propagation, reference :493-627
+        for (uint32_t li = 0; li < n_links; ++li) {
+            if (!alive[li]) continue;
+            const link& L = links[step][li];
+            if (++n_trks_per_seed[L.seed_idx] > cfg.max_num_branches_per_seed) continue;
+            if (L.n_skipped > cfg.max_num_skipping_per_cand) {
+                tips.push_back({step, li});
+                continue;
+            }
+            if (L.n_consecutive_skipped > cfg.max_num_consecutive_skipped) {
+                tips.push_back({step, li});
+                continue;
+            }
+            bound_state<S> p = updated[li];
+            // hypothesis charge follows the track's charge sign
+            const S q_abs = abs_q;
+            if (step == 0) add_material<S>(det.surfaces[p.surface], p, mass, q_abs);
+            bound_state<S> o;
+            ++stats.props;
+            const int r = propagate_bound<S>(cfg, det, field, p, mass, q_abs, o, nullptr, nullptr);
+            bool success = (r == 0);
+            bool valid = success;
+            if (success) {
+                add_material<S>(det.surfaces[o.surface], o, mass, q_abs);
+                const S th = o.vec.a[3][0];
+                if (th <= S(0) || th >= static_cast<S>(2.0 * kPi)) valid = false;
+                if (!std::isfinite(o.vec.a[2][0])) valid = false;
+                if (std::fabs(o.vec.a[4][0]) == S(0)) valid = false;
+                if (valid) {
+                    out_params.push_back(o);
+                    param_to_link[step].push_back(li);
+                }
+            }
+            if (!valid && step >= cfg.min_track_candidates_per_track - 1u) tips.push_back({step, li});
+            if (success && step == n_steps_max - 1u) tips.push_back({step, li});
+        }
+        in_params = std::move(out_params);
+        out_params.clear();
+    }
+    stats.tips = tips.size();
+
+    // build tracks, reference :637-715
+    std::vector<track_out> result;
+    result.reserve(tips.size());
+    for (const auto& tip : tips) {
+        link L = links[tip.first][tip.second];
+        const uint32_t n_cands = tip.first + 1 - L.n_skipped;
+        if (n_cands < cfg.min_track_candidates_per_track || n_cands > cfg.max_track_candidates_per_track)
+            continue;
+        std::vector<uint32_t> ids(n_cands);
+        S ndf = S(0), chi2 = S(0);
+        bool complete = false;
+        for (int k = static_cast<int>(n_cands) - 1; k >= 0; --k) {
+            while (L.meas_idx >= n_meas && L.step != 0u) L = back(L);
+            if (L.meas_idx >= n_meas) break;
+            ids[k] = L.meas_idx;
+            ndf += static_cast<S>(meas.dimensions[L.meas_idx]);
+            chi2 += static_cast<S>(L.chi2);
+            if (k == 0) {
+                complete = true;
+                break;
+            }
+            L = back(L);
+        }
+        if (!complete) continue;
+        ndf -= S(5);
+        track_out t;
+        t.seed = L.seed_idx;
+        t.ndf = static_cast<double>(ndf);
+        t.chi2 = static_cast<double>(chi2);
+        t.pval = static_cast<double>(prob<S>(chi2, ndf));
+        t.nholes = L.n_skipped;
+        t.meas = std::move(ids);
+        result.push_back(std::move(t));
+    }
+    return result;
+}
+
+template <typename T>
+T* alloc_n(size_t n) {
+    return static_cast<T*>(std::calloc(n ? n : 1, sizeof(T)));
+}
+
+void fill_out(const std::vector<track_out>& tr, const ckf_bound_params& seeds, const run_stats& st,
+              oracle_tracks* out) {
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t n = static_cast<uint32_t>(tr.size());
+    uint32_t nc = 0;
+    for (const auto& t : tr) nc += static_cast<uint32_t>(t.meas.size());
+    out->n_tracks = n;
+    out->n_constituents = nc;
+    out->params_surface_link = alloc_n<uint64_t>(n);
+    out->params_vector = alloc_n<float>(6u * n);
+    out->params_covariance = alloc_n<float>(36u * n);
+    out->ndf = alloc_n<double>(n);
+    out->chi2 = alloc_n<double>(n);
+    out->pval = alloc_n<double>(n);
+    out->nholes = alloc_n<uint32_t>(n);
+    out->constituent_offsets = alloc_n<uint32_t>(n + 1u);
+    out->constituent_index = alloc_n<uint32_t>(nc);
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const track_out& t = tr[i];
+        out->params_surface_link[i] = seeds.surface_link[t.seed];
+        std::memcpy(out->params_vector + 6u * i, seeds.vector + 6u * t.seed, 6u * sizeof(float));
+        std::memcpy(out->params_covariance + 36u * i, seeds.covariance + 36u * t.seed, 36u * sizeof(float));
+        out->ndf[i] = t.ndf;
+        out->chi2[i] = t.chi2;
+        out->pval[i] = t.pval;
+        out->nholes[i] = t.nholes;
+        out->constituent_offsets[i] = off;
+        for (uint32_t m : t.meas) out->constituent_index[off++] = m;
+    }
+    out->constituent_offsets[n] = off;
+    out->n_steps = st.steps;
+    out->n_links = st.links;
+    out->n_propagations = st.props;
+    out->n_tips = st.tips;
+}
+
+}  // namespace oracle
+
+extern "C" {
+
+int oracle_ckf_run(const ckf_finding_config* cfg, const ckf_detector* det, const ckf_bfield* field,
+                   const ckf_measurements* meas, const ckf_bound_params* seeds, int precision,
+                   oracle_tracks* out) {
+    if (!cfg || !det || !field || !meas || !seeds || !out) return CKF_ERR_INVALID_ARGUMENT;
+    if (cfg->min_track_candidates_per_track == 0u) return CKF_ERR_INVALID_ARGUMENT;
+    oracle::run_stats st;
+    std::vector<oracle::track_out> tr =
+        precision == 32 ? oracle::ckf<float>(*cfg, *det, *field, *meas, *seeds, st)
+                        : oracle::ckf<double>(*cfg, *det, *field, *meas, *seeds, st);
+    oracle::fill_out(tr, *seeds, st, out);
+    return CKF_OK;
+}
+
+void oracle_tracks_free(oracle_tracks* t) {
+    if (!t) return;
+    std::free(t->params_surface_link);
+    std::free(t->params_vector);
+    std::free(t->params_covariance);
+    std::free(t->ndf);
+    std::free(t->chi2);
+    std::free(t->pval);
+    std::free(t->nholes);
+    std::free(t->constituent_offsets);
+    std::free(t->constituent_index);
+    std::memset(t, 0, sizeof(*t));
+}
+
+int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,
+                        const ckf_bfield* field, const ckf_measurements* meas,
+                        const ckf_bound_params* seeds, uint32_t n_events, int precision,
+                        uint32_t n_threads, uint64_t* n_tracks_total, double* seconds) {
+    if (!cfg || !det || !field || !meas || !seeds || n_threads == 0u) return CKF_ERR_INVALID_ARGUMENT;
+    std::atomic<uint32_t> next{0u};
+    std::atomic<uint64_t> total{0u};
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < n_threads; ++t) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                const uint32_t e = next.fetch_add(1u);
+                if (e >= n_events) break;
+                oracle::run_stats st;
+                const size_t n = precision == 32
+                                     ? oracle::ckf<float>(*cfg, *det, *field, meas[e], seeds[e], st).size()
+                                     : oracle::ckf<double>(*cfg, *det, *field, meas[e], seeds[e], st).size();
+                total.fetch_add(n);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (n_tracks_total) *n_tracks_total = total.load();
+    return CKF_OK;
+}
+
+double oracle_prob(double chi2, double ndf) { return oracle::prob<double>(chi2, ndf); }
+float oracle_prob_f(float chi2, float ndf) { return oracle::prob<float>(chi2, ndf); }
+
+int oracle_kf_update(const double vec[6], const double cov[36], const float m[2], const float var[2],
+                     uint32_t dims, const uint8_t subspace[2], double* chi2, double out_vec[6],
+                     double out_cov[36]) {
+    using namespace oracle;
+    bound_state<double> in, f;
+    in.surface = 0;
+    for (int i = 0; i < 6; ++i) in.vec.a[i][0] = vec[i];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) in.cov.a[r][c] = cov[6 * r + c];
+    ckf_measurements M{};
+    uint32_t d = dims;
+    M.size = 1;
+    M.local_position = m;
+    M.local_variance = var;
+    M.dimensions = &d;
+    M.subspace = subspace;
+    const meas_view<double> mv = load_meas<double>(M, 0);
+    *chi2 = predicted_chi2<double>(mv, in);
+    const int r = gain_update<double>(mv, in, f);
+    if (r == CKF_FIT_SUCCESS) {
+        for (int i = 0; i < 6; ++i) out_vec[i] = f.vec.a[i][0];
+        for (int rr = 0; rr < 6; ++rr)
+            for (int c = 0; c < 6; ++c) out_cov[6 * rr + c] = f.cov.a[rr][c];
+    }
+    return r;
+}
+
+int oracle_propagate(const ckf_finding_config* cfg, const ckf_detector* det, const ckf_bfield* field,
+                     uint32_t start, const double vec[6], const double cov[36], int apply_start_material,
+                     uint32_t* end_surface, double out_vec[6], double out_cov[36], double* path,
+                     uint32_t* n_steps) {
+    using namespace oracle;
+    bound_state<double> in, o;
+    in.surface = start;
+    for (int i = 0; i < 6; ++i) in.vec.a[i][0] = vec[i];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) in.cov.a[r][c] = cov[6 * r + c];
+    const double mass = cfg->ptc_mass, aq = std::fabs(cfg->ptc_charge);
+    if (apply_start_material) add_material<double>(det->surfaces[start], in, mass, aq);
+    const int r = propagate_bound<double>(*cfg, *det, *field, in, mass, aq, o, path, n_steps);
+    if (r == 0) {
+        add_material<double>(det->surfaces[o.surface], o, mass, aq);
+        *end_surface = o.surface;
+        for (int i = 0; i < 6; ++i) out_vec[i] = o.vec.a[i][0];
+        for (int rr = 0; rr < 6; ++rr)
+            for (int c = 0; c < 6; ++c) out_cov[6 * rr + c] = o.cov.a[rr][c];
+    }
+    return r;
+}
+
+void oracle_rk_step(const ckf_bfield* field, double mass, double abs_q, double h, const double free_in[8],
+                    double free_out[8], double D[64]) {
+    using namespace oracle;
+    mat<double, 8, 1> F;
+    for (int i = 0; i < 8; ++i) F.a[i][0] = free_in[i];
+    mat<double, 8, 8> M = mat<double, 8, 8>::identity();
+    rk_step<double>(*field, mass, abs_q, h, F, &M);
+    for (int i = 0; i < 8; ++i) free_out[i] = F.a[i][0];
+    for (int r = 0; r < 8; ++r)
+        for (int c = 0; c < 8; ++c) D[8 * r + c] = M.a[r][c];
+}
+
+}  // extern "C"

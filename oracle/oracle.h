@@ -1,0 +1,84 @@
+/*
+ * CPU oracle for the CKF track-finding path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library, and only as the checker / the timed CPU baseline.  The
+ * product (traccc_amd) never links, loads or calls it.
+ *
+ * It is this project's restatement of traccc's host CKF
+ * (core/include/traccc/finding/details/combinatorial_kalman_filter.hpp:59-718),
+ * templated on the scalar: the FP64 instantiation is the correctness oracle,
+ * the FP32 instantiation is the "reference CPU CKF" timed beside the GPU.
+ * It shares with the product only the plain C data layouts of ckf.h.
+ */
+#ifndef TRACCC_AMD_ORACLE_H
+#define TRACCC_AMD_ORACLE_H
+
+#include <stdint.h>
+
+#include "traccc_amd/ckf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_tracks {
+    uint32_t n_tracks;
+    uint32_t n_constituents;
+    uint64_t* params_surface_link;
+    float* params_vector;      /* [n][6]  */
+    float* params_covariance;  /* [n][36] */
+    double* ndf;
+    double* chi2;
+    double* pval;
+    uint32_t* nholes;
+    uint32_t* constituent_offsets; /* [n + 1] */
+    uint32_t* constituent_index;
+    /* statistics */
+    uint32_t n_steps;
+    uint64_t n_links;
+    uint64_t n_propagations;
+    uint64_t n_tips;
+} oracle_tracks;
+
+/* precision: 64 (oracle) or 32 (timed CPU baseline). */
+int oracle_ckf_run(const ckf_finding_config* cfg, const ckf_detector* det, const ckf_bfield* field,
+                   const ckf_measurements* meas, const ckf_bound_params* seeds, int precision,
+                   oracle_tracks* out);
+void oracle_tracks_free(oracle_tracks* t);
+
+/* Events spread over n_threads host threads, one CKF instance per thread
+ * (traccc's throughput_mt pattern).  Returns wall seconds in *seconds. */
+int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,
+                        const ckf_bfield* field, const ckf_measurements* meas,
+                        const ckf_bound_params* seeds, uint32_t n_events, int precision,
+                        uint32_t n_threads, uint64_t* n_tracks_total, double* seconds);
+
+/* ---- unit-level entry points (FP64) ---- */
+double oracle_prob(double chi2, double ndf);
+float oracle_prob_f(float chi2, float ndf);
+
+/* Predicted chi2 and gain-matrix update of one bound state with one
+ * measurement.  Returns the ckf_fitter_status; out_vec/out_cov filled on
+ * success. */
+int oracle_kf_update(const double vec[6], const double cov[36], const float meas[2],
+                     const float var[2], uint32_t dims, const uint8_t subspace[2],
+                     double* chi2, double out_vec[6], double out_cov[36]);
+
+/* Propagate a bound state from surface `start` (index) to the next sensitive
+ * surface.  Returns 0 found, 1 exited the detector, 2 aborted. */
+int oracle_propagate(const ckf_finding_config* cfg, const ckf_detector* det,
+                     const ckf_bfield* field, uint32_t start, const double vec[6],
+                     const double cov[36], int apply_start_material, uint32_t* end_surface,
+                     double out_vec[6], double out_cov[36], double* path, uint32_t* n_steps);
+
+/* One RK4 step of a free state [x, y, z, t, Tx, Ty, Tz, q/p] with the
+ * transport matrix d(free_out)/d(free_in); for tests. */
+void oracle_rk_step(const ckf_bfield* field, double mass, double abs_q, double h,
+                    const double free_in[8], double free_out[8], double D[64]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

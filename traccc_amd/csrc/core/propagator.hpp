@@ -1,0 +1,533 @@
+// Track propagation to the next sensitive surface: RK4-Nystroem stepping with
+// covariance (Jacobian) transport, and a layer-window navigator.
+//
+// What this replaces: the CKF's detray propagator
+//   rk_stepper + caching_navigator + {pathlimit_aborter, parameter_updater,
+//   momentum_aborter, ckf_aborter}
+// (core/include/traccc/finding/details/combinatorial_kalman_filter_types.hpp:27-52,
+//  core/include/traccc/finding/actors/ckf_aborter.hpp:37-68), called from
+// core/include/traccc/finding/details/combinatorial_kalman_filter.hpp:528-627.
+//
+// Methods (published, restated here in this project's own code):
+//  * Runge-Kutta-Nystroem 4th order for r'' = (q/p) r' x B
+//    (Abramowitz & Stegun 25.5.20; Lund et al., JINST 4 (2009) P04001);
+//  * semi-analytic transport of d(free)/d(bound) through the RK stages
+//    (Bugge & Myrheim, NIM 179 (1981) 365), field gradients neglected;
+//  * path-length correction onto the destination plane, then the
+//    free-to-bound projection.
+//
+// Step size: a-priori accuracy control.  For a helix of curvature kappa the
+// RK4 local error is about h^5 kappa^4 / 120, so the step is
+//   h = (120 * rk_error_tol)^(1/5) * kappa^(-4/5), capped by step_constraint.
+// It is a smooth function of the state, so an FP32 and an FP64 evaluation of
+// the same track take the same sequence of steps.
+#pragma once
+
+#include <cmath>
+
+#include "defs.hpp"
+#include "field.hpp"
+#include "geometry.hpp"
+
+namespace tamd {
+
+template <typename T>
+struct free_state {
+    vec3<T> pos;
+    vec3<T> dir;
+    T qop;
+    T time;
+};
+
+// d(free)/d(bound at the start surface); bound columns: loc0, loc1, phi,
+// theta, q/p (the time column only feeds the time row and is implicit).
+template <typename T>
+struct transport_jac {
+    T p[3][5];
+    T d[3][5];
+    T t_qop;
+};
+
+template <typename T>
+struct nav_config {
+    T min_step_length;   // ckf_aborter: path needed before a surface counts
+    uint32_t max_steps;  // ckf_aborter: RK steps allowed per propagation
+    T on_surface_tol;
+    T overstep_tol;
+    T mask_tol_min, mask_tol_max, mask_tol_scalor;
+    T step_constraint;
+    T rk_step_scale;  // (120 * rk_error_tol)^(1/5)
+    T path_limit;
+    T approach_dist;
+    int win_phi, win_long;
+    T mass;   // particle hypothesis mass, GeV
+    T abs_q;  // |charge| of the hypothesis
+    T min_p, min_pT;
+};
+
+template <typename T>
+inline nav_config<T> make_nav_config(const ckf_finding_config& c) {
+    nav_config<T> n{};
+    n.min_step_length = static_cast<T>(c.min_step_length_for_next_surface);
+    n.max_steps = c.max_step_counts_for_next_surface;
+    n.on_surface_tol = static_cast<T>(c.propagation.on_surface_tolerance);
+    n.overstep_tol = static_cast<T>(c.propagation.overstep_tolerance);
+    n.mask_tol_min = static_cast<T>(c.propagation.min_mask_tolerance);
+    n.mask_tol_max = static_cast<T>(c.propagation.max_mask_tolerance);
+    n.mask_tol_scalor = static_cast<T>(c.propagation.mask_tolerance_scalor);
+    n.step_constraint = static_cast<T>(c.propagation.step_constraint);
+    n.rk_step_scale = static_cast<T>(
+        std::pow(120.0 * static_cast<double>(c.propagation.rk_error_tol), 0.2));
+    n.path_limit = static_cast<T>(c.propagation.path_limit);
+    n.approach_dist = static_cast<T>(25.0);
+    n.win_phi = static_cast<int>(c.propagation.search_window[0]);
+    n.win_long = static_cast<int>(c.propagation.search_window[1]);
+    n.mass = static_cast<T>(c.ptc_mass);
+    n.abs_q = static_cast<T>(std::fabs(c.ptc_charge));
+    n.min_p = static_cast<T>(c.min_p);
+    n.min_pT = static_cast<T>(c.min_pT);
+    return n;
+}
+
+// dt/ds = 1 / (beta c) for the hypothesis mass.
+template <typename T>
+TAMD_HD T dtds_of(T qop, T mass, T abs_q) {
+    const T mq = mass * qop / abs_q;
+    return std::sqrt(T(1) + mq * mq) / static_cast<T>(kSpeedOfLight);
+}
+
+// One RK4-Nystroem step of length h (may be negative).
+template <typename T, bool kJac>
+TAMD_HD void rk4_step(free_state<T>& s, transport_jac<T>& J,
+                      const field_view<T>& f, T h, T mass, T abs_q) {
+    const T q = s.qop;
+    const T h2 = h * T(0.5);
+    const vec3<T> B1 = field_at(f, s.pos);
+    const vec3<T> k1 = q * cross(s.dir, B1);
+    const vec3<T> p2 = s.pos + h2 * s.dir + (h * h * T(0.125)) * k1;
+    const vec3<T> B2 = field_at(f, p2);
+    const vec3<T> T2 = s.dir + h2 * k1;
+    const vec3<T> k2 = q * cross(T2, B2);
+    const vec3<T> T3 = s.dir + h2 * k2;
+    const vec3<T> k3 = q * cross(T3, B2);
+    const vec3<T> p4 = s.pos + h * s.dir + (h * h * T(0.5)) * k3;
+    const vec3<T> B4 = field_at(f, p4);
+    const vec3<T> T4 = s.dir + h * k3;
+    const vec3<T> k4 = q * cross(T4, B4);
+
+    const T h6 = h / T(6);
+    const T hh6 = h * h / T(6);
+
+    if constexpr (kJac) {
+        // Columns of dk_i/dT0 (3x3) and dk_i/dq.
+        T A1[3][3], A2[3][3], A3[3][3], A4[3][3];
+        for (int c = 0; c < 3; ++c) {
+            const vec3<T> e{c == 0 ? T(1) : T(0), c == 1 ? T(1) : T(0),
+                            c == 2 ? T(1) : T(0)};
+            const vec3<T> a1 = q * cross(e, B1);
+            const vec3<T> a2 = q * cross(e + h2 * a1, B2);
+            const vec3<T> a3 = q * cross(e + h2 * a2, B2);
+            const vec3<T> a4 = q * cross(e + h * a3, B4);
+            A1[0][c] = a1.x; A1[1][c] = a1.y; A1[2][c] = a1.z;
+            A2[0][c] = a2.x; A2[1][c] = a2.y; A2[2][c] = a2.z;
+            A3[0][c] = a3.x; A3[1][c] = a3.y; A3[2][c] = a3.z;
+            A4[0][c] = a4.x; A4[1][c] = a4.y; A4[2][c] = a4.z;
+        }
+        const vec3<T> d1 = cross(s.dir, B1);
+        const vec3<T> d2 = cross(T2, B2) + q * cross(h2 * d1, B2);
+        const vec3<T> d3 = cross(T3, B2) + q * cross(h2 * d2, B2);
+        const vec3<T> d4 = cross(T4, B4) + q * cross(h * d3, B4);
+        const T drl[3] = {hh6 * (d1.x + d2.x + d3.x), hh6 * (d1.y + d2.y + d3.y),
+                          hh6 * (d1.z + d2.z + d3.z)};
+        const T dtl[3] = {h6 * (d1.x + T(2) * d2.x + T(2) * d3.x + d4.x),
+                          h6 * (d1.y + T(2) * d2.y + T(2) * d3.y + d4.y),
+                          h6 * (d1.z + T(2) * d2.z + T(2) * d3.z + d4.z)};
+        T DrT[3][3], DTT[3][3];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                const T id = (r == c) ? T(1) : T(0);
+                DrT[r][c] = h * id + hh6 * (A1[r][c] + A2[r][c] + A3[r][c]);
+                DTT[r][c] = id + h6 * (A1[r][c] + T(2) * A2[r][c] +
+                                       T(2) * A3[r][c] + A4[r][c]);
+            }
+        }
+        for (int j = 0; j < 5; ++j) {
+            const T d0 = J.d[0][j], dd1 = J.d[1][j], dd2 = J.d[2][j];
+            for (int r = 0; r < 3; ++r) {
+                J.p[r][j] += DrT[r][0] * d0 + DrT[r][1] * dd1 + DrT[r][2] * dd2;
+            }
+            T nd[3];
+            for (int r = 0; r < 3; ++r) {
+                nd[r] = DTT[r][0] * d0 + DTT[r][1] * dd1 + DTT[r][2] * dd2;
+            }
+            J.d[0][j] = nd[0];
+            J.d[1][j] = nd[1];
+            J.d[2][j] = nd[2];
+        }
+        for (int r = 0; r < 3; ++r) {
+            J.p[r][4] += drl[r];
+            J.d[r][4] += dtl[r];
+        }
+        const T dt = dtds_of(q, mass, abs_q);
+        const T mq = mass / abs_q;
+        J.t_qop += h * (mq * mq * q) /
+                   (static_cast<T>(kSpeedOfLight) * static_cast<T>(kSpeedOfLight) * dt);
+    }
+
+    s.pos = s.pos + h * s.dir + hh6 * (k1 + k2 + k3);
+    vec3<T> nd = s.dir + h6 * (k1 + T(2) * k2 + T(2) * k3 + k4);
+    const T inv = T(1) / std::sqrt(dot(nd, nd));
+    s.dir = inv * nd;
+    s.time += h * dtds_of(q, mass, abs_q);
+}
+
+// Bound parameters on surface sf -> free state; Jacobian initialised to
+// d(free)/d(bound).
+template <typename T, bool kJac>
+TAMD_HD void bound_to_free(const ckf_surface& sf, const T* v, free_state<T>& s,
+                           transport_jac<T>& J) {
+    const vec3<T> c = sf_center<T>(sf);
+    const vec3<T> u = sf_u<T>(sf);
+    const vec3<T> w = sf_v<T>(sf);
+    s.pos = c + v[e_loc0] * u + v[e_loc1] * w;
+    const T sp = std::sin(v[e_phi]), cp = std::cos(v[e_phi]);
+    const T st = std::sin(v[e_theta]), ct = std::cos(v[e_theta]);
+    s.dir = {cp * st, sp * st, ct};
+    s.qop = v[e_qop];
+    s.time = v[e_time];
+    if constexpr (kJac) {
+        for (int r = 0; r < 3; ++r) {
+            for (int j = 0; j < 5; ++j) {
+                J.p[r][j] = T(0);
+                J.d[r][j] = T(0);
+            }
+        }
+        J.p[0][0] = u.x; J.p[1][0] = u.y; J.p[2][0] = u.z;
+        J.p[0][1] = w.x; J.p[1][1] = w.y; J.p[2][1] = w.z;
+        J.d[0][2] = -sp * st; J.d[1][2] = cp * st; J.d[2][2] = T(0);
+        J.d[0][3] = cp * ct;  J.d[1][3] = sp * ct; J.d[2][3] = -st;
+        J.t_qop = T(0);
+    }
+}
+
+// Free state on surface sf -> bound vector; with the path correction onto
+// the plane, the full 6x6 bound-to-bound Jacobian.
+template <typename T, bool kJac>
+TAMD_HD void free_to_bound(const ckf_surface& sf, const free_state<T>& s,
+                           const transport_jac<T>& J, const field_view<T>& f,
+                           T mass, T abs_q, T* v, T (*Jb)[6]) {
+    const vec3<T> c = sf_center<T>(sf);
+    const vec3<T> u = sf_u<T>(sf);
+    const vec3<T> vv = sf_v<T>(sf);
+    const vec3<T> rel = s.pos - c;
+    v[e_loc0] = dot(rel, u);
+    v[e_loc1] = dot(rel, vv);
+    v[e_phi] = std::atan2(s.dir.y, s.dir.x);
+    T cz = s.dir.z;
+    cz = cz > T(1) ? T(1) : (cz < T(-1) ? T(-1) : cz);
+    v[e_theta] = std::acos(cz);
+    v[e_qop] = s.qop;
+    v[e_time] = s.time;
+    if constexpr (kJac) {
+        const vec3<T> w = sf_w<T>(sf);
+        const T tw = dot(s.dir, w);
+        const vec3<T> B = field_at(f, s.pos);
+        const vec3<T> dTds = s.qop * cross(s.dir, B);
+        const T dt = dtds_of(s.qop, mass, abs_q);
+        T pc[3][5], dc[3][5], tc[5];
+        for (int j = 0; j < 5; ++j) {
+            const T dds = -(w.x * J.p[0][j] + w.y * J.p[1][j] + w.z * J.p[2][j]) / tw;
+            pc[0][j] = J.p[0][j] + s.dir.x * dds;
+            pc[1][j] = J.p[1][j] + s.dir.y * dds;
+            pc[2][j] = J.p[2][j] + s.dir.z * dds;
+            dc[0][j] = J.d[0][j] + dTds.x * dds;
+            dc[1][j] = J.d[1][j] + dTds.y * dds;
+            dc[2][j] = J.d[2][j] + dTds.z * dds;
+            tc[j] = (j == 4 ? J.t_qop : T(0)) + dt * dds;
+        }
+        const T sin2 = s.dir.x * s.dir.x + s.dir.y * s.dir.y;
+        const T sinT = std::sqrt(sin2);
+        const vec3<T> gphi{-s.dir.y / sin2, s.dir.x / sin2, T(0)};
+        const vec3<T> gth{s.dir.x * s.dir.z / sinT, s.dir.y * s.dir.z / sinT, -sinT};
+        for (int j = 0; j < 5; ++j) {
+            Jb[0][j] = u.x * pc[0][j] + u.y * pc[1][j] + u.z * pc[2][j];
+            Jb[1][j] = vv.x * pc[0][j] + vv.y * pc[1][j] + vv.z * pc[2][j];
+            Jb[2][j] = gphi.x * dc[0][j] + gphi.y * dc[1][j];
+            Jb[3][j] = gth.x * dc[0][j] + gth.y * dc[1][j] + gth.z * dc[2][j];
+            Jb[4][j] = (j == 4) ? T(1) : T(0);
+            Jb[5][j] = tc[j];
+        }
+        for (int r = 0; r < 5; ++r) {
+            Jb[r][5] = T(0);
+        }
+        Jb[5][5] = T(1);
+    }
+}
+
+// Highland multiple-scattering width (PDG review, passage of particles
+// through matter, eq. 34.15): theta0 = 13.6 MeV / (beta c p) |z| sqrt(x/X0)
+// (1 + 0.038 ln(x z^2 / (X0 beta^2))).
+template <typename T>
+TAMD_HD T highland_theta0(T x_over_x0, T qop, T mass, T abs_q) {
+    if (!(x_over_x0 > T(0))) {
+        return T(0);
+    }
+    const T p = abs_q / std::fabs(qop);
+    const T e = std::sqrt(p * p + mass * mass);
+    const T beta = p / e;
+    const T corr = T(1) + T(0.038) * std::log(x_over_x0 * abs_q * abs_q / (beta * beta));
+    return T(0.0136) / (beta * p) * abs_q * std::sqrt(x_over_x0) * corr;
+}
+
+enum nav_status : int { NAV_FOUND = 0, NAV_EXIT = 1, NAV_ABORT = 2 };
+
+constexpr int kMaxCandidates = 9;
+
+// Navigator + stepper for one track.  The algorithm:
+//  1. If the start surface is given, visit the remaining modules of its layer
+//     (module overlaps), then mark the layer visited.
+//  2. Pick the unvisited layer whose (r, z) envelope the straight line from
+//     the current state enters first; none -> NAV_EXIT.
+//  3. Approach it with RK steps until the entry is within approach_dist.
+//  4. Collect the modules in the search window around the projected entry
+//     point whose straight-line intersection is ahead (>= overstep_tol) and
+//     inside the mask within a distance-scaled tolerance; visit them in
+//     increasing straight-line distance: step onto each plane (iterated RK
+//     steps until |distance| <= on_surface_tol, then a straight snap), and
+//     accept the first whose exact mask contains the point and whose path
+//     from the start exceeds min_step_length -> NAV_FOUND.
+//  5. Mark the layer visited, go to 2.
+// More than max_steps RK steps, a path beyond path_limit, or a start momentum
+// below min_p / min_pT -> NAV_ABORT.
+template <typename T, bool kJac>
+struct navigator {
+    const ckf_surface* sfs;
+    const ckf_layer* layers;
+    uint32_t n_layers;
+    field_view<T> field;
+    nav_config<T> cfg;
+
+    free_state<T> st;
+    transport_jac<T> J;
+    T path;
+    uint32_t n_steps;
+
+    TAMD_HD T step_limit() const {
+        const vec3<T> B = field_at(field, st.pos);
+        const vec3<T> tb = cross(st.dir, B);
+        const T kappa = std::fabs(st.qop) * std::sqrt(dot(tb, tb));
+        T h = cfg.step_constraint;
+        if (kappa > static_cast<T>(1e-12)) {
+            const T ha = cfg.rk_step_scale * std::pow(kappa, static_cast<T>(-0.8));
+            h = ha < h ? ha : h;
+        }
+        return h;
+    }
+
+    TAMD_HD bool do_step(T h) {
+        rk4_step<T, kJac>(st, J, field, h, cfg.mass, cfg.abs_q);
+        path += h;
+        ++n_steps;
+        return n_steps <= cfg.max_steps && std::fabs(path) <= cfg.path_limit;
+    }
+
+    // 0: on the plane, 1: abort, 2: plane parallel to the track
+    TAMD_HD int step_to_plane(const ckf_surface& sf) {
+        const vec3<T> c = sf_center<T>(sf);
+        const vec3<T> w = sf_w<T>(sf);
+        T sd = T(0);
+        for (;;) {
+            const T tw = dot(st.dir, w);
+            if (std::fabs(tw) < static_cast<T>(1e-8)) {
+                return 2;
+            }
+            sd = dot(c - st.pos, w) / tw;
+            if (std::fabs(sd) <= cfg.on_surface_tol) {
+                break;
+            }
+            const T hl = step_limit();
+            const T h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
+            if (!do_step(h)) {
+                return 1;
+            }
+        }
+        st.pos = st.pos + sd * st.dir;
+        path += sd;
+        st.time += sd * dtds_of(st.qop, cfg.mass, cfg.abs_q);
+        return 0;
+    }
+
+    // 0: found (out_sf), 1: nothing in this layer, 2: abort
+    TAMD_HD int process_layer(uint32_t li, int exclude, uint32_t& out_sf) {
+        const ckf_layer& L = layers[li];
+        T s_entry = layer_entry_distance(L, st.pos, st.dir, cfg.mask_tol_max);
+        if (!(s_entry < static_cast<T>(1e29))) {
+            s_entry = T(0);
+        }
+        const vec3<T> q = st.pos + s_entry * st.dir;
+        int ic = 0, jc = 0;
+        const int nphi = static_cast<int>(L.n_phi);
+        const int nlong = static_cast<int>(L.n_long);
+        int wp = cfg.win_phi;
+        int wl = (L.kind == CKF_LAYER_BARREL) ? cfg.win_long : 0;
+        if (L.kind == CKF_LAYER_PLANE) {
+            // every module of a plane layer is a candidate
+            wp = (nphi - 1) / 2;
+            wl = (nlong - 1) / 2;
+            ic = wp;
+            jc = wl;
+        } else {
+            layer_bin(L, q, ic, jc);
+        }
+        if (2 * wp + 1 > nphi) {
+            wp = (nphi - 1) / 2;
+        }
+        T cs[kMaxCandidates];
+        uint32_t cm[kMaxCandidates];
+        int nc = 0;
+        for (int dp = -wp; dp <= wp; ++dp) {
+            int ip = (ic + dp) % nphi;
+            if (ip < 0) {
+                ip += nphi;
+            }
+            for (int dl = -wl; dl <= wl; ++dl) {
+                const int il = jc + dl;
+                if (il < 0 || il >= nlong) {
+                    continue;
+                }
+                const uint32_t m = L.first_surface +
+                                   static_cast<uint32_t>(ip * nlong + il);
+                if (static_cast<int>(m) == exclude) {
+                    continue;
+                }
+                const ckf_surface& S = sfs[m];
+                const vec3<T> w = sf_w<T>(S);
+                const T tw = dot(st.dir, w);
+                if (std::fabs(tw) < static_cast<T>(1e-6)) {
+                    continue;
+                }
+                const vec3<T> c = sf_center<T>(S);
+                const T s = dot(c - st.pos, w) / tw;
+                if (s < cfg.overstep_tol) {
+                    continue;
+                }
+                const vec3<T> x = st.pos + s * st.dir - c;
+                T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
+                tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
+                if (!inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
+                    continue;
+                }
+                if (nc < kMaxCandidates) {
+                    cs[nc] = s;
+                    cm[nc] = m;
+                    ++nc;
+                }
+            }
+        }
+        uint32_t used = 0u;
+        for (int k = 0; k < nc; ++k) {
+            int pick = -1;
+            for (int i = 0; i < nc; ++i) {
+                if (used & (1u << i)) {
+                    continue;
+                }
+                if (pick < 0 || cs[i] < cs[pick] ||
+                    (cs[i] == cs[pick] && cm[i] < cm[pick])) {
+                    pick = i;
+                }
+            }
+            used |= (1u << pick);
+            const ckf_surface& S = sfs[cm[pick]];
+            {
+                const vec3<T> w = sf_w<T>(S);
+                const T tw = dot(st.dir, w);
+                if (std::fabs(tw) < static_cast<T>(1e-8)) {
+                    continue;
+                }
+                const T s_now = dot(sf_center<T>(S) - st.pos, w) / tw;
+                if (s_now < cfg.overstep_tol) {
+                    continue;
+                }
+            }
+            const int r = step_to_plane(S);
+            if (r == 1) {
+                return 2;
+            }
+            if (r == 2) {
+                continue;
+            }
+            const vec3<T> x = st.pos - sf_center<T>(S);
+            if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
+                path > cfg.min_step_length) {
+                out_sf = cm[pick];
+                return 0;
+            }
+        }
+        return 1;
+    }
+
+    // Propagate from the current state (on surface start_sf, or -1 for a
+    // free start) to the next sensitive surface.
+    TAMD_HD int propagate(int start_sf, uint32_t& out_sf) {
+        path = T(0);
+        n_steps = 0u;
+        {
+            const T p = cfg.abs_q / std::fabs(st.qop);
+            const T pT = p * std::sqrt(st.dir.x * st.dir.x + st.dir.y * st.dir.y);
+            if (p < cfg.min_p || pT < cfg.min_pT) {
+                return NAV_ABORT;
+            }
+        }
+        uint64_t visited = 0ull;
+        if (start_sf >= 0) {
+            const uint32_t l0 = sfs[start_sf].layer;
+            const int r = process_layer(l0, start_sf, out_sf);
+            if (r == 0) {
+                return NAV_FOUND;
+            }
+            if (r == 2) {
+                return NAV_ABORT;
+            }
+            visited |= (1ull << l0);
+        }
+        for (uint32_t guard = 0; guard <= n_layers; ++guard) {
+            int best = -1;
+            T bs = static_cast<T>(1e30);
+            for (uint32_t li = 0; li < n_layers; ++li) {
+                if (visited & (1ull << li)) {
+                    continue;
+                }
+                const T s = layer_entry_distance(layers[li], st.pos, st.dir,
+                                                 cfg.mask_tol_max);
+                if (s < bs) {
+                    bs = s;
+                    best = static_cast<int>(li);
+                }
+            }
+            if (best < 0 || !(bs < static_cast<T>(1e29))) {
+                return NAV_EXIT;
+            }
+            while (bs > cfg.approach_dist && bs < static_cast<T>(1e29)) {
+                const T hl = step_limit();
+                if (!do_step(bs < hl ? bs : hl)) {
+                    return NAV_ABORT;
+                }
+                bs = layer_entry_distance(layers[best], st.pos, st.dir,
+                                          cfg.mask_tol_max);
+            }
+            if (bs < static_cast<T>(1e29)) {
+                const int r = process_layer(static_cast<uint32_t>(best), -1, out_sf);
+                if (r == 0) {
+                    return NAV_FOUND;
+                }
+                if (r == 2) {
+                    return NAV_ABORT;
+                }
+            }
+            visited |= (1ull << best);
+        }
+        return NAV_EXIT;
+    }
+};
+
+}  // namespace tamd
