@@ -253,12 +253,15 @@ ckf_status ckf_device_detector_create(const ckf_detector* det, const ckf_bfield*
                                       ckf_device_detector** out);
 ckf_status ckf_device_detector_destroy(ckf_device_detector* det);
 
-/* Copy one event's measurements and seeds to HBM (vecmem::copy setup of the
- * traccc device chain).  Checks the grouped-by-surface precondition
- * (traccc device: input_is_valid, device/cuda/src/finding/combinatorial_kalman_filter_algorithm.cu:39-47). */
+/* Copy events' measurements and seeds to HBM (the vecmem::copy setup of the
+ * traccc device chain).  n_events >= 1 independent events form one batch that
+ * a single ckf_find_tracks call processes in one step pipeline.  Checks the
+ * grouped-by-surface precondition (traccc device: input_is_valid,
+ * device/cuda/src/finding/combinatorial_kalman_filter_algorithm.cu:39-47) and
+ * maps surface identifiers to detector surfaces. */
 ckf_status ckf_device_event_create(const ckf_device_detector* det,
                                    const ckf_measurements* meas,
-                                   const ckf_bound_params* seeds,
+                                   const ckf_bound_params* seeds, uint32_t n_events,
                                    ckf_device_event** out);
 ckf_status ckf_device_event_destroy(ckf_device_event* ev);
 
@@ -267,17 +270,13 @@ ckf_status ckf_device_event_destroy(ckf_device_event* ev);
  * core/src/finding/combinatorial_kalman_filter_algorithm.cpp:35-59 (host) and
  * device/common/src/finding/combinatorial_kalman_filter_algorithm.cpp:52-704 (device).
  * Inputs and outputs stay in HBM; the call is ordered on the algorithm's stream
- * and returns after the result sizes are known. */
+ * and returns once the result sizes are known.  For a batch, the tracks of
+ * event e are [track_offsets[e], track_offsets[e+1]) (ckf_tracks_event_offsets). */
 ckf_status ckf_find_tracks(ckf_algorithm* alg, const ckf_device_detector* det,
                            const ckf_device_event* ev, ckf_device_tracks** out);
 
-/* Batched operator: n_events independent events in one step pipeline (one
- * device work queue, one captured step graph).  out[i] receives event i's tracks. */
-ckf_status ckf_find_tracks_batched(ckf_algorithm* alg, const ckf_device_detector* det,
-                                   const ckf_device_event* const* events, uint32_t n_events,
-                                   ckf_device_tracks** out);
-
-/* Host-pointer convenience form of the operator (uploads, runs, downloads). */
+/* Host-pointer convenience form of the operator for one event (uploads,
+ * runs; download with ckf_tracks_download). */
 ckf_status ckf_find_tracks_host(ckf_algorithm* alg, const ckf_detector* det,
                                 const ckf_bfield* field, const ckf_measurements* meas,
                                 const ckf_bound_params* seeds, ckf_device_tracks** out);
@@ -285,6 +284,9 @@ ckf_status ckf_find_tracks_host(ckf_algorithm* alg, const ckf_detector* det,
 /* ---- results ---------------------------------------------------------------- */
 ckf_status ckf_tracks_size(const ckf_device_tracks* trk, uint32_t* n_tracks,
                            uint32_t* n_constituents);
+/* Per-event split of a batched result: n_events + 1 entries each (either may be NULL). */
+ckf_status ckf_tracks_event_offsets(const ckf_device_tracks* trk, uint32_t* n_events,
+                                    uint32_t* track_offsets, uint32_t* constituent_offsets);
 /* Copy into caller-owned host buffers sized from ckf_tracks_size(). */
 ckf_status ckf_tracks_download(const ckf_device_tracks* trk, ckf_track_candidates* out);
 ckf_status ckf_tracks_destroy(ckf_device_tracks* trk);

@@ -1,0 +1,53 @@
+/*
+ * Diagnostic entry points of the traccc_amd CKF library.
+ *
+ * No counterpart in the reference: these run single pieces of the device
+ * step pipeline on caller arrays so tests can compare them with a CPU oracle
+ * (the Kalman update of traccc's gain_matrix_updater,
+ * core/include/traccc/fitting/kalman_filter/gain_matrix_updater.hpp:69-181,
+ * and the propagation of details/combinatorial_kalman_filter.hpp:528-627).
+ * All pointers are host pointers; each call uploads, runs one kernel and
+ * downloads.
+ */
+#ifndef TRACCC_AMD_CKF_DEBUG_H
+#define TRACCC_AMD_CKF_DEBUG_H
+
+#include <stdint.h>
+
+#include "traccc_amd/ckf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* For n (state, measurement) pairs: predicted chi2 and gain-matrix update.
+ * vec [n][6], cov [n][36] (row-major, symmetric), meas/var [n][2], dims [n],
+ * subspace [n][2].  Outputs: chi2 [n], status [n] (ckf_fitter_status),
+ * out_vec [n][6], out_cov [n][36]. */
+ckf_status ckf_debug_kf_update(uint32_t n, const float* vec, const float* cov, const float* meas,
+                               const float* var, const uint32_t* dims, const uint8_t* subspace,
+                               float* chi2, int32_t* status, float* out_vec, float* out_cov);
+
+/* Propagate n bound states from surface index start[i] to the next sensitive
+ * surface with the CKF's propagation settings (material of the start surface
+ * applied first when apply_start_material != 0, of the end surface always).
+ * status[i]: 0 found, 1 left the detector, 2 aborted. */
+ckf_status ckf_debug_propagate(const ckf_finding_config* cfg, const ckf_detector* det,
+                               const ckf_bfield* field, uint32_t n, const uint32_t* start,
+                               const float* vec, const float* cov, int apply_start_material,
+                               int32_t* status, uint32_t* end_surface, float* out_vec,
+                               float* out_cov, float* path, uint32_t* n_steps);
+
+/* Per-kernel device time of the last ckf_find_tracks call, measured with HIP
+ * events on the algorithm's stream around each kernel group of every step
+ * (find; duplicate removal; propagation; step advance).  While enabled the
+ * step loop is launched step by step instead of as a captured graph. */
+ckf_status ckf_debug_set_kernel_timing(ckf_algorithm* alg, int enable);
+ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, double* ms_dedup,
+                                  double* ms_propagate, double* ms_other, uint32_t* n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

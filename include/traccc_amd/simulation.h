@@ -24,7 +24,7 @@ typedef enum tamd_detector_preset {
     TAMD_DET_BARREL4 = 0,        /* 4 barrel layers r = 32/72/116/172 mm, |z| < 500 mm   */
     TAMD_DET_BARREL4_DISK4 = 1,  /* + 2 x 4 endcap disks at |z| = 600..1500 mm           */
     TAMD_DET_BARREL5_DISK6 = 2,  /* 5 barrel layers + 2 x 6 disks at |z| = 600..1500 mm  */
-    TAMD_DET_TELESCOPE = 3       /* planes normal to x at x = 10 + k * spacing           */
+    TAMD_DET_TELESCOPE = 3       /* planes normal to x at x = (k + 1) * spacing          */
 } tamd_detector_preset;
 
 typedef struct tamd_detector {

@@ -391,9 +391,8 @@ struct propagation {
             const S h = dist > S(0) ? std::min(dist, lim) : std::max(dist, -lim);
             if (!step(h)) return 1;
         }
-        for (int i = 0; i < 3; ++i) F.a[i][0] += dist * F.a[4 + i][0];
-        path += dist;
-        F.a[3][0] += dist * dt_ds(F.a[7][0], mass, abs_q);
+        // the remaining sub-tolerance distance is one more RK step
+        if (dist != S(0) && !step(dist)) return 1;
         return 0;
     }
 

@@ -1,0 +1,396 @@
+"""Pins of the CPU oracle against facts fixed outside the oracle itself.
+
+Each oracle function and what pins it:
+  prob (chi2 p-value)     reference KATs (tests/core/test_pvalue.cpp:21-36) + scipy gammaincc
+  KF chi2 / gain update   numpy textbook Kalman filter (Fruehwirth 1987) in FP64, the
+                          information-form identity, symmetric / PSD invariants,
+                          1-D and swapped-subspace measurements, angle wrapping
+  RK4 step                closed-form helix in uniform Bz, straight line at B = 0,
+                          4th-order convergence, exact time of flight
+  transport Jacobian      central finite differences of the step map
+  propagation covariance  finite differences of the full bound -> bound map (path
+                          correction and free<->bound projections included)
+  plane intersection      brute-force root finding on the exact helix
+  material noise          PDG Highland formula evaluated independently
+  field map               scipy complete elliptic integrals, on-axis solenoid closed
+                          form, div B = 0, bilinear reproduction of linear fields
+CKF bookkeeping pins live in test_ckf_closed_forms.py.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import oracle
+from traccc_amd import simulation as S
+
+UNIT_T = 0.000299792458
+C_MMNS = 299.792458
+
+
+# ---------------------------------------------------------------------------
+# prob
+# ---------------------------------------------------------------------------
+def f32_eq(a, b, ulps=4):
+    a32, b32 = np.float32(a), np.float32(b)
+    if a32 == b32:
+        return True
+    ia = np.frombuffer(a32.tobytes(), np.int32)[0]
+    ib = np.frombuffer(b32.tobytes(), np.int32)[0]
+    return abs(int(ia) - int(ib)) <= ulps
+
+
+@pytest.mark.parametrize("fn", [oracle.prob, oracle.prob_f])
+def test_prob_reference_kats(fn):
+    # tests/core/test_pvalue.cpp:21-36 (EXPECT_FLOAT_EQ = 4 ulp)
+    assert f32_eq(fn(0.0, 0.1), 1.0)
+    assert f32_eq(fn(0.0, 1.0), 1.0)
+    assert f32_eq(fn(0.0, 10.0), 1.0)
+    assert f32_eq(fn(-1.0, 12.0), 0.0)
+    assert f32_eq(fn(-194.0, 12.0), 0.0)
+    assert f32_eq(fn(3.0, -17.4), 0.0)
+    assert f32_eq(fn(27.0, 0.0), 0.0)
+    assert f32_eq(fn(3.0, 5.0), 0.69998584)
+    assert f32_eq(fn(17.4, 6.26), 0.0094291369)
+
+
+def test_prob_matches_scipy():
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        ndf = float(rng.uniform(0.5, 60.0))
+        chi2 = float(rng.uniform(0.01, 4 * ndf + 20))
+        ref = sp.gammaincc(ndf / 2, chi2 / 2)
+        got = oracle.prob(chi2, ndf)
+        assert abs(got - ref) <= 1e-12 + 1e-9 * ref
+        got32 = oracle.prob_f(chi2, ndf)
+        assert abs(got32 - ref) <= 1e-6 + 1e-4 * ref
+
+
+# ---------------------------------------------------------------------------
+# Kalman update
+# ---------------------------------------------------------------------------
+def random_state(rng):
+    A = rng.normal(size=(6, 6))
+    scales = np.array([0.05, 0.05, 2e-3, 2e-3, 0.02, 1.0])
+    P = (A @ A.T + 0.5 * np.eye(6)) * np.outer(scales, scales)
+    x = np.array([rng.uniform(-8, 8), rng.uniform(-30, 30), rng.uniform(-3, 3),
+                  rng.uniform(0.3, 2.8), rng.uniform(-1, 1), 0.0])
+    return x, P
+
+
+def textbook_update(x, P, m, V, idx):
+    H = np.zeros((len(idx), 6))
+    for r, c in enumerate(idx):
+        H[r, c] = 1.0
+    S_ = H @ P @ H.T + V
+    K = P @ H.T @ np.linalg.inv(S_)
+    r = m - H @ x
+    chi2 = float(r @ np.linalg.solve(S_, r))
+    return x + K @ r, (np.eye(6) - K @ H) @ P, chi2, H
+
+
+def test_kf_update_matches_textbook():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        x, P = random_state(rng)
+        m = np.array([x[0] + rng.normal(0, 0.05), x[1] + rng.normal(0, 0.05)], np.float32)
+        var = np.array([rng.uniform(1e-4, 1e-2), rng.uniform(1e-4, 1e-2)], np.float32)
+        st, chi2, xv, cv = oracle.kf_update(x, P, m, var)
+        assert st == 0
+        xt, Pt, chi2t, H = textbook_update(x, P, m.astype(np.float64), np.diag(var.astype(np.float64)), [0, 1])
+        assert abs(chi2 - chi2t) <= 1e-9 * max(1.0, chi2t)
+        np.testing.assert_allclose(xv, xt, rtol=1e-8, atol=1e-10)
+        np.testing.assert_allclose(cv, Pt, rtol=1e-7, atol=1e-14)
+        # information form: P'^-1 = P^-1 + H^T V^-1 H
+        info = np.linalg.inv(P) + H.T @ np.diag(1.0 / var.astype(np.float64)) @ H
+        np.testing.assert_allclose(np.linalg.inv(cv), info, rtol=1e-6, atol=1e-6)
+        assert np.allclose(cv, cv.T, rtol=1e-12, atol=1e-15)
+        assert np.all(np.linalg.eigvalsh(cv) > 0)
+        assert chi2 >= 0
+
+
+def test_kf_update_one_dimensional_and_swapped_subspace():
+    rng = np.random.default_rng(3)
+    x, P = random_state(rng)
+    # 1-D measurement of loc0: only R00 enters
+    m = np.array([x[0] + 0.03, 123.0], np.float32)
+    var = np.array([0.0025, 7.0], np.float32)
+    st, chi2, xv, cv = oracle.kf_update(x, P, m, var, dims=1, subspace=(0, 1))
+    xt, Pt, chi2t, _ = textbook_update(x, P, m[:1].astype(np.float64), np.array([[0.0025]]), [0])
+    assert st == 0
+    assert abs(chi2 - chi2t) <= 1e-9
+    np.testing.assert_allclose(xv, xt, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(cv, Pt, rtol=1e-7, atol=1e-14)
+    # subspace (1, 0): positions are stored as (loc0, loc1); a subspace that
+    # starts at loc1 reads them in the order (loc1, loc0) (measurement_helpers.ipp)
+    m2 = np.array([x[0] - 0.01, x[1] + 0.02], np.float32)
+    var2 = np.array([0.004, 0.001], np.float32)
+    st, chi2, xv, cv = oracle.kf_update(x, P, m2, var2, dims=2, subspace=(1, 0))
+    xt, Pt, chi2t, _ = textbook_update(x, P, np.array([m2[1], m2[0]], np.float64),
+                                       np.diag([var2[1], var2[0]]).astype(np.float64), [1, 0])
+    assert st == 0
+    assert abs(chi2 - chi2t) <= 1e-9 * max(1, chi2t)
+    np.testing.assert_allclose(xv, xt, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(cv, Pt, rtol=1e-7, atol=1e-14)
+
+
+def test_kf_update_wraps_theta_over_the_pole():
+    # strong loc0-theta correlation drags theta beyond pi: phi flips by pi,
+    # theta reflects (traccc detail::wrap_phi_theta)
+    x = np.array([0.0, 0.0, 0.5, math.pi - 0.01, 0.1, 0.0])
+    P = np.eye(6) * 1e-4
+    P[0, 0] = 1.0
+    P[3, 3] = 1.0
+    P[0, 3] = P[3, 0] = 0.999
+    m = np.array([0.5, 0.0], np.float32)
+    var = np.array([1e-6, 1e-6], np.float32)
+    st, chi2, xv, cv = oracle.kf_update(x, P, m, var)
+    xt, _, _, _ = textbook_update(x, P, m.astype(np.float64), np.diag(var).astype(np.float64), [0, 1])
+    assert xt[3] > math.pi
+    assert st == 0
+    assert abs(xv[3] - (2 * math.pi - xt[3])) < 1e-9
+    assert abs(((xv[2] - (xt[2] + math.pi) + math.pi) % (2 * math.pi)) - math.pi) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# RK4 stepper
+# ---------------------------------------------------------------------------
+def helix_exact(free, bz, h):
+    x, y, z, t, tx, ty, tz, qop = free
+    k = qop * bz * UNIT_T
+    c0 = complex(tx, ty)
+    if abs(k) < 1e-15:
+        pos = complex(x, y) + c0 * h
+        T = c0
+    else:
+        T = c0 * np.exp(-1j * k * h)
+        pos = complex(x, y) + c0 * (1 - np.exp(-1j * k * h)) / (1j * k)
+    return np.array([pos.real, pos.imag, z + tz * h]), np.array([T.real, T.imag, tz])
+
+
+def unit_dir(phi, theta):
+    return np.array([math.cos(phi) * math.sin(theta), math.sin(phi) * math.sin(theta), math.cos(theta)])
+
+
+@pytest.mark.parametrize("pt,qsign", [(0.5, 1), (1.0, -1), (10.0, 1)])
+def test_rk_step_matches_helix_and_converges(pt, qsign):
+    fld = S.constant_field(2.0)
+    theta = 1.1
+    d = unit_dir(0.3, theta)
+    p = pt / math.sin(theta)
+    free = np.array([1.0, -2.0, 3.0, 0.5, *d, qsign / p])
+    errs = []
+    for h in (40.0, 20.0):
+        out, _ = oracle.rk_step(fld, 0.1056583755, 1.0, h, free)
+        pos, T = helix_exact(free, 2.0, h)
+        errs.append(np.linalg.norm(out[:3] - pos))
+        assert np.linalg.norm(out[4:7] - T) < 1e-6
+        # time of flight is exact for a straight-line path length h
+        mass = 0.1056583755
+        assert abs(out[3] - (0.5 + h * math.sqrt(1 + (mass / p) ** 2) / C_MMNS)) < 1e-12
+    assert errs[0] < 1e-3
+    if errs[1] > 1e-11:
+        assert 20 < errs[0] / errs[1] < 45  # local error ~ h^5
+
+
+def test_rk_step_straight_line_without_field():
+    fld = S.constant_field(0.0)
+    d = unit_dir(-1.0, 0.7)
+    free = np.array([0.0, 0.0, 0.0, 0.0, *d, 1.0])
+    out, D = oracle.rk_step(fld, 0.1, 1.0, 37.0, free)
+    np.testing.assert_allclose(out[:3], 37.0 * d, atol=1e-12)
+    np.testing.assert_allclose(D[:3, 4:7], 37.0 * np.eye(3), atol=1e-12)
+
+
+def test_rk_transport_matrix_matches_finite_differences():
+    fld = S.constant_field(2.0)
+    d = unit_dir(0.4, 1.2)
+    free = np.array([10.0, 5.0, -3.0, 1.0, *d, -1 / 0.8])
+    _, D = oracle.rk_step(fld, 0.1395, 1.0, 30.0, free)
+    num = np.zeros((8, 8))
+    for j in range(8):
+        eps = 1e-6 if j != 7 else 1e-7
+        fp, fm = free.copy(), free.copy()
+        fp[j] += eps
+        fm[j] -= eps
+        op, _ = oracle.rk_step(fld, 0.1395, 1.0, 30.0, fp)
+        om, _ = oracle.rk_step(fld, 0.1395, 1.0, 30.0, fm)
+        num[:, j] = (op - om) / (2 * eps)
+    # direction columns: the step renormalises T, the transport does not;
+    # compare on the tangent space (perturbations orthogonal to T)
+    Pt = np.eye(3) - np.outer(d, d)
+    Dd = D.copy()
+    nd = num.copy()
+    Dd[:, 4:7] = D[:, 4:7] @ Pt
+    nd[:, 4:7] = num[:, 4:7] @ Pt
+    np.testing.assert_allclose(Dd, nd, rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# full propagation
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def barrel():
+    det = S.build_detector(S.DET_BARREL4, 0.0)
+    fld = S.constant_field(2.0)
+    cfg = S.finding_config_for("C1")
+    cfg.propagation.rk_error_tol = 1e-8
+    return det, fld, cfg
+
+
+def surface_frame(det, i):
+    s = det.surface(i)
+    return (np.array(s.center[:]), np.array(s.axis_u[:]), np.array(s.axis_v[:]),
+            np.array(s.normal[:]))
+
+
+def start_state(det, sf_idx, l0=1.0, l1=-5.0, phi_off=0.02, theta=1.3, qop=-1 / 1.5):
+    c, u, v, w = surface_frame(det, sf_idx)
+    phi = math.atan2(c[1], c[0]) + phi_off
+    return np.array([l0, l1, phi, theta, qop, 0.0])
+
+
+def test_propagation_hits_the_helix_plane_intersection(barrel):
+    det, fld, cfg = barrel
+    start = 3
+    vec = start_state(det, start)
+    st, end, ov, oc, path, nsteps = oracle.propagate(cfg, det, fld, start, vec, np.eye(6) * 1e-4)
+    assert st == 0
+    c0, u0, v0, _ = surface_frame(det, start)
+    pos0 = c0 + vec[0] * u0 + vec[1] * v0
+    d0 = unit_dir(vec[2], vec[3])
+    free = np.array([*pos0, 0.0, *d0, vec[4]])
+    c1, u1, v1, w1 = surface_frame(det, end)
+
+    def g(s):
+        p, _ = helix_exact(free, 2.0, s)
+        return float((p - c1) @ w1)
+    # brute force: bracket the first sign change, then bisect
+    grid = np.linspace(0.5, path * 1.5, 4000)
+    vals = [g(s) for s in grid]
+    k = next(i for i in range(len(grid) - 1) if vals[i] * vals[i + 1] <= 0)
+    a, b = grid[k], grid[k + 1]
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if g(a) * g(m) <= 0:
+            b = m
+        else:
+            a = m
+    s_star = 0.5 * (a + b)
+    p_star, T_star = helix_exact(free, 2.0, s_star)
+    assert abs(path - s_star) < 1e-5
+    assert abs(ov[0] - (p_star - c1) @ u1) < 1e-5
+    assert abs(ov[1] - (p_star - c1) @ v1) < 1e-5
+    assert abs(ov[2] - math.atan2(T_star[1], T_star[0])) < 1e-8
+    assert abs(ov[3] - math.acos(T_star[2])) < 1e-8
+
+
+def test_propagation_covariance_matches_finite_differences(barrel):
+    det, fld, cfg = barrel
+    start = 3
+    vec = start_state(det, start)
+    st, end, ov, oc, _, _ = oracle.propagate(cfg, det, fld, start, vec, np.eye(6))
+    assert st == 0
+    J = np.zeros((6, 6))
+    eps = np.array([1e-4, 1e-4, 1e-7, 1e-7, 1e-6, 1e-4])
+    for j in range(6):
+        vp, vm = vec.copy(), vec.copy()
+        vp[j] += eps[j]
+        vm[j] -= eps[j]
+        sp_, ep, op, _, _, _ = oracle.propagate(cfg, det, fld, start, vp, np.eye(6))
+        sm_, em, om, _, _, _ = oracle.propagate(cfg, det, fld, start, vm, np.eye(6))
+        assert sp_ == 0 and sm_ == 0 and ep == end and em == end
+        J[:, j] = (op - om) / (2 * eps[j])
+    np.testing.assert_allclose(oc, J @ J.T, rtol=2e-4, atol=1e-9)
+
+
+def test_material_noise_is_highland(barrel):
+    # straight track through a telescope plane with 2% X0 at 30 degrees
+    det = S.build_detector(S.DET_TELESCOPE, 0.02, telescope_planes=3, telescope_spacing=20.0)
+    fld = S.constant_field(0.0)
+    cfg = S.finding_config_for("C1")
+    mass = cfg.ptc_mass
+    p = 2.0
+    vec = np.array([0.0, 0.0, math.radians(30.0), math.pi / 2, -1 / p, 0.0])
+    cov = np.eye(6) * 1e-6
+    st, end, ov, oc, _, _ = oracle.propagate(cfg, det, fld, 0, vec, cov, apply_start_material=False)
+    st0, _, _, oc0, _, _ = oracle.propagate(cfg, S.build_detector(S.DET_TELESCOPE, 0.0, 3, 20.0), fld,
+                                             0, vec, cov, apply_start_material=False)
+    assert st == 0 and st0 == 0
+    beta = p / math.hypot(p, mass)
+    x = 0.02 / math.cos(math.radians(30.0))
+    th0 = 0.0136 / (beta * p) * math.sqrt(x) * (1 + 0.038 * math.log(x / beta ** 2))
+    assert abs((oc[3, 3] - oc0[3, 3]) - th0 ** 2) < 1e-12
+    assert abs((oc[2, 2] - oc0[2, 2]) - th0 ** 2 / math.sin(ov[3]) ** 2) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# field map
+# ---------------------------------------------------------------------------
+def test_cel_matches_scipy_elliptic_integrals():
+    for kc in (0.05, 0.3, 0.7, 0.99):
+        m = 1 - kc * kc
+        assert abs(S.cel(kc, 1, 1, 1) - sp.ellipk(m)) < 1e-10
+        assert abs(S.cel(kc, 1, 1, kc * kc) - sp.ellipe(m)) < 1e-10
+
+
+def test_solenoid_field_closed_forms():
+    R, L, B0 = 1200.0, 5600.0, 2.0
+    br, bz = S.solenoid_exact(0.0, 0.0)
+    assert abs(bz - B0) < 1e-9 and abs(br) < 1e-12
+
+    def onaxis(z):
+        b = L / 2
+        f = lambda u: u / math.sqrt(u * u + R * R)
+        f0 = f(b) - f(-b)
+        return B0 * (f(z + b) - f(z - b)) / f0
+    for z in (0.0, 1000.0, 2800.0, 4000.0):
+        assert abs(S.solenoid_exact(0.0, z)[1] - onaxis(z)) < 1e-7
+    # div B = 0: (1/r) d(r Br)/dr + dBz/dz
+    for r, z in ((300.0, 1000.0), (900.0, 2500.0), (1500.0, 3000.0)):
+        h = 1e-2
+        drbr = ((r + h) * S.solenoid_exact(r + h, z)[0] - (r - h) * S.solenoid_exact(r - h, z)[0]) / (2 * h)
+        dbz = (S.solenoid_exact(r, z + h)[1] - S.solenoid_exact(r, z - h)[1]) / (2 * h)
+        assert abs(drbr / r + dbz) < 1e-7
+
+
+def _linear_map(bz0, gz, br_coef):
+    """(r, z) table of B_r = br_coef * r, B_z = bz0 + gz * z (bilinear reproduces it)."""
+    f = S.solenoid_map(r_max=500.0, z_max=500.0, pitch=10.0)
+    n_r, n_z = f.struct.n_r, f.struct.n_z
+    rr = np.arange(n_r) * 10.0
+    zz = -500.0 + np.arange(n_z) * 10.0
+    t = np.zeros((n_z, n_r, 2), np.float32)
+    t[:, :, 0] = br_coef * rr[None, :]
+    t[:, :, 1] = bz0 + gz * zz[:, None]
+    f.table[:] = t.reshape(-1)
+    return f
+
+
+def test_map_field_bilinear_and_axes():
+    fm = _linear_map(2.0, 0.0, 0.0)
+    fc = S.constant_field(2.0)
+    d = unit_dir(0.2, 1.0)
+    free = np.array([13.0, 7.0, 40.0, 0.0, *d, 1 / 0.7])
+    a, Da = oracle.rk_step(fm, 0.1, 1.0, 25.0, free)
+    b, Db = oracle.rk_step(fc, 0.1, 1.0, 25.0, free)
+    np.testing.assert_allclose(a, b, atol=1e-9)
+    # a z-gradient: compare with numpy RK4 on the analytic field
+    fg = _linear_map(2.0, 1e-3, 0.0)
+    def fieldv(p):
+        return np.array([0.0, 0.0, (2.0 + 1e-3 * p[2]) * UNIT_T])
+    def rk4(p, T, q, h):
+        k1 = q * np.cross(T, fieldv(p))
+        p2 = p + h / 2 * T + h * h / 8 * k1
+        k2 = q * np.cross(T + h / 2 * k1, fieldv(p2))
+        k3 = q * np.cross(T + h / 2 * k2, fieldv(p2))
+        p4 = p + h * T + h * h / 2 * k3
+        k4 = q * np.cross(T + h * k3, fieldv(p4))
+        pn = p + h * T + h * h / 6 * (k1 + k2 + k3)
+        Tn = T + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        return pn, Tn / np.linalg.norm(Tn)
+    g, _ = oracle.rk_step(fg, 0.1, 1.0, 25.0, free)
+    pn, Tn = rk4(free[:3], free[4:7], free[7], 25.0)
+    np.testing.assert_allclose(g[:3], pn, atol=1e-9)
+    np.testing.assert_allclose(g[4:7], Tn, atol=1e-9)

@@ -1,0 +1,148 @@
+// Device-side data layout of the CKF step pipeline (shared by the kernels and
+// the host runtime).
+//
+// HBM layout (structure of arrays, one lane per branch):
+//   bound state : 28 x 4 B columns of `cap` rows: 6 parameters, the 21-entry
+//                 upper triangle of the covariance, the surface index (u32)
+//   link        : 32 B record {prev, meas, seed, step|skipped|consecutive,
+//                 chi2, chi2_sum, ndf_sum, state_ref}
+//   measurement : float2 position, float2 variance, u32 {dims, subspace}
+#pragma once
+
+#include <cstdint>
+
+#include "../core/propagator.hpp"
+
+namespace tamd {
+
+constexpr int kStateCols = 28;  // 6 + 21 + 1
+constexpr uint32_t kInvalid = 0xffffffffu;
+
+// state_ref encoding: where a link's (filtered) state lives
+constexpr uint32_t kRefKindShift = 30;
+constexpr uint32_t kRefFiltered = 0u;  // filtered-state slot
+constexpr uint32_t kRefInput = 1u;     // previous step's propagated state slot
+constexpr uint32_t kRefSeed = 2u;      // seed index (step 0)
+constexpr uint32_t kRefMask = (1u << kRefKindShift) - 1u;
+
+struct dlink {
+    uint32_t prev;      // global index of the link it continues, kInvalid at step 0
+    uint32_t meas;      // global measurement index, kInvalid for a hole
+    uint32_t seed;      // global seed index
+    uint32_t packed;    // step (16 b) | n_skipped (8 b) << 16 | n_consecutive (8 b) << 24
+    float chi2;
+    float chi2_sum;
+    uint32_t ndf_sum;
+    uint32_t state_ref;
+};
+static_assert(sizeof(dlink) == 32, "link record is 32 B");
+
+TAMD_HD uint32_t link_step(const dlink& l) { return l.packed & 0xffffu; }
+TAMD_HD uint32_t link_skipped(const dlink& l) { return (l.packed >> 16) & 0xffu; }
+TAMD_HD uint32_t link_consec(const dlink& l) { return (l.packed >> 24) & 0xffu; }
+TAMD_HD uint32_t pack_link(uint32_t step, uint32_t skipped, uint32_t consec) {
+    return (step & 0xffffu) | ((skipped > 255u ? 255u : skipped) << 16) |
+           ((consec > 255u ? 255u : consec) << 24);
+}
+
+// Step control block in device memory; the step loop reads and advances it
+// without host round trips.
+struct step_ctl {
+    uint32_t step;        // current CKF step
+    uint32_t epoch;       // tag of this step's look-back descriptors
+    uint32_t n_in;        // inputs of this step
+    uint32_t link_base;   // global index of this step's first link
+    uint32_t n_links;     // links made this step (written by find)
+    uint32_t n_live;      // inputs of the next step (written by propagate)
+    uint32_t n_tips;      // tips recorded before this step
+    uint32_t n_tips_step; // tips recorded this step (written by propagate)
+    uint32_t ticket[4];   // work-queue tickets: find, dedup, propagate, build
+    uint32_t overflow;    // bit 0 links, bit 1 per-step states, bit 2 filtered, bit 3 tips, bit 4 inputs
+    uint32_t done;
+    uint32_t n_steps_run;
+    uint32_t n_tracks;
+    uint32_t n_constituents;
+    uint32_t prev_link_base;  // global index of the previous step's first link
+    uint64_t n_links_total;
+    uint64_t n_props;
+};
+
+struct ckf_params {
+    // detector + field
+    const ckf_surface* sfs;
+    const ckf_layer* layers;
+    uint32_t n_layers;
+    uint32_t n_surfaces;
+    field_view<float> field;
+    nav_config<float> nav;
+    // finding config
+    uint32_t n_branch_surface;  // max_num_branches_per_surface
+    uint32_t n_branch_seed;     // max_num_branches_per_seed
+    uint32_t min_cands, max_cands;
+    uint32_t max_skip, max_consec;
+    uint32_t dup_min_len;
+    float chi2_max;
+    // inputs (batch of events)
+    uint32_t n_events;
+    uint32_t n_seeds;
+    uint32_t n_meas;
+    const float2* meas_pos;
+    const float2* meas_var;
+    const uint32_t* meas_info;    // dims | subspace0 << 8 | subspace1 << 16
+    const uint32_t* meas_ranges;  // [n_events][n_surfaces + 1]: event offset, then upper bounds
+    const uint32_t* seed_event;
+    const uint32_t* seed_surface;
+    const float* seed_vec;        // [n_seeds][6]
+    const float* seed_cov;        // [n_seeds][36]
+    // work buffers
+    float* state[2];              // ping-pong propagated states, SoA [28][state_cap]
+    uint32_t state_cap;           // links per step
+    float* flt;                   // filtered states SoA [28][flt_cap], slot = input * N + k
+    uint32_t* cand_meas;          // [flt_cap] measurement of the filtered slot
+    float* cand_chi2;             // [flt_cap] its predicted chi2
+    uint32_t flt_cap;
+    uint32_t in_cap;              // inputs per step
+    dlink* links;
+    uint32_t link_cap;
+    uint32_t* live;               // [in_cap] previous-step link slots of this step's inputs
+    uint32_t* tips;               // [tip_cap] global link indices
+    uint32_t tip_cap;
+    uint32_t* seed_first;         // [n_seeds] local index of the seed's first link this step
+    uint32_t* seed_base;          // [n_seeds] alive-prefix at that link (dedup steps)
+    uint32_t* alive_prefix;       // [state_cap] bit 31 = alive, low bits = exclusive alive prefix
+    uint32_t* dedup_head;         // [n_meas + 1]
+    uint32_t* dedup_next;         // [state_cap]
+    uint32_t* lb_flag[4];         // look-back descriptors per kernel
+    uint64_t* lb_agg[4];
+    uint64_t* lb_inc[4];
+    uint32_t lb_cap;
+    step_ctl* ctl;
+    // build outputs
+    uint32_t* tip_keep;           // [tip_cap] bit 31 kept | n_cands
+    uint32_t* tip_rank;           // [tip_cap] kept prefix
+    uint32_t* tip_cprefix;        // [tip_cap] constituent prefix
+    uint32_t* seg_count;          // [n_events][max_cands] kept tracks per (event, step)
+    uint32_t* seg_ccount;         // [n_events][max_cands] constituents per (event, step)
+    uint32_t* seg_base;           // [n_events][max_cands] kept prefix at segment head
+    uint32_t* seg_cbase;          // [n_events][max_cands] constituent prefix at segment head
+    uint32_t* seg_off;            // [n_events][max_cands] output track offset
+    uint32_t* seg_coff;           // [n_events][max_cands] output constituent offset
+    uint32_t* event_track_off;    // [n_events + 1]
+    uint32_t* event_const_off;    // [n_events + 1]
+    const uint32_t* event_meas_off;  // [n_events]
+    // output arrays
+    uint16_t* out_outcome;
+    uint32_t* out_surface;        // surface index of the seed
+    float* out_vec;               // [n][6]
+    float* out_cov;               // [n][36]
+    float* out_ndf;
+    float* out_chi2;
+    float* out_pval;
+    uint32_t* out_nholes;
+    uint32_t* out_offsets;        // [n + 1]
+    uint32_t* out_const;          // [n_const]
+    uint32_t out_cap;
+    uint32_t out_const_cap;
+};
+
+}  // namespace tamd

@@ -1,0 +1,865 @@
+// CKF step kernels for CDNA (gfx950).
+//
+// Per step (counts live in the device step control block):
+//   k_find        one lane per input branch: chi2 over the surface's
+//                 measurements, best-N selection by (chi2, measurement index),
+//                 gain-matrix update, hole links; links are compacted in input
+//                 order with a wave scan + decoupled look-back
+//                 (traccc host: details/combinatorial_kalman_filter.hpp:174-364)
+//   k_dedup_*     duplicate removal grouped by last measurement
+//                 (traccc host :371-488)
+//   k_propagate   one lane per link: per-seed cap, hole limits, RK4 transport
+//                 to the next sensitive surface, tips; next-step inputs and
+//                 tips compacted with ballots + look-back (traccc host :493-627)
+//   k_advance     one lane: advances the control block
+// build_tracks after the loop (traccc host :637-715).
+//
+// Work distribution: every kernel is a persistent grid whose waves take
+// 64-item partitions from an atomic ticket (device work queue), so uneven
+// per-branch cost (measurements per surface, RK steps per crossing) balances
+// across the chip.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "device_types.hpp"
+#include "kalman_device.hpp"
+#include "kernels.hpp"
+#include "wave.hpp"
+
+namespace tamd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline void load_soa(const float* __restrict__ base, uint32_t cap, uint32_t i, float* x,
+                                float* c, uint32_t& sf) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        x[k] = base[static_cast<size_t>(k) * cap + i];
+    }
+#pragma unroll
+    for (int k = 0; k < 21; ++k) {
+        c[k] = base[static_cast<size_t>(6 + k) * cap + i];
+    }
+    sf = __float_as_uint(base[static_cast<size_t>(27) * cap + i]);
+}
+
+__device__ inline void store_soa(float* __restrict__ base, uint32_t cap, uint32_t i, const float* x,
+                                 const float* c, uint32_t sf) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        base[static_cast<size_t>(k) * cap + i] = x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 21; ++k) {
+        base[static_cast<size_t>(6 + k) * cap + i] = c[k];
+    }
+    base[static_cast<size_t>(27) * cap + i] = __uint_as_float(sf);
+}
+
+__device__ inline void load_seed(const ckf_params& P, uint32_t s, float* x, float* c, uint32_t& sf) {
+    const float* v = P.seed_vec + 6u * static_cast<size_t>(s);
+    const float* m = P.seed_cov + 36u * static_cast<size_t>(s);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        x[k] = v[k];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+            c[sym_idx(i, j)] = m[6 * i + j];
+        }
+    }
+    sf = P.seed_surface[s];
+}
+
+__device__ inline void load_ref(const ckf_params& P, uint32_t ref, uint32_t in_parity, float* x,
+                                float* c, uint32_t& sf) {
+    const uint32_t kind = ref >> kRefKindShift;
+    const uint32_t idx = ref & kRefMask;
+    if (kind == kRefFiltered) {
+        load_soa(P.flt, P.flt_cap, idx, x, c, sf);
+    } else if (kind == kRefInput) {
+        load_soa(P.state[in_parity], P.state_cap, idx, x, c, sf);
+    } else {
+        load_seed(P, idx, x, c, sf);
+    }
+}
+
+// Multiple-scattering noise of the module the state sits on (covariance part
+// of the pointwise material interaction).
+__device__ inline void add_material(const ckf_params& P, uint32_t sf, const float* x, float* c) {
+    const float t = P.sfs[sf].thickness_x0;
+    if (!(t > 0.f)) {
+        return;
+    }
+    const ckf_surface& S = P.sfs[sf];
+    float sp, cp, st, ct;
+    sincosf(x[e_phi], &sp, &cp);
+    sincosf(x[e_theta], &st, &ct);
+    const float cosi = fabsf(cp * st * S.normal[0] + sp * st * S.normal[1] + ct * S.normal[2]);
+    const float t0 = highland_theta0<float>(t / cosi, x[e_qop], P.nav.mass, P.nav.abs_q);
+    const float v = t0 * t0;
+    c[sym_idx(e_phi, e_phi)] += v / (st * st);
+    c[sym_idx(e_theta, e_theta)] += v;
+}
+
+__device__ inline void set_overflow(step_ctl* ctl, uint32_t bit) { atomicOr(&ctl->overflow, bit); }
+
+__device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.links[g]; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// find_tracks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t step = ctl->step;
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t link_base = ctl->link_base;
+    const uint32_t prev_base = ctl->prev_link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t n_parts = (n_in + kWave - 1) / kWave;
+    const uint32_t in_par = (step + 1u) & 1u;
+    const uint32_t N = P.n_branch_surface;
+
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[0]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i = p * kWave + lane();
+        const bool valid = i < n_in;
+        uint32_t count = 0u, seed = 0u, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
+        uint32_t in_slot = 0u;
+        float chi2_prev = 0.f;
+        if (valid) {
+            float x[6], c[21];
+            uint32_t sf = 0u;
+            if (step == 0u) {
+                seed = i;
+                load_seed(P, i, x, c, sf);
+            } else {
+                in_slot = P.live[i];
+                prev_g = prev_base + in_slot;
+                const dlink L = load_link(P, prev_g);
+                seed = L.seed;
+                skip = link_skipped(L);
+                consec = link_consec(L);
+                chi2_prev = L.chi2_sum;
+                ndf_prev = L.ndf_sum;
+                load_soa(P.state[in_par], P.state_cap, in_slot, x, c, sf);
+            }
+            const uint32_t ev = P.seed_event[seed];
+            const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
+            const uint32_t lo = rg[sf];
+            const uint32_t up = rg[sf + 1u];
+            const bool rule0 = (step == 0u) && !(P.sfs[sf].thickness_x0 > 0.f);
+            bool have_last = false;
+            float last_c = 0.f;
+            uint32_t last_m = 0u;
+            while (count < N) {
+                float best_c = 0.f;
+                uint32_t best_m = kInvalid;
+                for (uint32_t m = lo; m < up; ++m) {
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, m);
+                    const float chi2 = predicted_chi2(md, x, c);
+                    if (!(chi2 <= P.chi2_max && chi2 >= 0.f)) {
+                        continue;  // outside the cut (a NaN never yields a link either)
+                    }
+                    if (rule0 && chi2 != 0.f) {
+                        continue;
+                    }
+                    if (have_last && (chi2 < last_c || (chi2 == last_c && m <= last_m))) {
+                        continue;
+                    }
+                    if (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m)) {
+                        best_c = chi2;
+                        best_m = m;
+                    }
+                }
+                if (best_m == kInvalid) {
+                    break;
+                }
+                have_last = true;
+                last_c = best_c;
+                last_m = best_m;
+                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                float fx[6], fc[21];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) fx[k] = x[k];
+#pragma unroll
+                for (int k = 0; k < 21; ++k) fc[k] = c[k];
+                int res = CKF_FIT_SUCCESS;
+                if (rule0) {
+                    // step 0 on a surface without material: keep the seed,
+                    // take the measurement variances
+                    fc[sym_idx(0, 0)] = md.v0;
+                    fc[sym_idx(1, 1)] = md.v1;
+                } else {
+                    res = gain_update(md, fx, fc);
+                }
+                if (res == CKF_FIT_SUCCESS) {
+                    const uint32_t slot = i * N + count;
+                    if (slot < P.flt_cap) {
+                        store_soa(P.flt, P.flt_cap, slot, fx, fc, sf);
+                        P.cand_meas[slot] = best_m;
+                        P.cand_chi2[slot] = best_c;
+                    } else {
+                        set_overflow(ctl, 4u);
+                    }
+                    ++count;
+                }
+            }
+        }
+        const uint32_t n_out = valid ? (count > 0u ? count : 1u) : 0u;
+        const uint32_t incl = wave_inclusive_scan(n_out);
+        const uint32_t wtotal = wave_broadcast(incl, kWave - 1);
+        const uint64_t base = wave_lookback(P.lb_flag[0], P.lb_agg[0], P.lb_inc[0], p, epoch,
+                                            pack2(wtotal, 0u));
+        const uint32_t first = lo32(base) + incl - n_out;
+        if (p == n_parts - 1u && lane() == 0u) {
+            ctl->n_links = lo32(base) + wtotal;
+        }
+        // head of the seed's contiguous run of inputs
+        uint32_t prev_seed = __shfl_up(seed, 1, kWave);
+        if (valid && lane() == 0u && i > 0u) {
+            prev_seed = (step == 0u) ? i - 1u : load_link(P, prev_base + P.live[i - 1u]).seed;
+        }
+        if (valid) {
+            if (i == 0u || prev_seed != seed) {
+                P.seed_first[seed] = first;
+            }
+            if (count == 0u) {
+                dlink L;
+                L.prev = prev_g;
+                L.meas = kInvalid;
+                L.seed = seed;
+                L.packed = pack_link(step, skip + 1u, consec + 1u);
+                L.chi2 = FLT_MAX;
+                L.chi2_sum = chi2_prev;
+                L.ndf_sum = ndf_prev;
+                L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
+                                           : ((kRefInput << kRefKindShift) | in_slot);
+                const uint32_t g = link_base + first;
+                if (g < P.link_cap && first < P.state_cap) {
+                    P.links[g] = L;
+                } else {
+                    set_overflow(ctl, 1u);
+                }
+            } else {
+                for (uint32_t k = 0; k < count; ++k) {
+                    const uint32_t slot = i * N + k;
+                    if (slot >= P.flt_cap) {
+                        break;
+                    }
+                    const uint32_t m = P.cand_meas[slot];
+                    const float chi2 = P.cand_chi2[slot];
+                    dlink L;
+                    L.prev = prev_g;
+                    L.meas = m;
+                    L.seed = seed;
+                    L.packed = pack_link(step, skip, 0u);
+                    L.chi2 = chi2;
+                    L.chi2_sum = chi2_prev + chi2;
+                    L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
+                    L.state_ref = (kRefFiltered << kRefKindShift) | slot;
+                    const uint32_t g = link_base + first + k;
+                    if (g < P.link_cap && first + k < P.state_cap) {
+                        P.links[g] = L;
+                    } else {
+                        set_overflow(ctl, 1u);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// duplicate removal
+// ---------------------------------------------------------------------------
+// Overflow bits raised by k_find (links, filtered states).  Later kernels of
+// the step only read these: the bits k_propagate raises itself must not change
+// the partition count its waves see mid-kernel.
+constexpr uint32_t kFindOverflow = 1u | 4u;
+
+// Links of this step that exist: a step whose find kernel overflowed a buffer
+// is abandoned (the host regrows and re-runs), so nothing walks missing links.
+__device__ inline uint32_t links_this_step(const ckf_params& P, const step_ctl* ctl) {
+    if ((ctl->overflow & kFindOverflow) != 0u) {
+        return 0u;
+    }
+    uint32_t n = ctl->n_links;
+    n = n < P.state_cap ? n : P.state_cap;
+    const uint32_t room = P.link_cap > ctl->link_base ? P.link_cap - ctl->link_base : 0u;
+    return n < room ? n : room;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_clear(ckf_params P) {
+    const step_ctl* ctl = P.ctl;
+    if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
+        return;
+    }
+    const uint32_t n = P.n_meas + 1u;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        P.dedup_head[j] = kInvalid;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dedup_insert(ckf_params P) {
+    const step_ctl* ctl = P.ctl;
+    if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
+        return;
+    }
+    const uint32_t n = links_this_step(P, ctl);
+    const uint32_t base = ctl->link_base;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        dlink L = load_link(P, base + i);
+        while (L.meas == kInvalid && link_step(L) != 0u) {
+            L = load_link(P, L.prev);
+        }
+        const uint32_t key = (L.meas == kInvalid) ? P.n_meas : L.meas;
+        P.alive_prefix[i] = key;
+        P.dedup_next[i] = atomicExch(&P.dedup_head[key], i);
+    }
+}
+
+namespace {
+// Is track `a` dominated by `b` (identical measurement history down to a's
+// first link)?  traccc host :431-471.
+__device__ inline bool history_contained(const ckf_params& P, dlink a, dlink b) {
+    for (;;) {
+        while (a.meas == kInvalid && link_step(a) != 0u) a = load_link(P, a.prev);
+        while (b.meas == kInvalid && link_step(b) != 0u) b = load_link(P, b.prev);
+        if (a.meas == b.meas) {
+            if (link_step(a) == 0u) return true;
+            if (link_step(b) == 0u) return false;
+            a = load_link(P, a.prev);
+            b = load_link(P, b.prev);
+        } else {
+            return false;
+        }
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
+        return;
+    }
+    const uint32_t step = ctl->step;
+    const uint32_t n = links_this_step(P, ctl);
+    const uint32_t base = ctl->link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t n_parts = (n + kWave - 1) / kWave;
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[1]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i = p * kWave + lane();
+        const bool valid = i < n;
+        bool alive = true;
+        uint32_t seed = 0u;
+        if (valid) {
+            const dlink Li = load_link(P, base + i);
+            seed = Li.seed;
+            const uint32_t key = P.alive_prefix[i];
+            if (step + 1u - link_skipped(Li) > P.dup_min_len && Li.ndf_sum > 5u) {
+                const float p_this = prob_chi2(Li.chi2_sum, static_cast<float>(Li.ndf_sum - 5u));
+                for (uint32_t j = P.dedup_head[key]; j != kInvalid; j = P.dedup_next[j]) {
+                    if (j == i) continue;
+                    const dlink Lj = load_link(P, base + j);
+                    if (step + 1u - link_skipped(Lj) <= P.dup_min_len || Li.ndf_sum <= 5u) continue;
+                    bool dominated = history_contained(P, Li, Lj);
+                    const float p_that = prob_chi2(Lj.chi2_sum, static_cast<float>(Lj.ndf_sum - 5u));
+                    if (p_this != p_that) {
+                        dominated = dominated && (p_that >= p_this);
+                    } else {
+                        dominated = dominated && (j < i);
+                    }
+                    if (dominated) {
+                        alive = false;
+                        break;
+                    }
+                }
+            }
+        }
+        const uint64_t m = wave_ballot(valid && alive);
+        const uint32_t wtotal = static_cast<uint32_t>(__popcll(m));
+        const uint64_t b = wave_lookback(P.lb_flag[1], P.lb_agg[1], P.lb_inc[1], p, epoch,
+                                         pack2(wtotal, 0u));
+        const uint32_t before = lo32(b) + prefix_count(m);
+        if (valid) {
+            P.alive_prefix[i] = (alive ? 0x80000000u : 0u) | before;
+            if (P.seed_first[seed] == i) {
+                P.seed_base[seed] = before;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// propagate_to_next_surface
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t step = ctl->step;
+    const uint32_t n = links_this_step(P, ctl);
+    const uint32_t base = ctl->link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t tip_base = ctl->n_tips;
+    const uint32_t n_parts = (n + kWave - 1) / kWave;
+    const uint32_t in_par = (step + 1u) & 1u;
+    const uint32_t out_par = step & 1u;
+    const bool dedup = step >= P.dup_min_len;
+
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[2]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i = p * kWave + lane();
+        const bool valid = i < n;
+        bool succ = false;
+        bool propagated = false;
+        uint32_t ntip = 0u;
+        if (valid) {
+            const dlink L = load_link(P, base + i);
+            bool alive = true;
+            uint32_t rank = 0u;
+            if (dedup) {
+                const uint32_t ap = P.alive_prefix[i];
+                alive = (ap >> 31) != 0u;
+                rank = (ap & 0x7fffffffu) - P.seed_base[L.seed];
+            } else {
+                rank = i - P.seed_first[L.seed];
+            }
+            if (alive && rank < P.n_branch_seed) {
+                if (link_skipped(L) > P.max_skip || link_consec(L) > P.max_consec) {
+                    ntip = 1u;
+                } else {
+                    float x[6], c[21];
+                    uint32_t sf = 0u;
+                    load_ref(P, L.state_ref, in_par, x, c, sf);
+                    if (step == 0u) {
+                        add_material(P, sf, x, c);
+                    }
+                    navigator<float, true> nav;
+                    nav.sfs = P.sfs;
+                    nav.layers = P.layers;
+                    nav.n_layers = P.n_layers;
+                    nav.field = P.field;
+                    nav.cfg = P.nav;
+                    bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
+                    uint32_t end_sf = 0u;
+                    propagated = true;
+                    const int status = nav.propagate(static_cast<int>(sf), end_sf);
+                    const bool success = (status == NAV_FOUND);
+                    bool ok = success;
+                    if (success) {
+                        float nx[6];
+                        float Jb[6][6];
+                        free_to_bound<float, true>(P.sfs[end_sf], nav.st, nav.J, P.field, P.nav.mass,
+                                                   P.nav.abs_q, nx, Jb);
+                        // C' = Jb C Jb^T
+                        float M[6][6];
+#pragma unroll
+                        for (int r = 0; r < 6; ++r) {
+#pragma unroll
+                            for (int q = 0; q < 6; ++q) {
+                                float acc = 0.f;
+#pragma unroll
+                                for (int k = 0; k < 6; ++k) acc += Jb[r][k] * c[sym_idx(k, q)];
+                                M[r][q] = acc;
+                            }
+                        }
+                        float nc[21];
+#pragma unroll
+                        for (int r = 0; r < 6; ++r) {
+#pragma unroll
+                            for (int q = r; q < 6; ++q) {
+                                float acc = 0.f;
+#pragma unroll
+                                for (int k = 0; k < 6; ++k) acc += M[r][k] * Jb[q][k];
+                                nc[sym_idx(r, q)] = acc;
+                            }
+                        }
+                        add_material(P, end_sf, nx, nc);
+                        const float th = nx[e_theta];
+                        if (th <= 0.f || th >= 6.28318530717958647692f) ok = false;
+                        if (!isfinite(nx[e_phi])) ok = false;
+                        if (fabsf(nx[e_qop]) == 0.f) ok = false;
+                        if (ok) {
+                            store_soa(P.state[out_par], P.state_cap, i, nx, nc, end_sf);
+                            succ = true;
+                        }
+                    }
+                    if (!ok && step + 1u >= P.min_cands) ++ntip;
+                    if (success && step == P.max_cands - 1u) ++ntip;
+                }
+            }
+        }
+        const uint64_t ms = wave_ballot(succ);
+        const uint64_t m1 = wave_ballot(ntip >= 1u);
+        const uint64_t m2 = wave_ballot(ntip >= 2u);
+        const uint64_t mp = wave_ballot(propagated);
+        const uint32_t s_tot = static_cast<uint32_t>(__popcll(ms));
+        const uint32_t t_tot = static_cast<uint32_t>(__popcll(m1) + __popcll(m2));
+        const uint64_t b = wave_lookback(P.lb_flag[2], P.lb_agg[2], P.lb_inc[2], p, epoch,
+                                         pack2(s_tot, t_tot));
+        if (lane() == 0u && mp != 0ull) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props),
+                      static_cast<unsigned long long>(__popcll(mp)));
+        }
+        if (succ) {
+            const uint32_t pos = lo32(b) + prefix_count(ms);
+            if (pos < P.in_cap) {
+                P.live[pos] = i;
+            } else {
+                set_overflow(ctl, 16u);
+            }
+        }
+        if (ntip > 0u) {
+            uint32_t pos = tip_base + hi32(b) + prefix_count(m1) + prefix_count(m2);
+            for (uint32_t k = 0; k < ntip; ++k, ++pos) {
+                if (pos < P.tip_cap) {
+                    P.tips[pos] = base + i;
+                } else {
+                    set_overflow(ctl, 8u);
+                }
+            }
+        }
+        if (p == n_parts - 1u && lane() == 0u) {
+            ctl->n_live = lo32(b) + s_tot;
+            ctl->n_tips_step = hi32(b) + t_tot;
+        }
+    }
+}
+
+__global__ void k_advance(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t n_links = ctl->n_links;
+    ctl->n_links_total += n_links;
+    ctl->n_tips += ctl->n_tips_step;
+    ctl->n_tips_step = 0u;
+    ctl->prev_link_base = ctl->link_base;
+    ctl->link_base += n_links;
+    ctl->n_in = ctl->n_live;
+    ctl->n_live = 0u;
+    ctl->n_links = 0u;
+    ctl->step += 1u;
+    ctl->n_steps_run = ctl->step;
+    ctl->epoch += 1u;
+    for (int k = 0; k < 4; ++k) ctl->ticket[k] = 0u;
+    if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
+        ctl->done = 1u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// build_tracks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    const uint32_t n = ctl->n_tips < P.tip_cap ? ctl->n_tips : P.tip_cap;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t n_parts = (n + kWave - 1) / kWave;
+    const uint32_t S = P.max_cands;
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[3]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t t = p * kWave + lane();
+        const bool valid = t < n;
+        uint32_t keep = 0u, nc = 0u, ev = 0u, st = 0u;
+        if (valid) {
+            const dlink L = load_link(P, P.tips[t]);
+            st = link_step(L);
+            nc = st + 1u - link_skipped(L);
+            keep = (nc >= P.min_cands && nc <= P.max_cands) ? 1u : 0u;
+            ev = P.seed_event[L.seed];
+        }
+        const uint32_t w = keep ? nc : 0u;
+        const uint32_t cincl = wave_inclusive_scan(w);
+        const uint64_t mk = wave_ballot(keep != 0u);
+        const uint32_t k_tot = static_cast<uint32_t>(__popcll(mk));
+        const uint32_t c_tot = wave_broadcast(cincl, kWave - 1);
+        const uint64_t b = wave_lookback(P.lb_flag[3], P.lb_agg[3], P.lb_inc[3], p, epoch,
+                                         pack2(k_tot, c_tot));
+        if (valid) {
+            const uint32_t kp = lo32(b) + prefix_count(mk);
+            const uint32_t cp = hi32(b) + cincl - w;
+            P.tip_keep[t] = (keep ? 0x80000000u : 0u) | nc;
+            P.tip_rank[t] = kp;
+            P.tip_cprefix[t] = cp;
+            bool head = (t == 0u);
+            if (!head) {
+                const dlink Lp = load_link(P, P.tips[t - 1u]);
+                head = (link_step(Lp) != st) || (P.seed_event[Lp.seed] != ev);
+            }
+            const size_t seg = static_cast<size_t>(ev) * S + st;
+            if (head) {
+                P.seg_base[seg] = kp;
+                P.seg_cbase[seg] = cp;
+            }
+            if (keep) {
+                atomicAdd(&P.seg_count[seg], 1u);
+                atomicAdd(&P.seg_ccount[seg], nc);
+            }
+        }
+    }
+}
+
+// One workgroup: exclusive scan of the (event, step) table, event offsets and
+// totals.
+__global__ __launch_bounds__(1024) void k_build_table(ckf_params P) {
+    __shared__ uint32_t s_a[1024];
+    __shared__ uint32_t s_b[1024];
+    const uint32_t S = P.max_cands;
+    const uint32_t total = P.n_events * S;
+    const uint32_t per = (total + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = lo + per < total ? lo + per : total;
+    uint32_t a = 0u, b = 0u;
+    for (uint32_t k = lo; k < hi; ++k) {
+        a += P.seg_count[k];
+        b += P.seg_ccount[k];
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+        uint32_t va = 0u, vb = 0u;
+        if (threadIdx.x >= d) {
+            va = s_a[threadIdx.x - d];
+            vb = s_b[threadIdx.x - d];
+        }
+        __syncthreads();
+        s_a[threadIdx.x] += va;
+        s_b[threadIdx.x] += vb;
+        __syncthreads();
+    }
+    uint32_t ra = s_a[threadIdx.x] - a;
+    uint32_t rb = s_b[threadIdx.x] - b;
+    for (uint32_t k = lo; k < hi; ++k) {
+        if (k % S == 0u) {
+            P.event_track_off[k / S] = ra;
+            P.event_const_off[k / S] = rb;
+        }
+        P.seg_off[k] = ra;
+        P.seg_coff[k] = rb;
+        ra += P.seg_count[k];
+        rb += P.seg_ccount[k];
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        P.event_track_off[P.n_events] = s_a[threadIdx.x];
+        P.event_const_off[P.n_events] = s_b[threadIdx.x];
+        P.ctl->n_tracks = s_a[threadIdx.x];
+        P.ctl->n_constituents = s_b[threadIdx.x];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
+    const step_ctl* ctl = P.ctl;
+    const uint32_t n = ctl->n_tips < P.tip_cap ? ctl->n_tips : P.tip_cap;
+    const uint32_t S = P.max_cands;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t kv = P.tip_keep[t];
+        if ((kv >> 31) == 0u) {
+            continue;
+        }
+        const uint32_t nc = kv & 0x7fffffffu;
+        dlink L = load_link(P, P.tips[t]);
+        const uint32_t ev = P.seed_event[L.seed];
+        const size_t seg = static_cast<size_t>(ev) * S + link_step(L);
+        const uint32_t pos = P.seg_off[seg] + (P.tip_rank[t] - P.seg_base[seg]);
+        const uint32_t cpos = P.seg_coff[seg] + (P.tip_cprefix[t] - P.seg_cbase[seg]);
+        if (pos >= P.out_cap || cpos + nc > P.out_const_cap) {
+            continue;
+        }
+        const uint32_t moff = P.event_meas_off[ev];
+        float ndf = 0.f, chi2 = 0.f;
+        for (int k = static_cast<int>(nc) - 1; k >= 0; --k) {
+            while (L.meas == kInvalid && link_step(L) != 0u) L = load_link(P, L.prev);
+            P.out_const[cpos + static_cast<uint32_t>(k)] = L.meas - moff;
+            ndf += static_cast<float>(P.meas_info[L.meas] & 0xffu);
+            chi2 += L.chi2;
+            if (k > 0) L = load_link(P, L.prev);
+        }
+        ndf -= 5.f;
+        const uint32_t s = L.seed;
+        P.out_outcome[pos] = 0u;
+        P.out_surface[pos] = P.seed_surface[s];
+        for (int k = 0; k < 6; ++k) P.out_vec[6u * pos + k] = P.seed_vec[6u * s + k];
+        for (int k = 0; k < 36; ++k) P.out_cov[36u * pos + k] = P.seed_cov[36u * s + k];
+        P.out_ndf[pos] = ndf;
+        P.out_chi2[pos] = chi2;
+        P.out_pval[pos] = prob_chi2(chi2, ndf);
+        P.out_nholes[pos] = link_skipped(L);
+        P.out_offsets[pos] = cpos;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.out_offsets[ctl->n_tracks < P.out_cap ? ctl->n_tracks : P.out_cap] = ctl->n_constituents;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: single pieces of the step pipeline on plain arrays
+// ---------------------------------------------------------------------------
+__global__ void k_debug_kf(uint32_t n, const float* vec, const float* cov, const float2* meas,
+                           const float2* var, const uint32_t* info, float* chi2, int32_t* status,
+                           float* out_vec, float* out_cov) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    float x[6], c[21];
+    for (int k = 0; k < 6; ++k) x[k] = vec[6u * i + k];
+    for (int r = 0; r < 6; ++r)
+        for (int q = r; q < 6; ++q) c[sym_idx(r, q)] = cov[36u * i + 6u * r + q];
+    const meas_data md = load_meas(meas, var, info, i);
+    chi2[i] = predicted_chi2(md, x, c);
+    const int st = gain_update(md, x, c);
+    status[i] = st;
+    for (int k = 0; k < 6; ++k) out_vec[6u * i + k] = x[k];
+    for (int r = 0; r < 6; ++r)
+        for (int q = 0; q < 6; ++q) out_cov[36u * i + 6u * r + q] = c[sym_idx(r, q)];
+}
+
+__global__ void k_debug_propagate(ckf_params P, uint32_t n, const uint32_t* start, const float* vec,
+                                  const float* cov, int apply_start_material, int32_t* status,
+                                  uint32_t* end_surface, float* out_vec, float* out_cov, float* path,
+                                  uint32_t* n_steps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    float x[6], c[21];
+    for (int k = 0; k < 6; ++k) x[k] = vec[6u * i + k];
+    for (int r = 0; r < 6; ++r)
+        for (int q = r; q < 6; ++q) c[sym_idx(r, q)] = cov[36u * i + 6u * r + q];
+    const uint32_t sf = start[i];
+    if (apply_start_material) {
+        add_material(P, sf, x, c);
+    }
+    navigator<float, true> nav;
+    nav.sfs = P.sfs;
+    nav.layers = P.layers;
+    nav.n_layers = P.n_layers;
+    nav.field = P.field;
+    nav.cfg = P.nav;
+    bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
+    uint32_t end_sf = 0u;
+    const int st = nav.propagate(static_cast<int>(sf), end_sf);
+    status[i] = st;
+    path[i] = nav.path;
+    n_steps[i] = nav.n_steps;
+    end_surface[i] = end_sf;
+    if (st != NAV_FOUND) {
+        return;
+    }
+    float nx[6];
+    float Jb[6][6];
+    free_to_bound<float, true>(P.sfs[end_sf], nav.st, nav.J, P.field, P.nav.mass, P.nav.abs_q, nx,
+                               Jb);
+    float M[6][6];
+    for (int r = 0; r < 6; ++r)
+        for (int q = 0; q < 6; ++q) {
+            float acc = 0.f;
+            for (int k = 0; k < 6; ++k) acc += Jb[r][k] * c[sym_idx(k, q)];
+            M[r][q] = acc;
+        }
+    float nc[21];
+    for (int r = 0; r < 6; ++r)
+        for (int q = r; q < 6; ++q) {
+            float acc = 0.f;
+            for (int k = 0; k < 6; ++k) acc += M[r][k] * Jb[q][k];
+            nc[sym_idx(r, q)] = acc;
+        }
+    add_material(P, end_sf, nx, nc);
+    for (int k = 0; k < 6; ++k) out_vec[6u * i + k] = nx[k];
+    for (int r = 0; r < 6; ++r)
+        for (int q = 0; q < 6; ++q) out_cov[36u * i + 6u * r + q] = nc[sym_idx(r, q)];
+}
+
+hipError_t launch_debug_kf(uint32_t n, const float* vec, const float* cov, const float2* meas,
+                           const float2* var, const uint32_t* info, float* chi2, int32_t* status,
+                           float* out_vec, float* out_cov, hipStream_t stream) {
+    hipLaunchKernelGGL(k_debug_kf, dim3((n + 255u) / 256u), dim3(256), 0, stream, n, vec, cov, meas,
+                       var, info, chi2, status, out_vec, out_cov);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_t* start,
+                                  const float* vec, const float* cov, int apply_start_material,
+                                  int32_t* status, uint32_t* end_surface, float* out_vec,
+                                  float* out_cov, float* path, uint32_t* n_steps,
+                                  hipStream_t stream) {
+    hipLaunchKernelGGL(k_debug_propagate, dim3((n + 63u) / 64u), dim3(64), 0, stream, P, n, start,
+                       vec, cov, apply_start_material, status, end_surface, out_vec, out_cov, path,
+                       n_steps);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_find, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_dedup_clear, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
+    hipLaunchKernelGGL(k_dedup_insert, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
+    hipLaunchKernelGGL(k_dedup_mark, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_propagate, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(const ckf_params& P, hipStream_t stream) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipError_t e = launch_find(P, s, stream);
+    if (e == hipSuccess) e = launch_dedup(P, s, stream);
+    if (e == hipSuccess) e = launch_propagate(P, s, stream);
+    if (e == hipSuccess) e = launch_advance(P, stream);
+    return e;
+}
+
+hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_build_scan, dim3(s.build_blocks), dim3(kBlock), 0, stream, P);
+    hipLaunchKernelGGL(k_build_table, dim3(1), dim3(1024), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_build_write, dim3(s.build_blocks), dim3(kBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+}  // namespace tamd

@@ -1,0 +1,42 @@
+// Host-side launchers of the CKF kernels (defined in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_types.hpp"
+
+namespace tamd {
+
+struct launch_shape {
+    uint32_t find_blocks;
+    uint32_t dedup_blocks;
+    uint32_t prop_blocks;
+    uint32_t build_blocks;
+};
+
+// One CKF step: find_tracks, duplicate removal (active from
+// duplicate_removal_minimum_length on; kernels early-exit before), propagation
+// to the next surface, and the step advance.  Every count is read from the
+// device step control block, so the sequence can be captured in a graph.
+hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+// the same step, kernel group by kernel group (timed runs)
+hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
+
+// build_tracks: tip scan, (event, step) segment table, output fill.
+hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+
+// diagnostics (include/traccc_amd/ckf_debug.h)
+hipError_t launch_debug_kf(uint32_t n, const float* vec, const float* cov, const float2* meas,
+                           const float2* var, const uint32_t* info, float* chi2, int32_t* status,
+                           float* out_vec, float* out_cov, hipStream_t stream);
+hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_t* start,
+                                  const float* vec, const float* cov, int apply_start_material,
+                                  int32_t* status, uint32_t* end_surface, float* out_vec,
+                                  float* out_cov, float* path, uint32_t* n_steps,
+                                  hipStream_t stream);
+
+}  // namespace tamd

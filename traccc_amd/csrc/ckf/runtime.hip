@@ -1,0 +1,1011 @@
+// Host runtime of the CKF: the C ABI of include/traccc_amd/ckf.h.
+//
+// What it replaces: the orchestration of traccc's device CKF
+// (device/common/src/finding/combinatorial_kalman_filter_algorithm.cpp:52-704),
+// which reads sizes back every step.  Here the step loop runs from a device
+// control block; the host only checks it once per chunk of steps (and replays
+// the chunk as a captured HIP graph), then once more to size the output.
+// Link / tip / state buffers that turn out too small are regrown and the call
+// is re-run (traccc regrows its link buffer, :310-357).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "traccc_amd/ckf.h"
+#include "traccc_amd/ckf_debug.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ckf_error : std::runtime_error {
+    ckf_status code;
+    ckf_error(ckf_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        throw ckf_error(e == hipErrorOutOfMemory ? CKF_ERR_OUT_OF_MEMORY : CKF_ERR_DEVICE,
+                        std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template <typename F>
+ckf_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return CKF_OK;
+    } catch (const ckf_error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return CKF_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CKF_ERR_INTERNAL;
+    }
+}
+
+// Device allocation that grows by doubling and is reused across calls.
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) {
+            return;
+        }
+        if (p != nullptr) {
+            hip_check(hipFree(p), "hipFree");
+            p = nullptr;
+        }
+        size_t nb = std::max<size_t>(need, bytes * 2);
+        nb = (nb + 255) & ~size_t(255);
+        hip_check(hipMalloc(&p, nb), "hipMalloc");
+        bytes = nb;
+    }
+    void release() {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <typename T>
+T* dev_copy(const T* src, size_t n) {
+    T* d = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hip_check(hipMalloc(&d, bytes), "hipMalloc");
+    if (n > 0 && src != nullptr) {
+        hip_check(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    return d;
+}
+
+void validate_config(const ckf_finding_config& c) {
+    // traccc: core/src/finding/combinatorial_kalman_filter_algorithm.cpp:27-31
+    if (c.min_track_candidates_per_track == 0u) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                        "min_track_candidates_per_track must be at least 1");
+    }
+    // traccc device ctor: device/common/src/finding/combinatorial_kalman_filter_algorithm.cpp:33-46
+    if (!(c.min_step_length_for_next_surface > std::fabs(c.propagation.overstep_tolerance))) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                        "min_step_length_for_next_surface must exceed |overstep_tolerance|");
+    }
+    if (c.run_smoother == CKF_SMOOTHER_KALMAN) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                        "a Kalman smoother needs a fitter, which this algorithm does not own");
+    }
+    if (c.run_pkf != 0u) {
+        throw ckf_error(CKF_ERR_UNSUPPORTED, "the progressive Kalman filter is not provided");
+    }
+    if (c.max_num_tracks_per_measurement != 0u) {
+        throw ckf_error(CKF_ERR_UNSUPPORTED, "measurement-vote pruning is not provided");
+    }
+    if (c.max_num_branches_per_surface == 0u || c.max_track_candidates_per_track == 0u ||
+        c.max_track_candidates_per_track > 65535u) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "branch / candidate limits out of range");
+    }
+    if (c.propagation.search_window[0] > 1u || c.propagation.search_window[1] > 1u) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "search_window is limited to 1 bin per side");
+    }
+    if (c.max_step_counts_for_next_surface == 0u || !(std::fabs(c.ptc_charge) > 0.f)) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "invalid propagation settings");
+    }
+}
+
+}  // namespace
+
+struct ckf_device_detector {
+    std::vector<ckf_surface> h_sfs;
+    std::vector<ckf_layer> h_layers;
+    std::vector<uint64_t> ids;
+    ckf_bfield field{};
+    ckf_surface* d_sfs = nullptr;
+    ckf_layer* d_layers = nullptr;
+    float* d_map = nullptr;
+    ~ckf_device_detector() {
+        if (d_sfs) (void)hipFree(d_sfs);
+        if (d_layers) (void)hipFree(d_layers);
+        if (d_map) (void)hipFree(d_map);
+    }
+};
+
+struct ckf_device_event {
+    uint32_t n_events = 0, n_meas = 0, n_seeds = 0, n_surfaces = 0;
+    std::vector<uint32_t> meas_off, seed_off;  // [n_events + 1]
+    float2* d_pos = nullptr;
+    float2* d_var = nullptr;
+    uint32_t* d_info = nullptr;
+    uint32_t* d_ranges = nullptr;
+    uint32_t* d_seed_event = nullptr;
+    uint32_t* d_seed_surface = nullptr;
+    float* d_seed_vec = nullptr;
+    float* d_seed_cov = nullptr;
+    uint32_t* d_meas_off = nullptr;
+    ~ckf_device_event() {
+        void* ps[] = {d_pos, d_var, d_info, d_ranges, d_seed_event, d_seed_surface,
+                      d_seed_vec, d_seed_cov, d_meas_off};
+        for (void* p : ps) {
+            if (p) (void)hipFree(p);
+        }
+    }
+};
+
+struct ckf_device_tracks {
+    uint32_t n_events = 0, n_tracks = 0, n_const = 0;
+    std::vector<uint32_t> ev_track_off, ev_const_off;
+    std::vector<uint64_t> ids;
+    uint16_t* outcome = nullptr;
+    uint32_t* surface = nullptr;
+    float* vec = nullptr;
+    float* cov = nullptr;
+    float* ndf = nullptr;
+    float* chi2 = nullptr;
+    float* pval = nullptr;
+    uint32_t* nholes = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* cons = nullptr;
+    ~ckf_device_tracks() {
+        void* ps[] = {outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons};
+        for (void* p : ps) {
+            if (p) (void)hipFree(p);
+        }
+    }
+};
+
+struct ckf_algorithm {
+    ckf_finding_config cfg{};
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int n_cu = 256;
+    uint32_t epoch_next = 1u;
+    ckf_run_stats stats{};
+    // capacities
+    uint32_t in_cap = 0, state_cap = 0, flt_cap = 0, link_cap = 0, tip_cap = 0, lb_cap = 0;
+    uint32_t seed_cap = 0, meas_cap = 0, seg_cap = 0, ev_cap = 0;
+    // work buffers
+    dbuf state[2], flt, cand_meas, cand_chi2, links, live, tips, seed_first, seed_base;
+    dbuf alive_prefix, dedup_head, dedup_next, lb_flag[4], lb_agg[4], lb_inc[4], ctl;
+    dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
+        seg_coff, ev_toff, ev_coff;
+    tamd::step_ctl* h_ctl = nullptr;  // pinned mirror
+    // captured step chunk
+    hipGraphExec_t graph = nullptr;
+    std::vector<char> graph_key;
+    // per-kernel event timing (diagnostics; the step loop then runs uncaptured)
+    bool timing = false;
+    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0;
+    uint32_t n_prop_launches = 0;
+
+    ~ckf_algorithm() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        dbuf* all[] = {&state[0], &state[1], &flt, &cand_meas, &cand_chi2, &links, &live, &tips,
+                       &seed_first, &seed_base, &alive_prefix, &dedup_head, &dedup_next, &ctl,
+                       &tip_keep, &tip_rank, &tip_cprefix, &seg_count, &seg_ccount, &seg_base,
+                       &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff};
+        for (dbuf* b : all) b->release();
+        for (int k = 0; k < 4; ++k) {
+            lb_flag[k].release();
+            lb_agg[k].release();
+            lb_inc[k].release();
+        }
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+};
+
+namespace {
+
+constexpr uint32_t kStepsPerChunk = 8u;
+
+uint32_t parts(uint32_t n) { return (n + 63u) / 64u; }
+
+void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
+    const uint32_t N = A.cfg.max_num_branches_per_surface;
+    A.flt_cap = static_cast<uint32_t>(std::min<uint64_t>(uint64_t(A.in_cap) * N, 0x3fffffffu));
+    A.state_cap = A.flt_cap;
+    A.lb_cap = std::max({parts(A.in_cap), parts(A.state_cap), parts(A.tip_cap)}) + 1u;
+    const size_t st = size_t(tamd::kStateCols) * 4u;
+    A.state[0].ensure(st * A.state_cap);
+    A.state[1].ensure(st * A.state_cap);
+    A.flt.ensure(st * A.flt_cap);
+    A.cand_meas.ensure(4u * size_t(A.flt_cap));
+    A.cand_chi2.ensure(4u * size_t(A.flt_cap));
+    A.links.ensure(sizeof(tamd::dlink) * size_t(A.link_cap));
+    A.live.ensure(4u * size_t(A.in_cap));
+    A.tips.ensure(4u * size_t(A.tip_cap));
+    A.seed_first.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
+    A.seed_base.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
+    A.alive_prefix.ensure(4u * size_t(A.state_cap));
+    A.dedup_head.ensure(4u * (size_t(ev.n_meas) + 1u));
+    A.dedup_next.ensure(4u * size_t(A.state_cap));
+    for (int k = 0; k < 4; ++k) {
+        const size_t old = A.lb_flag[k].bytes;
+        A.lb_flag[k].ensure(4u * size_t(A.lb_cap));
+        if (A.lb_flag[k].bytes != old) {
+            // fresh descriptors carry epoch 0, which no step uses
+            hip_check(hipMemsetAsync(A.lb_flag[k].p, 0, A.lb_flag[k].bytes, A.stream), "memset");
+        }
+        A.lb_agg[k].ensure(8u * size_t(A.lb_cap));
+        A.lb_inc[k].ensure(8u * size_t(A.lb_cap));
+    }
+    A.ctl.ensure(sizeof(tamd::step_ctl));
+    A.tip_keep.ensure(4u * size_t(A.tip_cap));
+    A.tip_rank.ensure(4u * size_t(A.tip_cap));
+    A.tip_cprefix.ensure(4u * size_t(A.tip_cap));
+    const size_t seg = size_t(ev.n_events) * A.cfg.max_track_candidates_per_track;
+    A.seg_count.ensure(4u * seg);
+    A.seg_ccount.ensure(4u * seg);
+    A.seg_base.ensure(4u * seg);
+    A.seg_cbase.ensure(4u * seg);
+    A.seg_off.ensure(4u * seg);
+    A.seg_coff.ensure(4u * seg);
+    A.ev_toff.ensure(4u * (size_t(ev.n_events) + 1u));
+    A.ev_coff.ensure(4u * (size_t(ev.n_events) + 1u));
+    if (A.h_ctl == nullptr) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
+                                hipHostMallocDefault),
+                  "hipHostMalloc");
+    }
+}
+
+tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
+                             const ckf_device_event& ev) {
+    tamd::ckf_params P;
+    std::memset(&P, 0, sizeof(P));  // padding too: P's bytes key the captured graph
+    const ckf_finding_config& c = A.cfg;
+    P.sfs = D.d_sfs;
+    P.layers = D.d_layers;
+    P.n_layers = static_cast<uint32_t>(D.h_layers.size());
+    P.n_surfaces = static_cast<uint32_t>(D.h_sfs.size());
+    P.field = tamd::make_field_view<float>(D.field, D.d_map);
+    P.nav = tamd::make_nav_config<float>(c);
+    P.n_branch_surface = c.max_num_branches_per_surface;
+    P.n_branch_seed = c.max_num_branches_per_seed;
+    P.min_cands = c.min_track_candidates_per_track;
+    P.max_cands = c.max_track_candidates_per_track;
+    P.max_skip = c.max_num_skipping_per_cand;
+    P.max_consec = c.max_num_consecutive_skipped;
+    P.dup_min_len = c.duplicate_removal_minimum_length;
+    P.chi2_max = c.chi2_max;
+    P.n_events = ev.n_events;
+    P.n_seeds = ev.n_seeds;
+    P.n_meas = ev.n_meas;
+    P.meas_pos = ev.d_pos;
+    P.meas_var = ev.d_var;
+    P.meas_info = ev.d_info;
+    P.meas_ranges = ev.d_ranges;
+    P.seed_event = ev.d_seed_event;
+    P.seed_surface = ev.d_seed_surface;
+    P.seed_vec = ev.d_seed_vec;
+    P.seed_cov = ev.d_seed_cov;
+    P.state[0] = A.state[0].as<float>();
+    P.state[1] = A.state[1].as<float>();
+    P.state_cap = A.state_cap;
+    P.flt = A.flt.as<float>();
+    P.cand_meas = A.cand_meas.as<uint32_t>();
+    P.cand_chi2 = A.cand_chi2.as<float>();
+    P.flt_cap = A.flt_cap;
+    P.in_cap = A.in_cap;
+    P.links = A.links.as<tamd::dlink>();
+    P.link_cap = A.link_cap;
+    P.live = A.live.as<uint32_t>();
+    P.tips = A.tips.as<uint32_t>();
+    P.tip_cap = A.tip_cap;
+    P.seed_first = A.seed_first.as<uint32_t>();
+    P.seed_base = A.seed_base.as<uint32_t>();
+    P.alive_prefix = A.alive_prefix.as<uint32_t>();
+    P.dedup_head = A.dedup_head.as<uint32_t>();
+    P.dedup_next = A.dedup_next.as<uint32_t>();
+    for (int k = 0; k < 4; ++k) {
+        P.lb_flag[k] = A.lb_flag[k].as<uint32_t>();
+        P.lb_agg[k] = A.lb_agg[k].as<uint64_t>();
+        P.lb_inc[k] = A.lb_inc[k].as<uint64_t>();
+    }
+    P.lb_cap = A.lb_cap;
+    P.ctl = A.ctl.as<tamd::step_ctl>();
+    P.tip_keep = A.tip_keep.as<uint32_t>();
+    P.tip_rank = A.tip_rank.as<uint32_t>();
+    P.tip_cprefix = A.tip_cprefix.as<uint32_t>();
+    P.seg_count = A.seg_count.as<uint32_t>();
+    P.seg_ccount = A.seg_ccount.as<uint32_t>();
+    P.seg_base = A.seg_base.as<uint32_t>();
+    P.seg_cbase = A.seg_cbase.as<uint32_t>();
+    P.seg_off = A.seg_off.as<uint32_t>();
+    P.seg_coff = A.seg_coff.as<uint32_t>();
+    P.event_track_off = A.ev_toff.as<uint32_t>();
+    P.event_const_off = A.ev_coff.as<uint32_t>();
+    P.event_meas_off = ev.d_meas_off;
+    return P;
+}
+
+tamd::launch_shape make_shape(const ckf_algorithm& A) {
+    const uint32_t max_blocks = static_cast<uint32_t>(A.n_cu) * 8u;
+    auto blocks = [&](uint32_t cap) {
+        const uint32_t b = (parts(cap) + 3u) / 4u;
+        return std::max(1u, std::min(b, max_blocks));
+    };
+    tamd::launch_shape s;
+    s.find_blocks = blocks(A.in_cap);
+    s.dedup_blocks = blocks(A.state_cap);
+    s.prop_blocks = blocks(A.state_cap);
+    s.build_blocks = blocks(A.tip_cap);
+    return s;
+}
+
+// Step loop with HIP events around every kernel group, one step at a time.
+uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::launch_shape& s) {
+    hipEvent_t ev[5];
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
+    A.t_find = A.t_dedup = A.t_prop = A.t_other = 0.0;
+    A.n_prop_launches = 0u;
+    for (uint32_t k = 0; k < A.cfg.max_track_candidates_per_track + 1u; ++k) {
+        hip_check(hipEventRecord(ev[0], A.stream), "record");
+        hip_check(tamd::launch_find(P, s, A.stream), "find");
+        hip_check(hipEventRecord(ev[1], A.stream), "record");
+        hip_check(tamd::launch_dedup(P, s, A.stream), "dedup");
+        hip_check(hipEventRecord(ev[2], A.stream), "record");
+        hip_check(tamd::launch_propagate(P, s, A.stream), "propagate");
+        hip_check(hipEventRecord(ev[3], A.stream), "record");
+        hip_check(tamd::launch_advance(P, A.stream), "advance");
+        hip_check(hipEventRecord(ev[4], A.stream), "record");
+        hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
+                                 A.stream),
+                  "ctl readback");
+        hip_check(hipStreamSynchronize(A.stream), "sync");
+        float ms[4];
+        for (int j = 0; j < 4; ++j) hip_check(hipEventElapsedTime(&ms[j], ev[j], ev[j + 1]), "elapsed");
+        A.t_find += ms[0];
+        A.t_dedup += ms[1];
+        A.t_prop += ms[2];
+        A.t_other += ms[3];
+        A.n_prop_launches += 1u;
+        if (A.h_ctl->done) break;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    A.epoch_next = A.h_ctl->epoch + 2u;
+    return A.h_ctl->overflow;
+}
+
+// Runs the step loop once; returns the overflow bits (0 = complete).
+uint32_t run_steps(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::launch_shape& s,
+                   const ckf_device_event& ev) {
+    tamd::step_ctl init{};
+    init.epoch = A.epoch_next;
+    init.n_in = ev.n_seeds;
+    init.done = (ev.n_seeds == 0u) ? 1u : 0u;
+    std::memcpy(A.h_ctl, &init, sizeof(init));
+    hip_check(hipMemcpyAsync(P.ctl, A.h_ctl, sizeof(init), hipMemcpyHostToDevice, A.stream),
+              "ctl upload");
+    hip_check(hipStreamSynchronize(A.stream), "sync");
+
+    if (A.timing) {
+        return run_steps_timed(A, P, s);
+    }
+    // capture one chunk of steps; re-capture when the launch parameters change
+    std::vector<char> key(sizeof(P) + sizeof(s));
+    std::memcpy(key.data(), &P, sizeof(P));
+    std::memcpy(key.data() + sizeof(P), &s, sizeof(s));
+    if (A.graph == nullptr || key != A.graph_key) {
+        if (A.graph) {
+            (void)hipGraphExecDestroy(A.graph);
+            A.graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(A.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        for (uint32_t k = 0; k < kStepsPerChunk; ++k) {
+            hip_check(tamd::launch_ckf_step(P, s, A.stream), "step launch");
+        }
+        hip_check(hipStreamEndCapture(A.stream, &g), "end capture");
+        hip_check(hipGraphInstantiate(&A.graph, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        A.graph_key = key;
+    }
+    uint32_t launched = 0u;
+    for (;;) {
+        hip_check(hipGraphLaunch(A.graph, A.stream), "graph launch");
+        launched += kStepsPerChunk;
+        hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
+                                 A.stream),
+                  "ctl readback");
+        hip_check(hipStreamSynchronize(A.stream), "step sync");
+        if (A.h_ctl->done || launched >= A.cfg.max_track_candidates_per_track + kStepsPerChunk) {
+            break;
+        }
+    }
+    A.epoch_next = A.h_ctl->epoch + 2u;
+    return A.h_ctl->overflow;
+}
+
+// Grow the buffers that overflowed: at least 4x, and at least twice what the
+// aborted step already needed (the control block holds the true counts).
+void grow(ckf_algorithm& A, uint32_t overflow) {
+    const tamd::step_ctl& c = *A.h_ctl;
+    auto up = [](uint32_t cur, uint64_t need) {
+        const uint64_t v = std::max<uint64_t>(uint64_t(cur) * 4u, need * 2u);
+        return static_cast<uint32_t>(std::min<uint64_t>(v, 0x3fffffffu));
+    };
+    const uint64_t N = A.cfg.max_num_branches_per_surface;
+    if (overflow & (2u | 4u)) A.in_cap = up(A.in_cap, c.n_in);
+    if (overflow & 16u) A.in_cap = up(A.in_cap, std::max(c.n_in, c.n_live));
+    if (overflow & 1u) {
+        A.link_cap = up(A.link_cap, uint64_t(c.link_base) + c.n_links + uint64_t(c.n_in) * N);
+        A.tip_cap = std::max(A.tip_cap, A.link_cap / 2u);
+    }
+    if (overflow & 8u) A.tip_cap = up(A.tip_cap, uint64_t(c.n_tips) + c.n_tips_step);
+}
+
+ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
+                               const ckf_device_event& ev) {
+    const ckf_finding_config& c = A.cfg;
+    // initial capacities; grown and re-run on overflow
+    const uint32_t ns = std::max(ev.n_seeds, 1u);
+    const uint64_t branchy = std::min<uint64_t>(c.max_num_branches_per_seed,
+                                                uint64_t(c.max_num_branches_per_surface) *
+                                                    c.max_num_branches_per_surface);
+    A.in_cap = std::max<uint32_t>(A.in_cap, static_cast<uint32_t>(std::min<uint64_t>(
+                                                uint64_t(ns) * std::max<uint64_t>(branchy, 1u),
+                                                0x3fffffffu)));
+    A.link_cap = std::max<uint32_t>(A.link_cap, static_cast<uint32_t>(std::min<uint64_t>(
+                                                    uint64_t(ns) * c.initial_links_per_seed,
+                                                    0x3fffffffu)));
+    A.tip_cap = std::max<uint32_t>(A.tip_cap, std::max<uint32_t>(ns * 2u, A.link_cap / 4u));
+    A.stats = ckf_run_stats{};
+
+    tamd::ckf_params P{};
+    tamd::launch_shape shape{};
+    for (int attempt = 0;; ++attempt) {
+        ensure_buffers(A, ev);
+        P = make_params(A, D, ev);
+        shape = make_shape(A);
+        const uint32_t ovf = run_steps(A, P, shape, ev);
+        if (ovf == 0u) {
+            break;
+        }
+        if (attempt > 40) {
+            throw ckf_error(CKF_ERR_OUT_OF_MEMORY, "CKF buffers kept overflowing");
+        }
+        grow(A, ovf);
+        A.stats.n_regrows += 1u;
+    }
+    const tamd::step_ctl done = *A.h_ctl;
+    A.stats.n_steps = done.n_steps_run;
+    A.stats.n_links = done.n_links_total;
+    A.stats.n_propagations = done.n_props;
+    A.stats.n_tips = done.n_tips;
+
+    // build_tracks: count, size, fill
+    const size_t seg = size_t(ev.n_events) * c.max_track_candidates_per_track;
+    hip_check(hipMemsetAsync(P.seg_count, 0, 4u * seg, A.stream), "memset");
+    hip_check(hipMemsetAsync(P.seg_ccount, 0, 4u * seg, A.stream), "memset");
+    hip_check(hipMemsetAsync(&P.ctl->ticket[3], 0, 4u, A.stream), "memset");
+    hip_check(tamd::launch_build_count(P, shape, A.stream), "build count");
+    hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
+                             A.stream),
+              "ctl readback");
+    hip_check(hipStreamSynchronize(A.stream), "build sync");
+    A.epoch_next = A.h_ctl->epoch + 2u;
+
+    auto* T = new ckf_device_tracks();
+    try {
+        const uint32_t nt = A.h_ctl->n_tracks;
+        const uint32_t nc = A.h_ctl->n_constituents;
+        T->n_events = ev.n_events;
+        T->n_tracks = nt;
+        T->n_const = nc;
+        T->ids = D.ids;
+        const size_t n1 = std::max<size_t>(nt, 1);
+        hip_check(hipMalloc(&T->outcome, 2u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->surface, 4u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->vec, 24u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->cov, 144u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->ndf, 4u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->chi2, 4u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->pval, 4u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->nholes, 4u * n1), "hipMalloc");
+        hip_check(hipMalloc(&T->offsets, 4u * (size_t(nt) + 1u)), "hipMalloc");
+        hip_check(hipMalloc(&T->cons, 4u * std::max<size_t>(nc, 1)), "hipMalloc");
+        P.out_outcome = T->outcome;
+        P.out_surface = T->surface;
+        P.out_vec = T->vec;
+        P.out_cov = T->cov;
+        P.out_ndf = T->ndf;
+        P.out_chi2 = T->chi2;
+        P.out_pval = T->pval;
+        P.out_nholes = T->nholes;
+        P.out_offsets = T->offsets;
+        P.out_const = T->cons;
+        P.out_cap = nt;
+        P.out_const_cap = nc;
+        hip_check(tamd::launch_build_write(P, shape, A.stream), "build write");
+        T->ev_track_off.resize(ev.n_events + 1u);
+        T->ev_const_off.resize(ev.n_events + 1u);
+        hip_check(hipMemcpyAsync(T->ev_track_off.data(), P.event_track_off,
+                                 4u * (ev.n_events + 1u), hipMemcpyDeviceToHost, A.stream),
+                  "offsets");
+        hip_check(hipMemcpyAsync(T->ev_const_off.data(), P.event_const_off,
+                                 4u * (ev.n_events + 1u), hipMemcpyDeviceToHost, A.stream),
+                  "offsets");
+        hip_check(hipStreamSynchronize(A.stream), "build write sync");
+    } catch (...) {
+        delete T;
+        throw;
+    }
+    return T;
+}
+
+ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measurements* meas,
+                               const ckf_bound_params* seeds, uint32_t n_events) {
+    if (meas == nullptr || seeds == nullptr || n_events == 0u) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null event input");
+    }
+    const uint32_t S = static_cast<uint32_t>(D.h_sfs.size());
+    auto* E = new ckf_device_event();
+    try {
+        E->n_events = n_events;
+        E->n_surfaces = S;
+        E->meas_off.assign(n_events + 1u, 0u);
+        E->seed_off.assign(n_events + 1u, 0u);
+        for (uint32_t e = 0; e < n_events; ++e) {
+            E->meas_off[e + 1] = E->meas_off[e] + meas[e].size;
+            E->seed_off[e + 1] = E->seed_off[e] + seeds[e].size;
+        }
+        const uint32_t nm = E->meas_off[n_events];
+        const uint32_t ns = E->seed_off[n_events];
+        E->n_meas = nm;
+        E->n_seeds = ns;
+        std::vector<float2> pos(nm), var(nm);
+        std::vector<uint32_t> info(nm), ranges(size_t(n_events) * (S + 1u));
+        std::vector<uint32_t> seed_event(ns), seed_surface(ns);
+        std::vector<float> seed_vec(size_t(ns) * 6u), seed_cov(size_t(ns) * 36u);
+        for (uint32_t e = 0; e < n_events; ++e) {
+            const ckf_measurements& M = meas[e];
+            const uint32_t off = E->meas_off[e];
+            if (M.size > 0u && (M.local_position == nullptr || M.local_variance == nullptr ||
+                                M.dimensions == nullptr || M.surface_link == nullptr ||
+                                M.subspace == nullptr)) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "missing measurement column");
+            }
+            for (uint32_t i = 0; i < M.size; ++i) {
+                if (i > 0u && M.surface_link[i] < M.surface_link[i - 1u]) {
+                    throw ckf_error(CKF_ERR_PRECONDITION,
+                                    "measurements are not grouped by surface");
+                }
+                const uint32_t d = M.dimensions[i];
+                if (d != 1u && d != 2u) {
+                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "measurement dimension must be 1 or 2");
+                }
+                const uint32_t s0 = M.subspace[2u * i], s1 = M.subspace[2u * i + 1u];
+                if (s0 > 1u || s1 > 5u) {
+                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "unsupported measurement subspace");
+                }
+                pos[off + i] = make_float2(M.local_position[2u * i], M.local_position[2u * i + 1u]);
+                var[off + i] = make_float2(M.local_variance[2u * i], M.local_variance[2u * i + 1u]);
+                info[off + i] = d | (s0 << 8) | (s1 << 16);
+            }
+            uint32_t* row = ranges.data() + size_t(e) * (S + 1u);
+            row[0] = off;
+            for (uint32_t s = 0; s < S; ++s) {
+                const uint64_t id = D.ids[s];
+                const uint64_t* b = M.surface_link;
+                const uint32_t ub = static_cast<uint32_t>(std::upper_bound(b, b + M.size, id) - b);
+                row[s + 1u] = off + ub;
+            }
+            const ckf_bound_params& B = seeds[e];
+            if (B.size > 0u && (B.surface_link == nullptr || B.vector == nullptr ||
+                                B.covariance == nullptr)) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "missing seed column");
+            }
+            for (uint32_t i = 0; i < B.size; ++i) {
+                const uint32_t g = E->seed_off[e] + i;
+                seed_event[g] = e;
+                const auto it = std::lower_bound(D.ids.begin(), D.ids.end(), B.surface_link[i]);
+                if (it == D.ids.end() || *it != B.surface_link[i]) {
+                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "seed on an unknown surface");
+                }
+                seed_surface[g] = static_cast<uint32_t>(it - D.ids.begin());
+                std::memcpy(&seed_vec[6u * size_t(g)], B.vector + 6u * size_t(i), 24u);
+                std::memcpy(&seed_cov[36u * size_t(g)], B.covariance + 36u * size_t(i), 144u);
+            }
+        }
+        E->d_pos = dev_copy(pos.data(), nm);
+        E->d_var = dev_copy(var.data(), nm);
+        E->d_info = dev_copy(info.data(), nm);
+        E->d_ranges = dev_copy(ranges.data(), ranges.size());
+        E->d_seed_event = dev_copy(seed_event.data(), ns);
+        E->d_seed_surface = dev_copy(seed_surface.data(), ns);
+        E->d_seed_vec = dev_copy(seed_vec.data(), seed_vec.size());
+        E->d_seed_cov = dev_copy(seed_cov.data(), seed_cov.size());
+        E->d_meas_off = dev_copy(E->meas_off.data(), n_events);
+    } catch (...) {
+        delete E;
+        throw;
+    }
+    return E;
+}
+
+ckf_device_detector* upload_detector(const ckf_detector* det, const ckf_bfield* field) {
+    if (det == nullptr || field == nullptr || det->surfaces == nullptr || det->layers == nullptr) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null detector or field");
+    }
+    if (det->n_layers > 64u) {
+        throw ckf_error(CKF_ERR_UNSUPPORTED, "at most 64 navigation layers are supported");
+    }
+    if (field->kind != CKF_BFIELD_CONSTANT && field->kind != CKF_BFIELD_RZ_MAP) {
+        throw ckf_error(CKF_ERR_UNSUPPORTED, "unsupported magnetic field type");
+    }
+    if (field->kind == CKF_BFIELD_RZ_MAP &&
+        (field->map == nullptr || field->n_r < 2u || field->n_z < 2u || !(field->dr > 0.f) ||
+         !(field->dz > 0.f))) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "invalid field map");
+    }
+    auto* D = new ckf_device_detector();
+    try {
+        D->h_sfs.assign(det->surfaces, det->surfaces + det->n_surfaces);
+        D->h_layers.assign(det->layers, det->layers + det->n_layers);
+        D->ids.resize(det->n_surfaces);
+        for (uint32_t i = 0; i < det->n_surfaces; ++i) {
+            D->ids[i] = det->surfaces[i].identifier;
+            if (i > 0u && D->ids[i] <= D->ids[i - 1u]) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                                "surface identifiers must increase with the index");
+            }
+            if (det->surfaces[i].layer >= det->n_layers) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "surface refers to an unknown layer");
+            }
+        }
+        D->field = *field;
+        D->d_sfs = dev_copy(det->surfaces, det->n_surfaces);
+        D->d_layers = dev_copy(det->layers, det->n_layers);
+        if (field->kind == CKF_BFIELD_RZ_MAP) {
+            D->d_map = dev_copy(field->map, size_t(2u) * field->n_r * field->n_z);
+            D->field.map = D->d_map;
+        }
+    } catch (...) {
+        delete D;
+        throw;
+    }
+    return D;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t ckf_abi_version(void) { return CKF_ABI_VERSION; }
+
+const char* ckf_last_error(void) { return g_last_error.c_str(); }
+
+ckf_status ckf_algorithm_create(const ckf_finding_config* cfg, ckf_algorithm** out) {
+    return guarded([&] {
+        if (cfg == nullptr || out == nullptr) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        validate_config(*cfg);
+        auto* A = new ckf_algorithm();
+        A->cfg = *cfg;
+        // a stream of our own (default priority): the legacy null stream
+        // cannot be captured into a graph
+        if (hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking) != hipSuccess) {
+            delete A;
+            throw ckf_error(CKF_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        A->stream = A->own_stream;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) {
+                A->n_cu = prop.multiProcessorCount;
+            }
+        }
+        *out = A;
+    });
+}
+
+ckf_status ckf_algorithm_destroy(ckf_algorithm* alg) {
+    return guarded([&] { delete alg; });
+}
+
+ckf_status ckf_algorithm_set_stream(ckf_algorithm* alg, void* hip_stream) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        alg->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : alg->own_stream;
+        if (alg->graph) {
+            (void)hipGraphExecDestroy(alg->graph);
+            alg->graph = nullptr;
+        }
+    });
+}
+
+ckf_status ckf_algorithm_last_stats(const ckf_algorithm* alg, ckf_run_stats* stats) {
+    return guarded([&] {
+        if (alg == nullptr || stats == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null");
+        *stats = alg->stats;
+    });
+}
+
+ckf_status ckf_device_detector_create(const ckf_detector* det, const ckf_bfield* field,
+                                      ckf_device_detector** out) {
+    return guarded([&] {
+        if (out == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null out");
+        *out = upload_detector(det, field);
+    });
+}
+
+ckf_status ckf_device_detector_destroy(ckf_device_detector* det) {
+    return guarded([&] { delete det; });
+}
+
+ckf_status ckf_device_event_create(const ckf_device_detector* det, const ckf_measurements* meas,
+                                   const ckf_bound_params* seeds, uint32_t n_events,
+                                   ckf_device_event** out) {
+    return guarded([&] {
+        if (det == nullptr || out == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null");
+        *out = upload_event(*det, meas, seeds, n_events);
+    });
+}
+
+ckf_status ckf_device_event_destroy(ckf_device_event* ev) {
+    return guarded([&] { delete ev; });
+}
+
+ckf_status ckf_find_tracks(ckf_algorithm* alg, const ckf_device_detector* det,
+                           const ckf_device_event* ev, ckf_device_tracks** out) {
+    return guarded([&] {
+        if (alg == nullptr || det == nullptr || ev == nullptr || out == nullptr) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        if (ev->n_surfaces != det->h_sfs.size()) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "event was uploaded for another detector");
+        }
+        *out = find_tracks(*alg, *det, *ev);
+    });
+}
+
+ckf_status ckf_find_tracks_host(ckf_algorithm* alg, const ckf_detector* det,
+                                const ckf_bfield* field, const ckf_measurements* meas,
+                                const ckf_bound_params* seeds, ckf_device_tracks** out) {
+    return guarded([&] {
+        if (alg == nullptr || out == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null");
+        ckf_device_detector* D = upload_detector(det, field);
+        ckf_device_event* E = nullptr;
+        try {
+            E = upload_event(*D, meas, seeds, 1u);
+            *out = find_tracks(*alg, *D, *E);
+        } catch (...) {
+            delete E;
+            delete D;
+            throw;
+        }
+        delete E;
+        delete D;
+    });
+}
+
+ckf_status ckf_tracks_size(const ckf_device_tracks* trk, uint32_t* n_tracks,
+                           uint32_t* n_constituents) {
+    return guarded([&] {
+        if (trk == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null tracks");
+        if (n_tracks) *n_tracks = trk->n_tracks;
+        if (n_constituents) *n_constituents = trk->n_const;
+    });
+}
+
+ckf_status ckf_tracks_event_offsets(const ckf_device_tracks* trk, uint32_t* n_events,
+                                    uint32_t* track_offsets, uint32_t* constituent_offsets) {
+    return guarded([&] {
+        if (trk == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null tracks");
+        if (n_events) *n_events = trk->n_events;
+        if (track_offsets) {
+            std::memcpy(track_offsets, trk->ev_track_off.data(), 4u * trk->ev_track_off.size());
+        }
+        if (constituent_offsets) {
+            std::memcpy(constituent_offsets, trk->ev_const_off.data(), 4u * trk->ev_const_off.size());
+        }
+    });
+}
+
+ckf_status ckf_tracks_download(const ckf_device_tracks* trk, ckf_track_candidates* out) {
+    return guarded([&] {
+        if (trk == nullptr || out == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null");
+        const size_t nt = trk->n_tracks, nc = trk->n_const;
+        out->n_tracks = trk->n_tracks;
+        out->n_constituents = trk->n_const;
+        auto d2h = [](void* dst, const void* src, size_t bytes) {
+            if (dst != nullptr && bytes > 0) {
+                hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+            }
+        };
+        d2h(out->fit_outcome, trk->outcome, 2u * nt);
+        d2h(out->params_vector, trk->vec, 24u * nt);
+        d2h(out->params_covariance, trk->cov, 144u * nt);
+        d2h(out->ndf, trk->ndf, 4u * nt);
+        d2h(out->chi2, trk->chi2, 4u * nt);
+        d2h(out->pval, trk->pval, 4u * nt);
+        d2h(out->nholes, trk->nholes, 4u * nt);
+        d2h(out->constituent_index, trk->cons, 4u * nc);
+        if (out->constituent_offsets != nullptr) {
+            // offsets index the whole (all-event) constituent array
+            d2h(out->constituent_offsets, trk->offsets, 4u * (nt + 1u));
+        }
+        if (out->params_surface_link != nullptr && nt > 0) {
+            std::vector<uint32_t> sf(nt);
+            d2h(sf.data(), trk->surface, 4u * nt);
+            for (size_t i = 0; i < nt; ++i) out->params_surface_link[i] = trk->ids[sf[i]];
+        }
+        if (out->constituent_type != nullptr) {
+            std::fill(out->constituent_type, out->constituent_type + nc, uint16_t(0));
+        }
+    });
+}
+
+ckf_status ckf_tracks_destroy(ckf_device_tracks* trk) {
+    return guarded([&] { delete trk; });
+}
+
+
+}  // extern "C"
+
+namespace {
+// scoped device copy of a host array; also the destination of downloads
+template <typename T>
+struct scoped_dev {
+    T* p = nullptr;
+    size_t n = 0;
+    scoped_dev(const T* src, size_t count) : n(count) {
+        p = dev_copy(src, count);
+    }
+    explicit scoped_dev(size_t count) : n(count) { p = dev_copy<T>(nullptr, count); }
+    ~scoped_dev() {
+        if (p) (void)hipFree(p);
+    }
+    void download(T* dst) const {
+        if (dst != nullptr && n > 0) {
+            hip_check(hipMemcpy(dst, p, n * sizeof(T), hipMemcpyDeviceToHost), "download");
+        }
+    }
+};
+}  // namespace
+
+extern "C" {
+
+ckf_status ckf_debug_kf_update(uint32_t n, const float* vec, const float* cov, const float* meas,
+                               const float* var, const uint32_t* dims, const uint8_t* subspace,
+                               float* chi2, int32_t* status, float* out_vec, float* out_cov) {
+    return guarded([&] {
+        if (n == 0u) return;
+        if (!vec || !cov || !meas || !var || !dims || !subspace || !chi2 || !status || !out_vec ||
+            !out_cov) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        std::vector<float2> pos(n), vv(n);
+        std::vector<uint32_t> info(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            pos[i] = make_float2(meas[2u * i], meas[2u * i + 1u]);
+            vv[i] = make_float2(var[2u * i], var[2u * i + 1u]);
+            if ((dims[i] != 1u && dims[i] != 2u) || subspace[2u * i] > 1u || subspace[2u * i + 1u] > 5u) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "bad measurement");
+            }
+            info[i] = dims[i] | (uint32_t(subspace[2u * i]) << 8) | (uint32_t(subspace[2u * i + 1u]) << 16);
+        }
+        scoped_dev<float> dv(vec, 6u * size_t(n)), dc(cov, 36u * size_t(n));
+        scoped_dev<float2> dp(pos.data(), n), dva(vv.data(), n);
+        scoped_dev<uint32_t> di(info.data(), n);
+        scoped_dev<float> dchi(n), dov(6u * size_t(n)), doc(36u * size_t(n));
+        scoped_dev<int32_t> dst(n);
+        hip_check(tamd::launch_debug_kf(n, dv.p, dc.p, dp.p, dva.p, di.p, dchi.p, dst.p, dov.p,
+                                        doc.p, nullptr),
+                  "debug kf");
+        hip_check(hipDeviceSynchronize(), "sync");
+        dchi.download(chi2);
+        dst.download(status);
+        dov.download(out_vec);
+        doc.download(out_cov);
+    });
+}
+
+ckf_status ckf_debug_propagate(const ckf_finding_config* cfg, const ckf_detector* det,
+                               const ckf_bfield* field, uint32_t n, const uint32_t* start,
+                               const float* vec, const float* cov, int apply_start_material,
+                               int32_t* status, uint32_t* end_surface, float* out_vec,
+                               float* out_cov, float* path, uint32_t* n_steps) {
+    return guarded([&] {
+        if (cfg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null config");
+        validate_config(*cfg);
+        if (n == 0u) return;
+        if (!start || !vec || !cov || !status || !end_surface || !out_vec || !out_cov || !path ||
+            !n_steps) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        ckf_device_detector* D = upload_detector(det, field);
+        try {
+            for (uint32_t i = 0; i < n; ++i) {
+                if (start[i] >= D->h_sfs.size()) {
+                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "start surface out of range");
+                }
+            }
+            ckf_algorithm A;
+            A.cfg = *cfg;
+            ckf_device_event E;
+            tamd::ckf_params P = make_params(A, *D, E);
+            scoped_dev<uint32_t> ds(start, n);
+            scoped_dev<float> dv(vec, 6u * size_t(n)), dc(cov, 36u * size_t(n));
+            scoped_dev<int32_t> dst(n);
+            scoped_dev<uint32_t> des(n), dns(n);
+            scoped_dev<float> dov(6u * size_t(n)), doc(36u * size_t(n)), dpath(n);
+            hip_check(tamd::launch_debug_propagate(P, n, ds.p, dv.p, dc.p, apply_start_material,
+                                                   dst.p, des.p, dov.p, doc.p, dpath.p, dns.p,
+                                                   nullptr),
+                      "debug propagate");
+            hip_check(hipDeviceSynchronize(), "sync");
+            dst.download(status);
+            des.download(end_surface);
+            dov.download(out_vec);
+            doc.download(out_cov);
+            dpath.download(path);
+            dns.download(n_steps);
+        } catch (...) {
+            delete D;
+            throw;
+        }
+        delete D;
+    });
+}
+
+
+ckf_status ckf_debug_set_kernel_timing(ckf_algorithm* alg, int enable) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        alg->timing = enable != 0;
+    });
+}
+
+ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, double* ms_dedup,
+                                  double* ms_propagate, double* ms_other, uint32_t* n_steps) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        if (ms_find) *ms_find = alg->t_find;
+        if (ms_dedup) *ms_dedup = alg->t_dedup;
+        if (ms_propagate) *ms_propagate = alg->t_prop;
+        if (ms_other) *ms_other = alg->t_other;
+        if (n_steps) *n_steps = alg->n_prop_launches;
+    });
+}
+
+}  // extern "C"

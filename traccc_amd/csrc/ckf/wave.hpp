@@ -1,0 +1,112 @@
+// Wavefront (64-lane) primitives for branch compaction on CDNA:
+//  * ballot/mbcnt prefix counts for 0/1 flags,
+//  * DPP-free shuffle scans for small integer counts,
+//  * a per-wave work queue (one atomic ticket per 64 items),
+//  * decoupled look-back across waves (Merrill & Garland, "Single-pass
+//    Parallel Prefix Scan with Decoupled Look-back", NVR-2016-002), with
+//    separate aggregate / inclusive slots and an epoch tag so descriptor
+//    arrays are reused by every step without clearing.
+// Waves never synchronise with each other except through the look-back,
+// whose predecessor chain always ends at partition 0; tickets are handed out
+// in increasing order, so every predecessor belongs to a running wave.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tamd {
+
+constexpr int kWave = 64;
+
+__device__ inline uint32_t lane() { return __lane_id(); }
+
+__device__ inline uint32_t prefix_count(uint64_t mask) {
+    // number of set bits of `mask` in lanes below this one
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ inline uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+// Inclusive wave scan of a u32.
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+    const uint32_t l = lane();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, kWave);
+        if (l >= static_cast<uint32_t>(d)) {
+            v += o;
+        }
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_broadcast(uint32_t v, int src) { return __shfl(v, src, kWave); }
+
+// Next work partition for the calling wave.
+__device__ inline uint32_t wave_ticket(uint32_t* counter) {
+    uint32_t t = 0;
+    if (lane() == 0) {
+        t = atomicAdd(counter, 1u);
+    }
+    return wave_broadcast(t, 0);
+}
+
+__device__ inline uint32_t ld_acquire(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_relaxed64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_relaxed64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two 32-bit counters packed in one u64 (sums never carry across halves).
+__device__ inline uint64_t pack2(uint32_t lo, uint32_t hi) {
+    return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+__device__ inline uint32_t lo32(uint64_t v) { return static_cast<uint32_t>(v); }
+__device__ inline uint32_t hi32(uint64_t v) { return static_cast<uint32_t>(v >> 32); }
+
+// Exclusive prefix of partition p's packed aggregate over partitions 0..p-1.
+// Called by all lanes of the wave; lane 0 publishes and looks back.
+__device__ inline uint64_t wave_lookback(uint32_t* flags, uint64_t* agg_slot, uint64_t* inc_slot,
+                                         uint32_t p, uint32_t epoch, uint64_t agg) {
+    uint64_t excl = 0;
+    if (lane() == 0) {
+        const uint32_t tag = epoch << 2;
+        if (p == 0u) {
+            st_relaxed64(&inc_slot[0], agg);
+            st_release(&flags[0], tag | 2u);
+        } else {
+            st_relaxed64(&agg_slot[p], agg);
+            st_release(&flags[p], tag | 1u);
+            uint32_t q = p - 1u;
+            for (;;) {
+                const uint32_t f = ld_acquire(&flags[q]);
+                if ((f >> 2) != epoch || (f & 3u) == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if ((f & 3u) == 2u) {
+                    excl += ld_relaxed64(&inc_slot[q]);
+                    break;
+                }
+                excl += ld_relaxed64(&agg_slot[q]);
+                --q;
+            }
+            st_relaxed64(&inc_slot[p], excl + agg);
+            st_release(&flags[p], tag | 2u);
+        }
+    }
+    const uint32_t elo = wave_broadcast(lo32(excl), 0);
+    const uint32_t ehi = wave_broadcast(hi32(excl), 0);
+    return pack2(elo, ehi);
+}
+
+}  // namespace tamd

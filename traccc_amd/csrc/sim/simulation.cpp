@@ -333,7 +333,7 @@ int tamd_detector_build(uint32_t preset, float thickness_x0, uint32_t telescope_
             return CKF_ERR_INVALID_ARGUMENT;
         }
         for (uint32_t k = 0; k < telescope_planes; ++k) {
-            b.telescope_plane(10.0 + telescope_spacing * k, 500.0, x0);
+            b.telescope_plane(telescope_spacing * (k + 1u), 100000.0, x0);
         }
     } else if (preset <= TAMD_DET_BARREL5_DISK6) {
         std::vector<double> radii = {32.0, 72.0, 116.0, 172.0};
@@ -517,6 +517,7 @@ int tamd_event_generate(const ckf_detector* det, const ckf_bfield* field,
         pq[ip] = static_cast<float>(charge);
 
         nav.st.pos = {vtx[0], vtx[1], vtx[2]};
+        nav.st.lo = {0.0, 0.0, 0.0};
         nav.st.dir = {dir[0], dir[1], dir[2]};
         nav.st.qop = charge / p;
         nav.st.time = 0.0;
@@ -530,7 +531,7 @@ int tamd_event_generate(const ckf_detector* det, const ckf_bfield* field,
                 break;
             }
             const ckf_surface& S = det->surfaces[sf];
-            const tamd::vec3<double> x = nav.st.pos - tamd::sf_center<double>(S);
+            const tamd::vec3<double> x = tamd::rel_to(nav.st, tamd::sf_center<double>(S));
             hit_record h;
             h.sf = sf;
             h.l0 = tamd::dot(x, tamd::sf_u<double>(S));
