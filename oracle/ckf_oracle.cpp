@@ -404,14 +404,34 @@ struct propagation {
         if (!(se < static_cast<S>(1e29))) se = S(0);
         const v3<S> at = pos() + scale3(se, dir());
         const int nphi = static_cast<int>(L.n_phi), nlong = static_cast<int>(L.n_long);
+        struct cand {
+            S s;
+            uint32_t idx;
+        };
+        cand plane_cands[9];
+        size_t n_plane = 0;
         int cphi = 0, clong = 0;
         int wphi = static_cast<int>(cfg.propagation.search_window[0]);
         int wlong = L.kind == CKF_LAYER_BARREL ? static_cast<int>(cfg.propagation.search_window[1]) : 0;
         if (L.kind == CKF_LAYER_PLANE) {
-            wphi = (nphi - 1) / 2;
-            wlong = (nlong - 1) / 2;
-            cphi = wphi;
-            clong = wlong;
+            // every module of a plane layer (up to 9) is a candidate
+            for (int k = 0; k < nphi * nlong && k < 9; ++k) {
+                const uint32_t m = L.first_surface + static_cast<uint32_t>(k);
+                if (static_cast<int>(m) == skip) continue;
+                const ckf_surface& sf = det.surfaces[m];
+                const v3<S> w = fl3<S>(sf.normal);
+                const S cosw = dot3(dir(), w);
+                if (std::fabs(cosw) < static_cast<S>(1e-6)) continue;
+                const S s = dot3(fl3<S>(sf.center) - pos(), w) / cosw;
+                if (s < static_cast<S>(cfg.propagation.overstep_tolerance)) continue;
+                const v3<S> rel = pos() + scale3(s, dir()) - fl3<S>(sf.center);
+                const S tol = std::min(static_cast<S>(cfg.propagation.min_mask_tolerance) +
+                                           static_cast<S>(cfg.propagation.mask_tolerance_scalor) * std::fabs(s),
+                                       static_cast<S>(cfg.propagation.max_mask_tolerance));
+                if (!in_mask<S>(sf, dot3(rel, fl3<S>(sf.axis_u)), dot3(rel, fl3<S>(sf.axis_v)), tol)) continue;
+                plane_cands[n_plane++] = {s, m};
+            }
+            wphi = -1;  // no window scan below
         } else {
             const S ph = std::atan2(at.a[1][0], at.a[0][0]);
             const S pitch = static_cast<S>(2.0 * kPi) / static_cast<S>(nphi);
@@ -423,13 +443,10 @@ struct propagation {
                 clong = std::min(std::max(clong, 0), nlong - 1);
             }
         }
-        if (2 * wphi + 1 > nphi) wphi = (nphi - 1) / 2;
-        struct cand {
-            S s;
-            uint32_t idx;
-        };
+        if (wphi >= 0 && 2 * wphi + 1 > nphi) wphi = (nphi - 1) / 2;
         cand cbuf[9];
         size_t nc = 0;
+        for (size_t k = 0; k < n_plane; ++k) cbuf[nc++] = plane_cands[k];
         for (int a = -wphi; a <= wphi; ++a) {
             const int ip = ((cphi + a) % nphi + nphi) % nphi;
             for (int b = -wlong; b <= wlong; ++b) {

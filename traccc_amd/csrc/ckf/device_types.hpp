@@ -63,6 +63,8 @@ struct step_ctl {
     uint32_t n_tracks;
     uint32_t n_constituents;
     uint32_t prev_link_base;  // global index of the previous step's first link
+    uint32_t exit_count;      // waves of k_propagate that have finished this step
+    uint32_t pad2;
     uint64_t n_links_total;
     uint64_t n_props;
 };
@@ -110,7 +112,7 @@ struct ckf_params {
     uint32_t* seed_first;         // [n_seeds] local index of the seed's first link this step
     uint32_t* seed_base;          // [n_seeds] alive-prefix at that link (dedup steps)
     uint32_t* alive_prefix;       // [state_cap] bit 31 = alive, low bits = exclusive alive prefix
-    uint32_t* dedup_head;         // [n_meas + 1]
+    uint64_t* dedup_head;         // [n_meas + 1] epoch << 32 | link (epoch-tagged list heads)
     uint32_t* dedup_next;         // [state_cap]
     uint32_t* lb_flag[4];         // look-back descriptors per kernel
     uint64_t* lb_agg[4];

@@ -47,20 +47,53 @@ TAMD_HD meas_data load_meas(const float2* pos, const float2* var, const uint32_t
     return m;
 }
 
-TAMD_HD float cov_at(const float* c, int i, int j) { return c[sym_idx(i, j)]; }
+// Measured bound indices are loc0 / loc1 (a, b in {0, 1}; the library
+// rejects other subspaces at upload), so every runtime-indexed access below is
+// a two-way select between registers.
+TAMD_HD float pick01(const float* x, uint32_t k) { return k ? x[1] : x[0]; }
 
-TAMD_HD float predicted_chi2(const meas_data& m, const float* x, const float* c) {
-    const float r0 = m.m0 - x[m.a];
-    const float R00 = cov_at(c, m.a, m.a) + m.v0;
+// Column k (k in {0, 1}) of the packed covariance.
+TAMD_HD void cov_col(const float* c, uint32_t k, float* col) {
+TAMD_UNROLL
+    for (int i = 0; i < 6; ++i) {
+        col[i] = k ? c[sym_idx(i, 1)] : c[sym_idx(i, 0)];
+    }
+}
+
+// The 2x2 block and sub-vector a measurement with subspace (a, b) sees.
+struct proj_view {
+    float x0, x1;       // x[a], x[b]
+    float p00, p01, p11;
+};
+
+TAMD_HD proj_view project(const float* x, const float* c, uint32_t a, uint32_t b) {
+    proj_view v;
+    const float c00 = c[sym_idx(0, 0)], c01 = c[sym_idx(0, 1)], c11 = c[sym_idx(1, 1)];
+    v.x0 = pick01(x, a);
+    v.x1 = pick01(x, b);
+    v.p00 = a ? c11 : c00;
+    v.p11 = b ? c11 : c00;
+    v.p01 = (a == b) ? v.p00 : c01;
+    return v;
+}
+
+TAMD_HD float predicted_chi2(const meas_data& m, const proj_view& v) {
+    const float r0 = m.m0 - v.x0;
+    const float R00 = v.p00 + m.v0;
     if (m.dims == 1u) {
         return r0 * (1.f / R00) * r0;
     }
-    const float r1 = m.m1 - x[m.b];
-    const float R01 = cov_at(c, m.a, m.b);
-    const float R11 = cov_at(c, m.b, m.b) + m.v1;
+    const float r1 = m.m1 - v.x1;
+    const float R01 = v.p01;
+    const float R11 = v.p11 + m.v1;
     const float det = R00 * R11 - R01 * R01;
     const float i00 = R11 / det, i11 = R00 / det, i01 = -R01 / det;
     return r0 * (i00 * r0 + i01 * r1) + r1 * (i01 * r0 + i11 * r1);
+}
+
+TAMD_HD float predicted_chi2(const meas_data& m, const float* x, const float* c) {
+    const proj_view v = project(x, c, m.a, m.b);
+    return predicted_chi2(m, v);
 }
 
 // wrap_phi_theta semantics (core/include/traccc/utils/trigonometric_helpers.hpp:63-81)
@@ -95,35 +128,37 @@ TAMD_HD bool normalize_angles(float& phi, float& theta) {
 // Gain-matrix update in place: x (6), c (21 packed).  Returns a
 // ckf_fitter_status; x and c are left untouched on failure.
 TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
-    const int a = static_cast<int>(m.a);
-    const int b = static_cast<int>(m.b);
+    const uint32_t a = m.a;
+    const uint32_t b = m.b;
     float pa[6], pb[6];  // columns a and b of P = P H^T
-    for (int i = 0; i < 6; ++i) {
-        pa[i] = cov_at(c, i, a);
-        pb[i] = cov_at(c, i, b);
-    }
-    const float R00 = pa[a] + m.v0;
+    cov_col(c, a, pa);
+    cov_col(c, b, pb);
+    const float paa = pick01(pa, a), pab = pick01(pa, b), pba = pick01(pb, a), pbb = pick01(pb, b);
+    const float R00 = paa + m.v0;
     float k0[6], k1[6];
-    float r0 = m.m0 - x[a];
+    float r0 = m.m0 - pick01(x, a);
     float r1 = 0.f;
     if (m.dims == 1u) {
         const float inv = 1.f / R00;
+        TAMD_UNROLL
         for (int i = 0; i < 6; ++i) {
             k0[i] = pa[i] * inv;
             k1[i] = 0.f;
         }
     } else {
-        const float R01 = pa[b];
-        const float R11 = pb[b] + m.v1;
+        const float R01 = pab;
+        const float R11 = pbb + m.v1;
         const float det = R00 * R11 - R01 * R01;
         const float i00 = R11 / det, i11 = R00 / det, i01 = -R01 / det;
+        TAMD_UNROLL
         for (int i = 0; i < 6; ++i) {
             k0[i] = pa[i] * i00 + pb[i] * i01;
             k1[i] = pa[i] * i01 + pb[i] * i11;
         }
-        r1 = m.m1 - x[b];
+        r1 = m.m1 - pick01(x, b);
     }
     float nx[6];
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
         nx[i] = x[i] + (k0[i] * r0 + k1[i] * r1);
     }
@@ -139,15 +174,18 @@ TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
     // Joseph form: P' = (I - K H) P (I - K H)^T + K V K^T.
     // Q = (I - K H) P has columns Q[:, j] = P[:, j] - k0 P[a, j] - k1 P[b, j].
     float qa[6], qb[6];
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
-        qa[i] = pa[i] - k0[i] * pa[a] - k1[i] * pb[a];
-        qb[i] = pb[i] - k0[i] * pa[b] - k1[i] * pb[b];
+        qa[i] = pa[i] - k0[i] * paa - k1[i] * pba;
+        qb[i] = pb[i] - k0[i] * pab - k1[i] * pbb;
     }
     const bool two = (m.dims == 2u);
     float nc[21];
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
+        TAMD_UNROLL
         for (int j = i; j < 6; ++j) {
-            const float qij = cov_at(c, i, j) - k0[i] * pa[j] - k1[i] * pb[j];
+            const float qij = c[sym_idx(i, j)] - k0[i] * pa[j] - k1[i] * pb[j];
             float v = qij - qa[i] * k0[j] - qb[i] * k1[j] + k0[i] * m.v0 * k0[j];
             if (two) {
                 v += k1[i] * m.v1 * k1[j];
@@ -156,17 +194,21 @@ TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
         }
     }
     const float min_var = -0.01f;
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
         if (nc[sym_idx(i, i)] < min_var) {
             return CKF_FIT_ERROR_UPDATER_INVALID_COVARIANCE;
         }
     }
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
         nc[sym_idx(i, i)] = std::fabs(nc[sym_idx(i, i)]);
     }
+    TAMD_UNROLL
     for (int i = 0; i < 6; ++i) {
         x[i] = nx[i];
     }
+    TAMD_UNROLL
     for (int i = 0; i < 21; ++i) {
         c[i] = nc[i];
     }

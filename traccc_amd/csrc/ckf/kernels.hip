@@ -116,6 +116,13 @@ __device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.li
 // ---------------------------------------------------------------------------
 // find_tracks
 // ---------------------------------------------------------------------------
+// A team of kTeam lanes handles one input branch: the lanes split the
+// surface's measurements, reduce the best (chi2, index) candidate with
+// shuffles, and run the gain update redundantly (so the team stays uniform);
+// the team leader writes.  A wave holds kWave / kTeam inputs.
+constexpr int kTeam = 16;
+constexpr int kTeamsPerWave = kWave / kTeam;
+
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -126,16 +133,19 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t link_base = ctl->link_base;
     const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n_in + kWave - 1) / kWave;
+    const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
     const uint32_t in_par = (step + 1u) & 1u;
     const uint32_t N = P.n_branch_surface;
+    const uint32_t team = lane() / kTeam;
+    const uint32_t tl = lane() % kTeam;
+    const bool leader = (tl == 0u);
 
     for (;;) {
         const uint32_t p = wave_ticket(&ctl->ticket[0]);
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i = p * kWave + lane();
+        const uint32_t i = p * kTeamsPerWave + team;
         const bool valid = i < n_in;
         uint32_t count = 0u, seed = 0u, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
         uint32_t in_slot = 0u;
@@ -168,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             while (count < N) {
                 float best_c = 0.f;
                 uint32_t best_m = kInvalid;
-                for (uint32_t m = lo; m < up; ++m) {
+                for (uint32_t m = lo + tl; m < up; m += kTeam) {
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, m);
                     const float chi2 = predicted_chi2(md, x, c);
                     if (!(chi2 <= P.chi2_max && chi2 >= 0.f)) {
@@ -183,6 +193,18 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                     if (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m)) {
                         best_c = chi2;
                         best_m = m;
+                    }
+                }
+                // team minimum of (chi2, index); kInvalid sorts last
+#pragma unroll
+                for (int d = kTeam / 2; d >= 1; d >>= 1) {
+                    const float oc = __shfl_xor(best_c, d, kTeam);
+                    const uint32_t om = __shfl_xor(best_m, d, kTeam);
+                    const bool take = (om != kInvalid) &&
+                                      (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
+                    if (take) {
+                        best_c = oc;
+                        best_m = om;
                     }
                 }
                 if (best_m == kInvalid) {
@@ -208,18 +230,20 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                 }
                 if (res == CKF_FIT_SUCCESS) {
                     const uint32_t slot = i * N + count;
-                    if (slot < P.flt_cap) {
-                        store_soa(P.flt, P.flt_cap, slot, fx, fc, sf);
-                        P.cand_meas[slot] = best_m;
-                        P.cand_chi2[slot] = best_c;
-                    } else {
-                        set_overflow(ctl, 4u);
+                    if (leader) {
+                        if (slot < P.flt_cap) {
+                            store_soa(P.flt, P.flt_cap, slot, fx, fc, sf);
+                            P.cand_meas[slot] = best_m;
+                            P.cand_chi2[slot] = best_c;
+                        } else {
+                            set_overflow(ctl, 4u);
+                        }
                     }
                     ++count;
                 }
             }
         }
-        const uint32_t n_out = valid ? (count > 0u ? count : 1u) : 0u;
+        const uint32_t n_out = (valid && leader) ? (count > 0u ? count : 1u) : 0u;
         const uint32_t incl = wave_inclusive_scan(n_out);
         const uint32_t wtotal = wave_broadcast(incl, kWave - 1);
         const uint64_t base = wave_lookback(P.lb_flag[0], P.lb_agg[0], P.lb_inc[0], p, epoch,
@@ -229,11 +253,11 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             ctl->n_links = lo32(base) + wtotal;
         }
         // head of the seed's contiguous run of inputs
-        uint32_t prev_seed = __shfl_up(seed, 1, kWave);
-        if (valid && lane() == 0u && i > 0u) {
+        uint32_t prev_seed = __shfl(seed, (lane() + kWave - kTeam) % kWave, kWave);
+        if (valid && team == 0u && i > 0u) {
             prev_seed = (step == 0u) ? i - 1u : load_link(P, prev_base + P.live[i - 1u]).seed;
         }
-        if (valid) {
+        if (valid && leader) {
             if (i == 0u || prev_seed != seed) {
                 P.seed_first[seed] = first;
             }
@@ -283,9 +307,6 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// duplicate removal
-// ---------------------------------------------------------------------------
 // Overflow bits raised by k_find (links, filtered states).  Later kernels of
 // the step only read these: the bits k_propagate raises itself must not change
 // the partition count its waves see mid-kernel.
@@ -303,17 +324,6 @@ __device__ inline uint32_t links_this_step(const ckf_params& P, const step_ctl* 
     return n < room ? n : room;
 }
 
-__global__ __launch_bounds__(kBlock) void k_dedup_clear(ckf_params P) {
-    const step_ctl* ctl = P.ctl;
-    if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
-        return;
-    }
-    const uint32_t n = P.n_meas + 1u;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        P.dedup_head[j] = kInvalid;
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_dedup_insert(ckf_params P) {
     const step_ctl* ctl = P.ctl;
     if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
@@ -321,6 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_insert(ckf_params P) {
     }
     const uint32_t n = links_this_step(P, ctl);
     const uint32_t base = ctl->link_base;
+    const uint32_t epoch = ctl->epoch;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         dlink L = load_link(P, base + i);
         while (L.meas == kInvalid && link_step(L) != 0u) {
@@ -328,7 +339,12 @@ __global__ __launch_bounds__(kBlock) void k_dedup_insert(ckf_params P) {
         }
         const uint32_t key = (L.meas == kInvalid) ? P.n_meas : L.meas;
         P.alive_prefix[i] = key;
-        P.dedup_next[i] = atomicExch(&P.dedup_head[key], i);
+        // list heads carry the step's epoch: a head from an earlier step reads as empty
+        const unsigned long long tagged = (static_cast<unsigned long long>(epoch) << 32) | i;
+        const unsigned long long old = atomicExch(
+            reinterpret_cast<unsigned long long*>(&P.dedup_head[key]), tagged);
+        P.dedup_next[i] = (static_cast<uint32_t>(old >> 32) == epoch) ? static_cast<uint32_t>(old)
+                                                                       : kInvalid;
     }
 }
 
@@ -376,7 +392,10 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
             const uint32_t key = P.alive_prefix[i];
             if (step + 1u - link_skipped(Li) > P.dup_min_len && Li.ndf_sum > 5u) {
                 const float p_this = prob_chi2(Li.chi2_sum, static_cast<float>(Li.ndf_sum - 5u));
-                for (uint32_t j = P.dedup_head[key]; j != kInvalid; j = P.dedup_next[j]) {
+                const uint64_t h = P.dedup_head[key];
+                const uint32_t first = (static_cast<uint32_t>(h >> 32) == epoch)
+                                           ? static_cast<uint32_t>(h) : kInvalid;
+                for (uint32_t j = first; j != kInvalid; j = P.dedup_next[j]) {
                     if (j == i) continue;
                     const dlink Lj = load_link(P, base + j);
                     if (step + 1u - link_skipped(Lj) <= P.dup_min_len || Li.ndf_sum <= 5u) continue;
@@ -408,14 +427,22 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
     }
 }
 
+__device__ void advance_step(const ckf_params& P, step_ctl* ctl);
+
 // ---------------------------------------------------------------------------
 // propagate_to_next_surface
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
+__global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_advance) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
         return;
     }
+    // the navigation layer table lives in LDS for the whole kernel
+    __shared__ ckf_layer s_layers[64];
+    for (uint32_t k = threadIdx.x; k < P.n_layers; k += blockDim.x) {
+        s_layers[k] = P.layers[k];
+    }
+    __syncthreads();
     const uint32_t step = ctl->step;
     const uint32_t n = links_this_step(P, ctl);
     const uint32_t base = ctl->link_base;
@@ -459,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
                     }
                     navigator<float, true> nav;
                     nav.sfs = P.sfs;
-                    nav.layers = P.layers;
+                    nav.layers = s_layers;
                     nav.n_layers = P.n_layers;
                     nav.field = P.field;
                     nav.cfg = P.nav;
@@ -547,13 +574,29 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
             ctl->n_tips_step = hi32(b) + t_tot;
         }
     }
+    if (fused_advance) {
+        // the last wave to leave runs the step advance; every wave reaches
+        // this point once the ticket queue is empty
+        __threadfence();
+        uint32_t before = 0u;
+        if (lane() == 0u) {
+            before = atomicAdd(&ctl->exit_count, 1u);
+        }
+        before = wave_broadcast(before, 0);
+        const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+        if (before + 1u == waves && lane() == 0u) {
+            __threadfence();  // acquire: every other wave's control-block writes
+            advance_step(P, ctl);
+            ctl->exit_count = 0u;
+            __threadfence();
+        }
+    }
 }
 
-__global__ void k_advance(ckf_params P) {
-    step_ctl* ctl = P.ctl;
-    if (ctl->done) {
-        return;
-    }
+// Step advance: move the control block to the next step.  Run by one lane
+// once every wave of the step's propagation has finished (k_advance, or the
+// last wave to leave k_propagate).
+__device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     const uint32_t n_links = ctl->n_links;
     ctl->n_links_total += n_links;
     ctl->n_tips += ctl->n_tips_step;
@@ -570,6 +613,14 @@ __global__ void k_advance(ckf_params P) {
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }
+}
+
+__global__ void k_advance(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    advance_step(P, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,10 +669,17 @@ __global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
             if (head) {
                 P.seg_base[seg] = kp;
                 P.seg_cbase[seg] = cp;
+                // the previous segment ends where this one starts
+                if (t > 0u) {
+                    const dlink Lp = load_link(P, P.tips[t - 1u]);
+                    const size_t prev = static_cast<size_t>(P.seed_event[Lp.seed]) * S + link_step(Lp);
+                    P.seg_count[prev] = kp;
+                    P.seg_ccount[prev] = cp;
+                }
             }
-            if (keep) {
-                atomicAdd(&P.seg_count[seg], 1u);
-                atomicAdd(&P.seg_ccount[seg], nc);
+            if (t + 1u == n) {
+                P.seg_count[seg] = kp + (keep ? 1u : 0u);
+                P.seg_ccount[seg] = cp + (keep ? nc : 0u);
             }
         }
     }
@@ -637,6 +695,14 @@ __global__ __launch_bounds__(1024) void k_build_table(ckf_params P) {
     const uint32_t per = (total + blockDim.x - 1) / blockDim.x;
     const uint32_t lo = threadIdx.x * per;
     const uint32_t hi = lo + per < total ? lo + per : total;
+    // seg_count / seg_ccount hold the kept / constituent prefix at the end of
+    // each present segment (0 for absent ones); turn them into counts
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t e = P.seg_count[k];
+        const uint32_t ce = P.seg_ccount[k];
+        P.seg_count[k] = e - (e ? P.seg_base[k] : 0u);
+        P.seg_ccount[k] = ce - (e || ce ? P.seg_cbase[k] : 0u);
+    }
     uint32_t a = 0u, b = 0u;
     for (uint32_t k = lo; k < hi; ++k) {
         a += P.seg_count[k];
@@ -707,7 +773,9 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
         const uint32_t s = L.seed;
         P.out_outcome[pos] = 0u;
         P.out_surface[pos] = P.seed_surface[s];
+        #pragma unroll
         for (int k = 0; k < 6; ++k) P.out_vec[6u * pos + k] = P.seed_vec[6u * s + k];
+        #pragma unroll
         for (int k = 0; k < 36; ++k) P.out_cov[36u * pos + k] = P.seed_cov[36u * s + k];
         P.out_ndf[pos] = ndf;
         P.out_chi2[pos] = chi2;
@@ -731,15 +799,21 @@ __global__ void k_debug_kf(uint32_t n, const float* vec, const float* cov, const
         return;
     }
     float x[6], c[21];
+    #pragma unroll
     for (int k = 0; k < 6; ++k) x[k] = vec[6u * i + k];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = r; q < 6; ++q) c[sym_idx(r, q)] = cov[36u * i + 6u * r + q];
     const meas_data md = load_meas(meas, var, info, i);
     chi2[i] = predicted_chi2(md, x, c);
     const int st = gain_update(md, x, c);
     status[i] = st;
+    #pragma unroll
     for (int k = 0; k < 6; ++k) out_vec[6u * i + k] = x[k];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = 0; q < 6; ++q) out_cov[36u * i + 6u * r + q] = c[sym_idx(r, q)];
 }
 
@@ -752,8 +826,11 @@ __global__ void k_debug_propagate(ckf_params P, uint32_t n, const uint32_t* star
         return;
     }
     float x[6], c[21];
+    #pragma unroll
     for (int k = 0; k < 6; ++k) x[k] = vec[6u * i + k];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = r; q < 6; ++q) c[sym_idx(r, q)] = cov[36u * i + 6u * r + q];
     const uint32_t sf = start[i];
     if (apply_start_material) {
@@ -780,22 +857,31 @@ __global__ void k_debug_propagate(ckf_params P, uint32_t n, const uint32_t* star
     free_to_bound<float, true>(P.sfs[end_sf], nav.st, nav.J, P.field, P.nav.mass, P.nav.abs_q, nx,
                                Jb);
     float M[6][6];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = 0; q < 6; ++q) {
             float acc = 0.f;
+            #pragma unroll
             for (int k = 0; k < 6; ++k) acc += Jb[r][k] * c[sym_idx(k, q)];
             M[r][q] = acc;
         }
     float nc[21];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = r; q < 6; ++q) {
             float acc = 0.f;
+            #pragma unroll
             for (int k = 0; k < 6; ++k) acc += M[r][k] * Jb[q][k];
             nc[sym_idx(r, q)] = acc;
         }
     add_material(P, end_sf, nx, nc);
+    #pragma unroll
     for (int k = 0; k < 6; ++k) out_vec[6u * i + k] = nx[k];
+    #pragma unroll
     for (int r = 0; r < 6; ++r)
+        #pragma unroll
         for (int q = 0; q < 6; ++q) out_cov[36u * i + 6u * r + q] = nc[sym_idx(r, q)];
 }
 
@@ -827,14 +913,15 @@ hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t s
 }
 
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    hipLaunchKernelGGL(k_dedup_clear, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
     hipLaunchKernelGGL(k_dedup_insert, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
     hipLaunchKernelGGL(k_dedup_mark, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
     return hipGetLastError();
 }
 
-hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    hipLaunchKernelGGL(k_propagate, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
+hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                            bool fused_advance) {
+    hipLaunchKernelGGL(k_propagate, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P,
+                       fused_advance ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -843,11 +930,11 @@ hipError_t launch_advance(const ckf_params& P, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                           bool with_dedup) {
     hipError_t e = launch_find(P, s, stream);
-    if (e == hipSuccess) e = launch_dedup(P, s, stream);
-    if (e == hipSuccess) e = launch_propagate(P, s, stream);
-    if (e == hipSuccess) e = launch_advance(P, stream);
+    if (e == hipSuccess && with_dedup) e = launch_dedup(P, s, stream);
+    if (e == hipSuccess) e = launch_propagate(P, s, stream, true);
     return e;
 }
 

@@ -15,14 +15,18 @@ struct launch_shape {
 };
 
 // One CKF step: find_tracks, duplicate removal (active from
-// duplicate_removal_minimum_length on; kernels early-exit before), propagation
-// to the next surface, and the step advance.  Every count is read from the
+// duplicate_removal_minimum_length on), propagation to the next surface with
+// the step advance folded into its last wave.  Every count is read from the
 // device step control block, so the sequence can be captured in a graph.
-hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+// with_dedup = false drops the duplicate-removal kernels (steps before
+// duplicate_removal_minimum_length, where they would exit at once).
+hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                           bool with_dedup);
 // the same step, kernel group by kernel group (timed runs)
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                            bool fused_advance);
 hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
 
 // build_tracks: tip scan, (event, step) segment table, output fill.

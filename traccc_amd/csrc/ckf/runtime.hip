@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -84,6 +86,44 @@ struct dbuf {
         return static_cast<T*>(p);
     }
 };
+
+// Process-wide cache of device blocks for per-call results: freed blocks are
+// kept by power-of-two size class and handed out again, so a steady stream of
+// operator calls does not call hipMalloc / hipFree.
+struct block_cache {
+    std::mutex mu;
+    std::map<size_t, std::vector<void*>> free_blocks;
+    static size_t size_class(size_t bytes) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    void* get(size_t bytes) {
+        const size_t c = size_class(std::max<size_t>(bytes, 1));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = free_blocks.find(c);
+            if (it != free_blocks.end() && !it->second.empty()) {
+                void* p = it->second.back();
+                it->second.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, c), "hipMalloc");
+        return p;
+    }
+    void put(void* p, size_t bytes) {
+        if (p == nullptr) return;
+        std::lock_guard<std::mutex> lk(mu);
+        free_blocks[size_class(std::max<size_t>(bytes, 1))].push_back(p);
+    }
+};
+
+block_cache& result_cache() {
+    static block_cache* c = new block_cache();  // lives until the process ends
+    return *c;
+}
 
 template <typename T>
 T* dev_copy(const T* src, size_t n) {
@@ -181,11 +221,10 @@ struct ckf_device_tracks {
     uint32_t* nholes = nullptr;
     uint32_t* offsets = nullptr;
     uint32_t* cons = nullptr;
+    size_t track_bytes[10] = {};
     ~ckf_device_tracks() {
         void* ps[] = {outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons};
-        for (void* p : ps) {
-            if (p) (void)hipFree(p);
-        }
+        for (int i = 0; i < 10; ++i) result_cache().put(ps[i], track_bytes[i]);
     }
 };
 
@@ -205,16 +244,19 @@ struct ckf_algorithm {
     dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
         seg_coff, ev_toff, ev_coff;
     tamd::step_ctl* h_ctl = nullptr;  // pinned mirror
-    // captured step chunk
-    hipGraphExec_t graph = nullptr;
+    // captured step chunks: graph[1] with duplicate removal, graph[0] without
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::vector<char> graph_key;
     // per-kernel event timing (diagnostics; the step loop then runs uncaptured)
+    uint32_t last_steps = 0;  // steps the previous call ran (sizes the first launch)
     bool timing = false;
     double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0;
     uint32_t n_prop_launches = 0;
 
     ~ckf_algorithm() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& g : graph) {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
         dbuf* all[] = {&state[0], &state[1], &flt, &cand_meas, &cand_chi2, &links, &live, &tips,
                        &seed_first, &seed_base, &alive_prefix, &dedup_head, &dedup_next, &ctl,
                        &tip_keep, &tip_rank, &tip_cprefix, &seg_count, &seg_ccount, &seg_base,
@@ -232,7 +274,7 @@ struct ckf_algorithm {
 
 namespace {
 
-constexpr uint32_t kStepsPerChunk = 8u;
+constexpr uint32_t kStepsPerChunk = 4u;
 
 uint32_t parts(uint32_t n) { return (n + 63u) / 64u; }
 
@@ -240,7 +282,9 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     const uint32_t N = A.cfg.max_num_branches_per_surface;
     A.flt_cap = static_cast<uint32_t>(std::min<uint64_t>(uint64_t(A.in_cap) * N, 0x3fffffffu));
     A.state_cap = A.flt_cap;
-    A.lb_cap = std::max({parts(A.in_cap), parts(A.state_cap), parts(A.tip_cap)}) + 1u;
+    // look-back partitions: k_find takes 4 inputs per wave (16-lane teams),
+    // the other kernels 64 items per wave
+    A.lb_cap = std::max({(A.in_cap + 3u) / 4u, parts(A.state_cap), parts(A.tip_cap)}) + 1u;
     const size_t st = size_t(tamd::kStateCols) * 4u;
     A.state[0].ensure(st * A.state_cap);
     A.state[1].ensure(st * A.state_cap);
@@ -253,7 +297,14 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.seed_first.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
     A.seed_base.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
     A.alive_prefix.ensure(4u * size_t(A.state_cap));
-    A.dedup_head.ensure(4u * (size_t(ev.n_meas) + 1u));
+    {
+        const size_t old = A.dedup_head.bytes;
+        A.dedup_head.ensure(8u * (size_t(ev.n_meas) + 1u));
+        if (A.dedup_head.bytes != old) {
+            // epoch 0 tags no step: fresh heads read as empty lists
+            hip_check(hipMemsetAsync(A.dedup_head.p, 0, A.dedup_head.bytes, A.stream), "memset");
+        }
+    }
     A.dedup_next.ensure(4u * size_t(A.state_cap));
     for (int k = 0; k < 4; ++k) {
         const size_t old = A.lb_flag[k].bytes;
@@ -331,7 +382,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.seed_first = A.seed_first.as<uint32_t>();
     P.seed_base = A.seed_base.as<uint32_t>();
     P.alive_prefix = A.alive_prefix.as<uint32_t>();
-    P.dedup_head = A.dedup_head.as<uint32_t>();
+    P.dedup_head = A.dedup_head.as<uint64_t>();
     P.dedup_next = A.dedup_next.as<uint32_t>();
     for (int k = 0; k < 4; ++k) {
         P.lb_flag[k] = A.lb_flag[k].as<uint32_t>();
@@ -362,7 +413,7 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
         return std::max(1u, std::min(b, max_blocks));
     };
     tamd::launch_shape s;
-    s.find_blocks = blocks(A.in_cap);
+    s.find_blocks = std::max(1u, std::min((((A.in_cap + 3u) / 4u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
     s.prop_blocks = blocks(A.state_cap);
     s.build_blocks = blocks(A.tip_cap);
@@ -381,7 +432,7 @@ uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd
         hip_check(hipEventRecord(ev[1], A.stream), "record");
         hip_check(tamd::launch_dedup(P, s, A.stream), "dedup");
         hip_check(hipEventRecord(ev[2], A.stream), "record");
-        hip_check(tamd::launch_propagate(P, s, A.stream), "propagate");
+        hip_check(tamd::launch_propagate(P, s, A.stream, false), "propagate");
         hip_check(hipEventRecord(ev[3], A.stream), "record");
         hip_check(tamd::launch_advance(P, A.stream), "advance");
         hip_check(hipEventRecord(ev[4], A.stream), "record");
@@ -418,29 +469,42 @@ uint32_t run_steps(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::laun
     if (A.timing) {
         return run_steps_timed(A, P, s);
     }
-    // capture one chunk of steps; re-capture when the launch parameters change
+    // capture one chunk of steps (with and without duplicate removal);
+    // re-capture when the launch parameters change
     std::vector<char> key(sizeof(P) + sizeof(s));
     std::memcpy(key.data(), &P, sizeof(P));
     std::memcpy(key.data() + sizeof(P), &s, sizeof(s));
-    if (A.graph == nullptr || key != A.graph_key) {
-        if (A.graph) {
-            (void)hipGraphExecDestroy(A.graph);
-            A.graph = nullptr;
+    if (A.graph[0] == nullptr || key != A.graph_key) {
+        for (int w = 0; w < 2; ++w) {
+            if (A.graph[w]) {
+                (void)hipGraphExecDestroy(A.graph[w]);
+                A.graph[w] = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(A.stream, hipStreamCaptureModeThreadLocal),
+                      "begin capture");
+            for (uint32_t k = 0; k < kStepsPerChunk; ++k) {
+                hip_check(tamd::launch_ckf_step(P, s, A.stream, w == 1), "step launch");
+            }
+            hip_check(hipStreamEndCapture(A.stream, &g), "end capture");
+            hip_check(hipGraphInstantiate(&A.graph[w], g, nullptr, nullptr, 0), "graph instantiate");
+            (void)hipGraphDestroy(g);
         }
-        hipGraph_t g = nullptr;
-        hip_check(hipStreamBeginCapture(A.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        for (uint32_t k = 0; k < kStepsPerChunk; ++k) {
-            hip_check(tamd::launch_ckf_step(P, s, A.stream), "step launch");
-        }
-        hip_check(hipStreamEndCapture(A.stream, &g), "end capture");
-        hip_check(hipGraphInstantiate(&A.graph, g, nullptr, nullptr, 0), "graph instantiate");
-        (void)hipGraphDestroy(g);
         A.graph_key = key;
     }
     uint32_t launched = 0u;
+    // launch as many chunks as the previous call needed before the first check
+    uint32_t chunks = std::max(1u, (A.last_steps + kStepsPerChunk - 1u) / kStepsPerChunk);
     for (;;) {
-        hip_check(hipGraphLaunch(A.graph, A.stream), "graph launch");
-        launched += kStepsPerChunk;
+        for (uint32_t k = 0; k < chunks; ++k) {
+            // a chunk whose last step is below duplicate_removal_minimum_length
+            // runs without the duplicate-removal kernels
+            const uint32_t last_step = launched + (k + 1u) * kStepsPerChunk - 1u;
+            const int w = last_step >= A.cfg.duplicate_removal_minimum_length ? 1 : 0;
+            hip_check(hipGraphLaunch(A.graph[w], A.stream), "graph launch");
+        }
+        launched += chunks * kStepsPerChunk;
+        chunks = 1u;
         hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
                                  A.stream),
                   "ctl readback");
@@ -450,6 +514,7 @@ uint32_t run_steps(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::laun
         }
     }
     A.epoch_next = A.h_ctl->epoch + 2u;
+    A.last_steps = A.h_ctl->n_steps_run;
     return A.h_ctl->overflow;
 }
 
@@ -531,16 +596,18 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
         T->n_const = nc;
         T->ids = D.ids;
         const size_t n1 = std::max<size_t>(nt, 1);
-        hip_check(hipMalloc(&T->outcome, 2u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->surface, 4u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->vec, 24u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->cov, 144u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->ndf, 4u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->chi2, 4u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->pval, 4u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->nholes, 4u * n1), "hipMalloc");
-        hip_check(hipMalloc(&T->offsets, 4u * (size_t(nt) + 1u)), "hipMalloc");
-        hip_check(hipMalloc(&T->cons, 4u * std::max<size_t>(nc, 1)), "hipMalloc");
+        const size_t sizes[10] = {2u * n1, 4u * n1, 24u * n1, 144u * n1, 4u * n1, 4u * n1,
+                                  4u * n1, 4u * n1, 4u * (size_t(nt) + 1u),
+                                  4u * std::max<size_t>(nc, 1)};
+        void** slots[10] = {reinterpret_cast<void**>(&T->outcome), reinterpret_cast<void**>(&T->surface),
+                            reinterpret_cast<void**>(&T->vec), reinterpret_cast<void**>(&T->cov),
+                            reinterpret_cast<void**>(&T->ndf), reinterpret_cast<void**>(&T->chi2),
+                            reinterpret_cast<void**>(&T->pval), reinterpret_cast<void**>(&T->nholes),
+                            reinterpret_cast<void**>(&T->offsets), reinterpret_cast<void**>(&T->cons)};
+        for (int i = 0; i < 10; ++i) {
+            *slots[i] = result_cache().get(sizes[i]);
+            T->track_bytes[i] = sizes[i];
+        }
         P.out_outcome = T->outcome;
         P.out_surface = T->surface;
         P.out_vec = T->vec;
@@ -612,8 +679,9 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
                     throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "measurement dimension must be 1 or 2");
                 }
                 const uint32_t s0 = M.subspace[2u * i], s1 = M.subspace[2u * i + 1u];
-                if (s0 > 1u || s1 > 5u) {
-                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "unsupported measurement subspace");
+                if (s0 > 1u || (d == 2u && (s1 > 1u || s1 == s0))) {
+                    throw ckf_error(CKF_ERR_UNSUPPORTED,
+                                    "measurement subspaces other than loc0 / loc1 are not supported");
                 }
                 pos[off + i] = make_float2(M.local_position[2u * i], M.local_position[2u * i + 1u]);
                 var[off + i] = make_float2(M.local_variance[2u * i], M.local_variance[2u * i + 1u]);
@@ -746,9 +814,9 @@ ckf_status ckf_algorithm_set_stream(ckf_algorithm* alg, void* hip_stream) {
     return guarded([&] {
         if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
         alg->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : alg->own_stream;
-        if (alg->graph) {
-            (void)hipGraphExecDestroy(alg->graph);
-            alg->graph = nullptr;
+        for (auto& g : alg->graph) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
         }
     });
 }
@@ -919,7 +987,9 @@ ckf_status ckf_debug_kf_update(uint32_t n, const float* vec, const float* cov, c
         for (uint32_t i = 0; i < n; ++i) {
             pos[i] = make_float2(meas[2u * i], meas[2u * i + 1u]);
             vv[i] = make_float2(var[2u * i], var[2u * i + 1u]);
-            if ((dims[i] != 1u && dims[i] != 2u) || subspace[2u * i] > 1u || subspace[2u * i + 1u] > 5u) {
+            if ((dims[i] != 1u && dims[i] != 2u) || subspace[2u * i] > 1u ||
+                (dims[i] == 2u && (subspace[2u * i + 1u] > 1u ||
+                                   subspace[2u * i + 1u] == subspace[2u * i]))) {
                 throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "bad measurement");
             }
             info[i] = dims[i] | (uint32_t(subspace[2u * i]) << 8) | (uint32_t(subspace[2u * i + 1u]) << 16);

@@ -74,39 +74,67 @@ __device__ inline uint32_t lo32(uint64_t v) { return static_cast<uint32_t>(v); }
 __device__ inline uint32_t hi32(uint64_t v) { return static_cast<uint32_t>(v >> 32); }
 
 // Exclusive prefix of partition p's packed aggregate over partitions 0..p-1.
-// Called by all lanes of the wave; lane 0 publishes and looks back.
+// Called by all lanes of the wave.  Lane 0 publishes the aggregate; then the
+// whole wave inspects the 64 preceding descriptors per round trip (lane l
+// reads partition q - 1 - l), stops at the nearest inclusive prefix, and sums
+// the aggregates in between with a wave reduction.
 __device__ inline uint64_t wave_lookback(uint32_t* flags, uint64_t* agg_slot, uint64_t* inc_slot,
                                          uint32_t p, uint32_t epoch, uint64_t agg) {
-    uint64_t excl = 0;
-    if (lane() == 0) {
-        const uint32_t tag = epoch << 2;
-        if (p == 0u) {
+    const uint32_t l = lane();
+    const uint32_t tag = epoch << 2;
+    if (p == 0u) {
+        if (l == 0u) {
             st_relaxed64(&inc_slot[0], agg);
             st_release(&flags[0], tag | 2u);
-        } else {
-            st_relaxed64(&agg_slot[p], agg);
-            st_release(&flags[p], tag | 1u);
-            uint32_t q = p - 1u;
-            for (;;) {
-                const uint32_t f = ld_acquire(&flags[q]);
-                if ((f >> 2) != epoch || (f & 3u) == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                if ((f & 3u) == 2u) {
-                    excl += ld_relaxed64(&inc_slot[q]);
-                    break;
-                }
-                excl += ld_relaxed64(&agg_slot[q]);
-                --q;
-            }
-            st_relaxed64(&inc_slot[p], excl + agg);
-            st_release(&flags[p], tag | 2u);
         }
+        return 0ull;
     }
-    const uint32_t elo = wave_broadcast(lo32(excl), 0);
-    const uint32_t ehi = wave_broadcast(hi32(excl), 0);
-    return pack2(elo, ehi);
+    if (l == 0u) {
+        st_relaxed64(&agg_slot[p], agg);
+        st_release(&flags[p], tag | 1u);
+    }
+    uint64_t excl = 0ull;
+    int64_t window_end = static_cast<int64_t>(p);  // partitions below this are unread
+    for (;;) {
+        const int64_t q = window_end - 1 - static_cast<int64_t>(l);
+        uint32_t status = 2u;  // lanes beyond partition 0 act as a terminating prefix of 0
+        uint64_t v = 0ull;
+        if (q >= 0) {
+            const uint32_t f = ld_acquire(&flags[q]);
+            status = ((f >> 2) == epoch) ? (f & 3u) : 0u;
+            if (status == 2u) {
+                v = ld_relaxed64(&inc_slot[q]);
+            } else if (status == 1u) {
+                v = ld_relaxed64(&agg_slot[q]);
+            }
+        }
+        const uint64_t m_pref = wave_ballot(status == 2u);
+        const uint64_t m_ready = wave_ballot(status != 0u);
+        // lanes up to and including the nearest prefix must all be ready
+        const uint32_t first_pref = m_pref ? static_cast<uint32_t>(__ffsll(static_cast<long long>(m_pref)) - 1) : 64u;
+        const uint64_t need = (first_pref >= 63u) ? ~0ull : ((2ull << first_pref) - 1ull);
+        if ((m_ready & need) != need) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;  // re-read the same window
+        }
+        uint64_t take = (l <= first_pref) ? v : 0ull;
+        uint32_t lo = lo32(take), hi = hi32(take);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo += __shfl_xor(lo, d, kWave);
+            hi += __shfl_xor(hi, d, kWave);
+        }
+        excl += pack2(lo, hi);
+        if (first_pref < 64u) {
+            break;
+        }
+        window_end -= kWave;
+    }
+    if (l == 0u) {
+        st_relaxed64(&inc_slot[p], excl + agg);
+        st_release(&flags[p], tag | 2u);
+    }
+    return excl;
 }
 
 }  // namespace tamd

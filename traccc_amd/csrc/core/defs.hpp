@@ -11,9 +11,13 @@
 #include <hip/hip_runtime.h>
 #define TAMD_HD __host__ __device__ inline
 #define TAMD_DEV __device__ inline
+// full unrolling keeps small matrices in registers (a runtime index into a
+// register array sends it to scratch memory)
+#define TAMD_UNROLL _Pragma("unroll")
 #else
 #define TAMD_HD inline
 #define TAMD_DEV inline
+#define TAMD_UNROLL
 #endif
 
 #include "traccc_amd/ckf.h"

@@ -140,6 +140,7 @@ TAMD_HD void rk4_step(free_state<T>& s, transport_jac<T>& J,
     if constexpr (kJac) {
         // Columns of dk_i/dT0 (3x3) and dk_i/dq.
         T A1[3][3], A2[3][3], A3[3][3], A4[3][3];
+        TAMD_UNROLL
         for (int c = 0; c < 3; ++c) {
             const vec3<T> e{c == 0 ? T(1) : T(0), c == 1 ? T(1) : T(0),
                             c == 2 ? T(1) : T(0)};
@@ -162,7 +163,9 @@ TAMD_HD void rk4_step(free_state<T>& s, transport_jac<T>& J,
                           h6 * (d1.y + T(2) * d2.y + T(2) * d3.y + d4.y),
                           h6 * (d1.z + T(2) * d2.z + T(2) * d3.z + d4.z)};
         T DrT[3][3], DTT[3][3];
+        TAMD_UNROLL
         for (int r = 0; r < 3; ++r) {
+            TAMD_UNROLL
             for (int c = 0; c < 3; ++c) {
                 const T id = (r == c) ? T(1) : T(0);
                 DrT[r][c] = h * id + hh6 * (A1[r][c] + A2[r][c] + A3[r][c]);
@@ -170,12 +173,15 @@ TAMD_HD void rk4_step(free_state<T>& s, transport_jac<T>& J,
                                        T(2) * A3[r][c] + A4[r][c]);
             }
         }
+        TAMD_UNROLL
         for (int j = 0; j < 5; ++j) {
             const T d0 = J.d[0][j], dd1 = J.d[1][j], dd2 = J.d[2][j];
+            TAMD_UNROLL
             for (int r = 0; r < 3; ++r) {
                 J.p[r][j] += DrT[r][0] * d0 + DrT[r][1] * dd1 + DrT[r][2] * dd2;
             }
             T nd[3];
+            TAMD_UNROLL
             for (int r = 0; r < 3; ++r) {
                 nd[r] = DTT[r][0] * d0 + DTT[r][1] * dd1 + DTT[r][2] * dd2;
             }
@@ -183,6 +189,7 @@ TAMD_HD void rk4_step(free_state<T>& s, transport_jac<T>& J,
             J.d[1][j] = nd[1];
             J.d[2][j] = nd[2];
         }
+        TAMD_UNROLL
         for (int r = 0; r < 3; ++r) {
             J.p[r][4] += drl[r];
             J.d[r][4] += dtl[r];
@@ -223,7 +230,9 @@ TAMD_HD void bound_to_free(const ckf_surface& sf, const T* v, free_state<T>& s,
     s.qop = v[e_qop];
     s.time = v[e_time];
     if constexpr (kJac) {
+        TAMD_UNROLL
         for (int r = 0; r < 3; ++r) {
+            TAMD_UNROLL
             for (int j = 0; j < 5; ++j) {
                 J.p[r][j] = T(0);
                 J.d[r][j] = T(0);
@@ -262,6 +271,7 @@ TAMD_HD void free_to_bound(const ckf_surface& sf, const free_state<T>& s,
         const vec3<T> dTds = s.qop * cross(s.dir, B);
         const T dt = dtds_of(s.qop, mass, abs_q);
         T pc[3][5], dc[3][5], tc[5];
+        TAMD_UNROLL
         for (int j = 0; j < 5; ++j) {
             const T dds = -(w.x * J.p[0][j] + w.y * J.p[1][j] + w.z * J.p[2][j]) / tw;
             pc[0][j] = J.p[0][j] + s.dir.x * dds;
@@ -276,6 +286,7 @@ TAMD_HD void free_to_bound(const ckf_surface& sf, const free_state<T>& s,
         const T sinT = std::sqrt(sin2);
         const vec3<T> gphi{-s.dir.y / sin2, s.dir.x / sin2, T(0)};
         const vec3<T> gth{s.dir.x * s.dir.z / sinT, s.dir.y * s.dir.z / sinT, -sinT};
+        TAMD_UNROLL
         for (int j = 0; j < 5; ++j) {
             Jb[0][j] = u.x * pc[0][j] + u.y * pc[1][j] + u.z * pc[2][j];
             Jb[1][j] = vv.x * pc[0][j] + vv.y * pc[1][j] + vv.z * pc[2][j];
@@ -284,6 +295,7 @@ TAMD_HD void free_to_bound(const ckf_surface& sf, const free_state<T>& s,
             Jb[4][j] = (j == 4) ? T(1) : T(0);
             Jb[5][j] = tc[j];
         }
+        TAMD_UNROLL
         for (int r = 0; r < 5; ++r) {
             Jb[r][5] = T(0);
         }
@@ -317,7 +329,8 @@ constexpr int kMaxCandidates = 9;
 //     the current state enters first; none -> NAV_EXIT.
 //  3. Approach it with RK steps until the entry is within approach_dist.
 //  4. Collect the modules in the search window around the projected entry
-//     point whose straight-line intersection is ahead (>= overstep_tol) and
+//     point (every module, up to 9, of a plane layer) whose straight-line
+//     intersection is ahead (>= overstep_tol) and
 //     inside the mask within a distance-scaled tolerance; visit them in
 //     increasing straight-line distance: step onto each plane (iterated RK
 //     steps until |distance| <= on_surface_tol, then one last step of that
@@ -359,36 +372,10 @@ struct navigator {
         return n_steps <= cfg.max_steps && std::fabs(path) <= cfg.path_limit;
     }
 
-    // 0: on the plane, 1: abort, 2: plane parallel to the track
-    TAMD_HD int step_to_plane(const ckf_surface& sf) {
-        const vec3<T> c = sf_center<T>(sf);
-        const vec3<T> w = sf_w<T>(sf);
-        T sd = T(0);
-        for (;;) {
-            const T tw = dot(st.dir, w);
-            if (std::fabs(tw) < static_cast<T>(1e-8)) {
-                return 2;
-            }
-            sd = -dot(rel_to(st, c), w) / tw;
-            if (std::fabs(sd) <= cfg.on_surface_tol) {
-                break;
-            }
-            const T hl = step_limit();
-            const T h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
-            if (!do_step(h)) {
-                return 1;
-            }
-        }
-        // last (sub-tolerance) step onto the plane: an RK step keeps the
-        // direction, time and transport consistent with the position
-        if (sd != T(0) && !do_step(sd)) {
-            return 1;
-        }
-        return 0;
-    }
-
-    // 0: found (out_sf), 1: nothing in this layer, 2: abort
-    TAMD_HD int process_layer(uint32_t li, int exclude, uint32_t& out_sf) {
+    // Candidate modules of layer `li` in fixed slots (window position, or
+    // module of a plane layer); unused slots hold +inf.  No register array is
+    // indexed with a runtime value.
+    TAMD_HD void collect(uint32_t li, int exclude, T* cs, uint32_t* cm) const {
         const ckf_layer& L = layers[li];
         T s_entry = layer_entry_distance(L, st.pos, st.dir, cfg.mask_tol_max);
         if (!(s_entry < static_cast<T>(1e29))) {
@@ -398,107 +385,68 @@ struct navigator {
         int ic = 0, jc = 0;
         const int nphi = static_cast<int>(L.n_phi);
         const int nlong = static_cast<int>(L.n_long);
+        const bool plane = (L.kind == CKF_LAYER_PLANE);
         int wp = cfg.win_phi;
-        int wl = (L.kind == CKF_LAYER_BARREL) ? cfg.win_long : 0;
-        if (L.kind == CKF_LAYER_PLANE) {
-            // every module of a plane layer is a candidate
-            wp = (nphi - 1) / 2;
-            wl = (nlong - 1) / 2;
-            ic = wp;
-            jc = wl;
-        } else {
+        const int wl = (L.kind == CKF_LAYER_BARREL) ? cfg.win_long : 0;
+        if (!plane) {
             layer_bin(L, q, ic, jc);
         }
         if (2 * wp + 1 > nphi) {
             wp = (nphi - 1) / 2;
         }
-        T cs[kMaxCandidates];
-        uint32_t cm[kMaxCandidates];
-        int nc = 0;
-        for (int dp = -wp; dp <= wp; ++dp) {
-            int ip = (ic + dp) % nphi;
-            if (ip < 0) {
-                ip += nphi;
-            }
-            for (int dl = -wl; dl <= wl; ++dl) {
+        const T inf = static_cast<T>(1e30);
+TAMD_UNROLL
+        for (int k = 0; k < kMaxCandidates; ++k) {
+            cs[k] = inf;
+            cm[k] = 0u;
+            bool ok = true;
+            uint32_t m = 0u;
+            if (plane) {
+                ok = k < nphi * nlong;
+                m = L.first_surface + static_cast<uint32_t>(k);
+            } else {
+                const int dp = k / 3 - 1;
+                const int dl = k % 3 - 1;
+                ok = (dp >= -wp && dp <= wp && dl >= -wl && dl <= wl);
+                int ip = (ic + dp) % nphi;
+                if (ip < 0) {
+                    ip += nphi;
+                }
                 const int il = jc + dl;
-                if (il < 0 || il >= nlong) {
-                    continue;
-                }
-                const uint32_t m = L.first_surface +
-                                   static_cast<uint32_t>(ip * nlong + il);
-                if (static_cast<int>(m) == exclude) {
-                    continue;
-                }
-                const ckf_surface& S = sfs[m];
-                const vec3<T> w = sf_w<T>(S);
-                const T tw = dot(st.dir, w);
-                if (std::fabs(tw) < static_cast<T>(1e-6)) {
-                    continue;
-                }
-                const vec3<T> c = sf_center<T>(S);
-                const T s = dot(c - st.pos, w) / tw;
-                if (s < cfg.overstep_tol) {
-                    continue;
-                }
-                const vec3<T> x = st.pos + s * st.dir - c;
-                T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
-                tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
-                if (!inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
-                    continue;
-                }
-                if (nc < kMaxCandidates) {
-                    cs[nc] = s;
-                    cm[nc] = m;
-                    ++nc;
-                }
+                ok = ok && il >= 0 && il < nlong;
+                m = L.first_surface + static_cast<uint32_t>(ip * nlong + il);
             }
-        }
-        uint32_t used = 0u;
-        for (int k = 0; k < nc; ++k) {
-            int pick = -1;
-            for (int i = 0; i < nc; ++i) {
-                if (used & (1u << i)) {
-                    continue;
-                }
-                if (pick < 0 || cs[i] < cs[pick] ||
-                    (cs[i] == cs[pick] && cm[i] < cm[pick])) {
-                    pick = i;
-                }
-            }
-            used |= (1u << pick);
-            const ckf_surface& S = sfs[cm[pick]];
-            {
-                const vec3<T> w = sf_w<T>(S);
-                const T tw = dot(st.dir, w);
-                if (std::fabs(tw) < static_cast<T>(1e-8)) {
-                    continue;
-                }
-                const T s_now = dot(sf_center<T>(S) - st.pos, w) / tw;
-                if (s_now < cfg.overstep_tol) {
-                    continue;
-                }
-            }
-            const int r = step_to_plane(S);
-            if (r == 1) {
-                return 2;
-            }
-            if (r == 2) {
+            if (!ok || static_cast<int>(m) == exclude) {
                 continue;
             }
-            const vec3<T> x = rel_to(st, sf_center<T>(S));
-            if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
-                path > cfg.min_step_length) {
-                out_sf = cm[pick];
-                return 0;
+            const ckf_surface& S = sfs[m];
+            const vec3<T> w = sf_w<T>(S);
+            const T tw = dot(st.dir, w);
+            if (std::fabs(tw) < static_cast<T>(1e-6)) {
+                continue;
             }
+            const vec3<T> c = sf_center<T>(S);
+            const T s = dot(c - st.pos, w) / tw;
+            if (s < cfg.overstep_tol) {
+                continue;
+            }
+            const vec3<T> x = st.pos + s * st.dir - c;
+            T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
+            tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
+            if (!inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
+                continue;
+            }
+            cs[k] = s;
+            cm[k] = m;
         }
-        return 1;
     }
 
     // Propagate from the current state (on surface start_sf, or -1 for a
-    // free start) to the next sensitive surface.
+    // free start) to the next sensitive surface.  Written as a state machine
+    // around a single RK-step site (the step with transport is large; one copy
+    // of it keeps the kernel inside the instruction cache).
     TAMD_HD int propagate(int start_sf, uint32_t& out_sf) {
+        enum phase_t : int { PICK, APPROACH, COLLECT, NEXT, TO_PLANE, CHECK };
         path = T(0);
         n_steps = 0u;
         {
@@ -508,55 +456,151 @@ struct navigator {
                 return NAV_ABORT;
             }
         }
+        const T inf = static_cast<T>(1e30);
         uint64_t visited = 0ull;
+        uint32_t guard = 0u;
+        uint32_t cur = 0u;          // layer being approached / scanned
+        int exclude = -1;
+        T bs = inf;                 // entry distance of `cur`
+        T cs[kMaxCandidates];
+        uint32_t cm[kMaxCandidates];
+        uint32_t used = 0u;
+        int tries = 0;
+        uint32_t target = 0u;
+        T h = T(0);
+        int after = PICK;           // phase to resume after an RK step
+        int phase = PICK;
         if (start_sf >= 0) {
-            const uint32_t l0 = sfs[start_sf].layer;
-            const int r = process_layer(l0, start_sf, out_sf);
-            if (r == 0) {
-                return NAV_FOUND;
-            }
-            if (r == 2) {
-                return NAV_ABORT;
-            }
-            visited |= (1ull << l0);
+            cur = sfs[start_sf].layer;
+            exclude = start_sf;
+            phase = COLLECT;
         }
-        for (uint32_t guard = 0; guard <= n_layers; ++guard) {
-            int best = -1;
-            T bs = static_cast<T>(1e30);
-            for (uint32_t li = 0; li < n_layers; ++li) {
-                if (visited & (1ull << li)) {
+        for (;;) {
+            // advance this lane's phase machine to its next RK step, so the
+            // wave runs the (expensive) step for all its lanes at once
+            bool step_now = false;
+            int result = -1;
+            while (!step_now && result < 0) {
+            if (phase == PICK) {
+                if (guard > n_layers) {
+                    result = NAV_EXIT;
                     continue;
                 }
-                const T s = layer_entry_distance(layers[li], st.pos, st.dir,
-                                                 cfg.mask_tol_max);
-                if (s < bs) {
-                    bs = s;
-                    best = static_cast<int>(li);
+                ++guard;
+                int best = -1;
+                T b = inf;
+                for (uint32_t li = 0; li < n_layers; ++li) {
+                    if (visited & (1ull << li)) {
+                        continue;
+                    }
+                    const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
+                    if (s < b) {
+                        b = s;
+                        best = static_cast<int>(li);
+                    }
                 }
-            }
-            if (best < 0 || !(bs < static_cast<T>(1e29))) {
-                return NAV_EXIT;
-            }
-            while (bs > cfg.approach_dist && bs < static_cast<T>(1e29)) {
-                const T hl = step_limit();
-                if (!do_step(bs < hl ? bs : hl)) {
-                    return NAV_ABORT;
+                if (best < 0 || !(b < static_cast<T>(1e29))) {
+                    result = NAV_EXIT;
+                    continue;
                 }
-                bs = layer_entry_distance(layers[best], st.pos, st.dir,
-                                          cfg.mask_tol_max);
-            }
-            if (bs < static_cast<T>(1e29)) {
-                const int r = process_layer(static_cast<uint32_t>(best), -1, out_sf);
-                if (r == 0) {
-                    return NAV_FOUND;
+                cur = static_cast<uint32_t>(best);
+                bs = b;
+                exclude = -1;
+                phase = APPROACH;
+            } else if (phase == APPROACH) {
+                if (bs > cfg.approach_dist && bs < static_cast<T>(1e29)) {
+                    const T hl = step_limit();
+                    h = bs < hl ? bs : hl;
+                    after = APPROACH;
+                    step_now = true;
+                } else if (bs < static_cast<T>(1e29)) {
+                    phase = COLLECT;
+                } else {
+                    visited |= (1ull << cur);
+                    phase = PICK;
                 }
-                if (r == 2) {
-                    return NAV_ABORT;
+            } else if (phase == COLLECT) {
+                collect(cur, exclude, cs, cm);
+                used = 0u;
+                tries = 0;
+                phase = NEXT;
+            } else if (phase == NEXT) {
+                int pick = -1;
+                T best = inf;
+                uint32_t bm = 0u;
+                if (tries < kMaxCandidates) {
+                    ++tries;
+TAMD_UNROLL
+                    for (int k = 0; k < kMaxCandidates; ++k) {
+                        const bool avail = ((used >> k) & 1u) == 0u && cs[k] < inf;
+                        if (avail && (pick < 0 || cs[k] < best || (cs[k] == best && cm[k] < bm))) {
+                            pick = k;
+                            best = cs[k];
+                            bm = cm[k];
+                        }
+                    }
                 }
+                if (pick < 0) {
+                    visited |= (1ull << cur);
+                    phase = PICK;
+                } else {
+                    used |= (1u << pick);
+                    target = bm;
+                    const ckf_surface& S = sfs[target];
+                    const vec3<T> w = sf_w<T>(S);
+                    const T tw = dot(st.dir, w);
+                    if (std::fabs(tw) >= static_cast<T>(1e-8) &&
+                        dot(sf_center<T>(S) - st.pos, w) / tw >= cfg.overstep_tol) {
+                        phase = TO_PLANE;
+                    }
+                }
+            } else if (phase == TO_PLANE) {
+                const ckf_surface& S = sfs[target];
+                const vec3<T> w = sf_w<T>(S);
+                const T tw = dot(st.dir, w);
+                if (std::fabs(tw) < static_cast<T>(1e-8)) {
+                    phase = NEXT;
+                } else {
+                    const T sd = -dot(rel_to(st, sf_center<T>(S)), w) / tw;
+                    if (std::fabs(sd) <= cfg.on_surface_tol) {
+                        // last (sub-tolerance) step onto the plane
+                        if (sd != T(0)) {
+                            h = sd;
+                            after = CHECK;
+                            step_now = true;
+                        } else {
+                            phase = CHECK;
+                        }
+                    } else {
+                        const T hl = step_limit();
+                        h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
+                        after = TO_PLANE;
+                        step_now = true;
+                    }
+                }
+            } else {  // CHECK
+                const ckf_surface& S = sfs[target];
+                const vec3<T> x = rel_to(st, sf_center<T>(S));
+                if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
+                    path > cfg.min_step_length) {
+                    out_sf = target;
+                    result = NAV_FOUND;
+                    continue;
+                }
+                phase = NEXT;
             }
-            visited |= (1ull << best);
+            }
+            if (result >= 0) {
+                return result;
+            }
+            if (!do_step(h)) {
+                return NAV_ABORT;
+            }
+            phase = after;
+            if (after == APPROACH) {
+                bs = layer_entry_distance(layers[cur], st.pos, st.dir, cfg.mask_tol_max);
+            }
         }
-        return NAV_EXIT;
     }
 };
 
