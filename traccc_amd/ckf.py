@@ -123,10 +123,14 @@ class DeviceTracks:
 
     def __init__(self, handle: C.c_void_p):
         self._h = handle
+        self._L = _lib()
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib().ckf_tracks_destroy(self._h)
+        # the library handle is held by the object: module globals may
+        # already be gone when this runs at interpreter shutdown
+        lib = getattr(self, "_L", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ckf_tracks_destroy(self._h)
             self._h = None
 
     def sizes(self) -> tuple[int, int]:
@@ -171,10 +175,14 @@ class DeviceDetector:
         _check(_lib().ckf_device_detector_create(C.byref(detector.struct), C.byref(field.struct),
                                                  C.byref(h)))
         self._h = h
+        self._L = _lib()
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib().ckf_device_detector_destroy(self._h)
+        # the library handle is held by the object: module globals may
+        # already be gone when this runs at interpreter shutdown
+        lib = getattr(self, "_L", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ckf_device_detector_destroy(self._h)
             self._h = None
 
 
@@ -190,12 +198,16 @@ class DeviceEvents:
         h = C.c_void_p()
         _check(_lib().ckf_device_event_create(ddet._h, ms, ss, n, C.byref(h)))
         self._h = h
+        self._L = _lib()
         self.n_events = n
         self.n_seeds = sum(e.n_seeds for e in events)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib().ckf_device_event_destroy(self._h)
+        # the library handle is held by the object: module globals may
+        # already be gone when this runs at interpreter shutdown
+        lib = getattr(self, "_L", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ckf_device_event_destroy(self._h)
             self._h = None
 
 
@@ -207,12 +219,16 @@ class combinatorial_kalman_filter_algorithm:  # noqa: N801  (traccc's class name
         h = C.c_void_p()
         _check(_lib().ckf_algorithm_create(C.byref(self.config), C.byref(h)))
         self._h = h
+        self._L = _lib()
         if stream is not None:
             _check(_lib().ckf_algorithm_set_stream(self._h, C.c_void_p(stream)))
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib().ckf_algorithm_destroy(self._h)
+        # the library handle is held by the object: module globals may
+        # already be gone when this runs at interpreter shutdown
+        lib = getattr(self, "_L", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.ckf_algorithm_destroy(self._h)
             self._h = None
 
     def run_device(self, ddet: DeviceDetector, devents: DeviceEvents) -> DeviceTracks:

@@ -441,12 +441,89 @@ TAMD_UNROLL
         }
     }
 
+    // Step onto the plane of `S`: RK steps until |distance| <= on_surface_tol,
+    // then one last (sub-tolerance) RK step of that distance, which keeps the
+    // direction, time and transport consistent with the position.
+    // 0: on the plane, 1: abort, 2: plane parallel to the track.
+    TAMD_HD int step_to_plane(const ckf_surface& S) {
+        const vec3<T> c = sf_center<T>(S);
+        const vec3<T> w = sf_w<T>(S);
+        T sd = T(0);
+        for (;;) {
+            const T tw = dot(st.dir, w);
+            if (std::fabs(tw) < static_cast<T>(1e-8)) {
+                return 2;
+            }
+            sd = -dot(rel_to(st, c), w) / tw;
+            if (std::fabs(sd) <= cfg.on_surface_tol) {
+                break;
+            }
+            const T hl = step_limit();
+            const T h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
+            if (!do_step(h)) {
+                return 1;
+            }
+        }
+        if (sd != T(0) && !do_step(sd)) {
+            return 1;
+        }
+        return 0;
+    }
+
+    // Visit the candidates of layer `li` in increasing (distance, module).
+    // 0: found (out_sf), 1: nothing in this layer, 2: abort.
+    TAMD_HD int process_layer(uint32_t li, int exclude, uint32_t& out_sf) {
+        T cs[kMaxCandidates];
+        uint32_t cm[kMaxCandidates];
+        collect(li, exclude, cs, cm);
+        const T inf = static_cast<T>(1e30);
+        uint32_t used = 0u;
+        for (int tries = 0; tries < kMaxCandidates; ++tries) {
+            int pick = -1;
+            T best = inf;
+            uint32_t bm = 0u;
+TAMD_UNROLL
+            for (int k = 0; k < kMaxCandidates; ++k) {
+                const bool avail = ((used >> k) & 1u) == 0u && cs[k] < inf;
+                if (avail && (pick < 0 || cs[k] < best || (cs[k] == best && cm[k] < bm))) {
+                    pick = k;
+                    best = cs[k];
+                    bm = cm[k];
+                }
+            }
+            if (pick < 0) {
+                return 1;
+            }
+            used |= (1u << pick);
+            const ckf_surface& S = sfs[bm];
+            {
+                const vec3<T> w = sf_w<T>(S);
+                const T tw = dot(st.dir, w);
+                if (std::fabs(tw) < static_cast<T>(1e-8) ||
+                    dot(sf_center<T>(S) - st.pos, w) / tw < cfg.overstep_tol) {
+                    continue;
+                }
+            }
+            const int r = step_to_plane(S);
+            if (r == 1) {
+                return 2;
+            }
+            if (r == 2) {
+                continue;
+            }
+            const vec3<T> x = rel_to(st, sf_center<T>(S));
+            if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
+                path > cfg.min_step_length) {
+                out_sf = bm;
+                return 0;
+            }
+        }
+        return 1;
+    }
+
     // Propagate from the current state (on surface start_sf, or -1 for a
-    // free start) to the next sensitive surface.  Written as a state machine
-    // around a single RK-step site (the step with transport is large; one copy
-    // of it keeps the kernel inside the instruction cache).
+    // free start) to the next sensitive surface.
     TAMD_HD int propagate(int start_sf, uint32_t& out_sf) {
-        enum phase_t : int { PICK, APPROACH, COLLECT, NEXT, TO_PLANE, CHECK };
         path = T(0);
         n_steps = 0u;
         {
@@ -456,151 +533,53 @@ TAMD_UNROLL
                 return NAV_ABORT;
             }
         }
-        const T inf = static_cast<T>(1e30);
         uint64_t visited = 0ull;
-        uint32_t guard = 0u;
-        uint32_t cur = 0u;          // layer being approached / scanned
-        int exclude = -1;
-        T bs = inf;                 // entry distance of `cur`
-        T cs[kMaxCandidates];
-        uint32_t cm[kMaxCandidates];
-        uint32_t used = 0u;
-        int tries = 0;
-        uint32_t target = 0u;
-        T h = T(0);
-        int after = PICK;           // phase to resume after an RK step
-        int phase = PICK;
         if (start_sf >= 0) {
-            cur = sfs[start_sf].layer;
-            exclude = start_sf;
-            phase = COLLECT;
-        }
-        for (;;) {
-            // advance this lane's phase machine to its next RK step, so the
-            // wave runs the (expensive) step for all its lanes at once
-            bool step_now = false;
-            int result = -1;
-            while (!step_now && result < 0) {
-            if (phase == PICK) {
-                if (guard > n_layers) {
-                    result = NAV_EXIT;
-                    continue;
-                }
-                ++guard;
-                int best = -1;
-                T b = inf;
-                for (uint32_t li = 0; li < n_layers; ++li) {
-                    if (visited & (1ull << li)) {
-                        continue;
-                    }
-                    const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
-                    if (s < b) {
-                        b = s;
-                        best = static_cast<int>(li);
-                    }
-                }
-                if (best < 0 || !(b < static_cast<T>(1e29))) {
-                    result = NAV_EXIT;
-                    continue;
-                }
-                cur = static_cast<uint32_t>(best);
-                bs = b;
-                exclude = -1;
-                phase = APPROACH;
-            } else if (phase == APPROACH) {
-                if (bs > cfg.approach_dist && bs < static_cast<T>(1e29)) {
-                    const T hl = step_limit();
-                    h = bs < hl ? bs : hl;
-                    after = APPROACH;
-                    step_now = true;
-                } else if (bs < static_cast<T>(1e29)) {
-                    phase = COLLECT;
-                } else {
-                    visited |= (1ull << cur);
-                    phase = PICK;
-                }
-            } else if (phase == COLLECT) {
-                collect(cur, exclude, cs, cm);
-                used = 0u;
-                tries = 0;
-                phase = NEXT;
-            } else if (phase == NEXT) {
-                int pick = -1;
-                T best = inf;
-                uint32_t bm = 0u;
-                if (tries < kMaxCandidates) {
-                    ++tries;
-TAMD_UNROLL
-                    for (int k = 0; k < kMaxCandidates; ++k) {
-                        const bool avail = ((used >> k) & 1u) == 0u && cs[k] < inf;
-                        if (avail && (pick < 0 || cs[k] < best || (cs[k] == best && cm[k] < bm))) {
-                            pick = k;
-                            best = cs[k];
-                            bm = cm[k];
-                        }
-                    }
-                }
-                if (pick < 0) {
-                    visited |= (1ull << cur);
-                    phase = PICK;
-                } else {
-                    used |= (1u << pick);
-                    target = bm;
-                    const ckf_surface& S = sfs[target];
-                    const vec3<T> w = sf_w<T>(S);
-                    const T tw = dot(st.dir, w);
-                    if (std::fabs(tw) >= static_cast<T>(1e-8) &&
-                        dot(sf_center<T>(S) - st.pos, w) / tw >= cfg.overstep_tol) {
-                        phase = TO_PLANE;
-                    }
-                }
-            } else if (phase == TO_PLANE) {
-                const ckf_surface& S = sfs[target];
-                const vec3<T> w = sf_w<T>(S);
-                const T tw = dot(st.dir, w);
-                if (std::fabs(tw) < static_cast<T>(1e-8)) {
-                    phase = NEXT;
-                } else {
-                    const T sd = -dot(rel_to(st, sf_center<T>(S)), w) / tw;
-                    if (std::fabs(sd) <= cfg.on_surface_tol) {
-                        // last (sub-tolerance) step onto the plane
-                        if (sd != T(0)) {
-                            h = sd;
-                            after = CHECK;
-                            step_now = true;
-                        } else {
-                            phase = CHECK;
-                        }
-                    } else {
-                        const T hl = step_limit();
-                        h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
-                        after = TO_PLANE;
-                        step_now = true;
-                    }
-                }
-            } else {  // CHECK
-                const ckf_surface& S = sfs[target];
-                const vec3<T> x = rel_to(st, sf_center<T>(S));
-                if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
-                    path > cfg.min_step_length) {
-                    out_sf = target;
-                    result = NAV_FOUND;
-                    continue;
-                }
-                phase = NEXT;
+            const uint32_t l0 = sfs[start_sf].layer;
+            const int r = process_layer(l0, start_sf, out_sf);
+            if (r == 0) {
+                return NAV_FOUND;
             }
-            }
-            if (result >= 0) {
-                return result;
-            }
-            if (!do_step(h)) {
+            if (r == 2) {
                 return NAV_ABORT;
             }
-            phase = after;
-            if (after == APPROACH) {
-                bs = layer_entry_distance(layers[cur], st.pos, st.dir, cfg.mask_tol_max);
-            }
+            visited |= (1ull << l0);
         }
+        for (uint32_t guard = 0; guard <= n_layers; ++guard) {
+            int best = -1;
+            T bs = static_cast<T>(1e30);
+            for (uint32_t li = 0; li < n_layers; ++li) {
+                if (visited & (1ull << li)) {
+                    continue;
+                }
+                const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
+                if (s < bs) {
+                    bs = s;
+                    best = static_cast<int>(li);
+                }
+            }
+            if (best < 0 || !(bs < static_cast<T>(1e29))) {
+                return NAV_EXIT;
+            }
+            while (bs > cfg.approach_dist && bs < static_cast<T>(1e29)) {
+                const T hl = step_limit();
+                if (!do_step(bs < hl ? bs : hl)) {
+                    return NAV_ABORT;
+                }
+                bs = layer_entry_distance(layers[best], st.pos, st.dir, cfg.mask_tol_max);
+            }
+            if (bs < static_cast<T>(1e29)) {
+                const int r = process_layer(static_cast<uint32_t>(best), -1, out_sf);
+                if (r == 0) {
+                    return NAV_FOUND;
+                }
+                if (r == 2) {
+                    return NAV_ABORT;
+                }
+            }
+            visited |= (1ull << best);
+        }
+        return NAV_EXIT;
     }
 };
 
