@@ -58,6 +58,9 @@ def main() -> int:
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events-per-core", type=int, default=6)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the "
+                         "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -67,6 +70,16 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if a.copy_calibration > 0:
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "dispatches": 3}), flush=True)
+        return 0
 
     from traccc_amd import ckf
     from traccc_amd import performance as PERF

@@ -858,6 +858,7 @@ struct link {
 struct track_out {
     uint32_t seed;
     double ndf, chi2, pval;
+    double vote_pval;  // p-value of the tip link's running sums (measurement votes)
     uint32_t nholes;
     std::vector<uint32_t> meas;
 };
@@ -1092,6 +1093,7 @@ std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& de
     result.reserve(tips.size());
     for (const auto& tip : tips) {
         link L = links[tip.first][tip.second];
+        const S vote_pval = prob<S>(static_cast<S>(L.chi2_sum), static_cast<S>(L.ndf_sum) - S(5));
         const uint32_t n_cands = tip.first + 1 - L.n_skipped;
         if (n_cands < cfg.min_track_candidates_per_track || n_cands > cfg.max_track_candidates_per_track)
             continue;
@@ -1118,8 +1120,43 @@ std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& de
         t.chi2 = static_cast<double>(chi2);
         t.pval = static_cast<double>(prob<S>(chi2, ndf));
         t.nholes = L.n_skipped;
+        t.vote_pval = static_cast<double>(vote_pval);
         t.meas = std::move(ids);
         result.push_back(std::move(t));
+    }
+
+    // Measurement-vote pruning (the reference runs it on the device chain only):
+    // device/common/include/traccc/finding/device/impl/gather_best_tips_per_measurement.ipp:27-180,
+    // gather_measurement_votes.ipp:19-47, update_tip_length_buffer.ipp:20-54, driven by
+    // device/common/src/finding/combinatorial_kalman_filter_algorithm.cpp:568-634.
+    // Every measurement keeps the max_num_tracks_per_measurement tracks with the
+    // highest p-value of their tip's running chi2 / ndf sums; a track survives
+    // when the measurements that kept it are at least
+    // min_measurement_voting_fraction of its measurements.  Equal p-values rank
+    // by lower track index (the reference leaves that order to its insertion
+    // mutex); the survivors keep their order.
+    if (cfg.max_num_tracks_per_measurement > 0u && !result.empty()) {
+        const uint32_t K = cfg.max_num_tracks_per_measurement;
+        std::vector<std::vector<std::pair<double, uint32_t>>> per_meas(n_meas);
+        for (uint32_t i = 0; i < result.size(); ++i) {
+            for (uint32_t m : result[i].meas) per_meas[m].push_back({result[i].vote_pval, i});
+        }
+        std::vector<uint32_t> votes(result.size(), 0u);
+        for (auto& v : per_meas) {
+            std::sort(v.begin(), v.end(), [](const std::pair<double, uint32_t>& a,
+                                             const std::pair<double, uint32_t>& b) {
+                return a.first > b.first || (a.first == b.first && a.second < b.second);
+            });
+            for (size_t k = 0; k < v.size() && k < K; ++k) ++votes[v[k].second];
+        }
+        std::vector<track_out> kept;
+        kept.reserve(result.size());
+        for (uint32_t i = 0; i < result.size(); ++i) {
+            const float frac = static_cast<float>(votes[i]) /
+                               static_cast<float>(result[i].meas.size());
+            if (!(frac < cfg.min_measurement_voting_fraction)) kept.push_back(std::move(result[i]));
+        }
+        result = std::move(kept);
     }
     return result;
 }

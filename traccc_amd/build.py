@@ -26,6 +26,7 @@ ARCH = os.environ.get("TRACCC_AMD_ARCH", "gfx950")
 HIP_SOURCES = [
     os.path.join(CSRC, "ckf", "runtime.hip"),
     os.path.join(CSRC, "ckf", "kernels.hip"),
+    os.path.join(CSRC, "ckf", "votes.hip"),
     os.path.join(CSRC, "common", "config.cpp"),
 ]
 SIM_SOURCES = [

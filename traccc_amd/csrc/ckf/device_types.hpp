@@ -145,6 +145,7 @@ struct ckf_params {
     uint32_t* out_const;          // [n_const]
     uint32_t out_cap;
     uint32_t out_const_cap;
+    uint64_t* vote_key;           // [n]; written when measurement votes run, else null
 };
 
 }  // namespace tamd

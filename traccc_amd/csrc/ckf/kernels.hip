@@ -761,6 +761,12 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
             continue;
         }
         const uint32_t moff = P.event_meas_off[ev];
+        if (P.vote_key != nullptr) {
+            // p-value of the tip's running sums ranks the track in measurement votes
+            const float vp = prob_chi2(L.chi2_sum, static_cast<float>(L.ndf_sum) - 5.f);
+            P.vote_key[pos] = (static_cast<uint64_t>(__float_as_uint(vp)) << 32) |
+                              static_cast<uint64_t>(0xffffffffu - pos);
+        }
         float ndf = 0.f, chi2 = 0.f;
         for (int k = static_cast<int>(nc) - 1; k >= 0; --k) {
             while (L.meas == kInvalid && link_step(L) != 0u) L = load_link(P, L.prev);

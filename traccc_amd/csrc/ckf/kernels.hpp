@@ -33,6 +33,49 @@ hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
 hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 
+// Measurement-vote pruning of built tracks (max_num_tracks_per_measurement > 0).
+struct vote_params {
+    step_ctl* ctl;                  // epoch and ticket[1] of the compaction scan
+    uint32_t n_tracks, n_events, K;
+    float min_fraction;
+    const uint32_t* track_off;      // [n_events + 1] built tracks per event
+    const uint32_t* event_meas_off; // [n_events]
+    const uint32_t* offsets;        // [n_tracks + 1]
+    const uint32_t* cons;           // event-relative measurement indices
+    const uint64_t* key;            // [n_tracks] (p-value bits << 32) | ~track
+    uint64_t* slots;                // [n_meas_total][K], zeroed
+    uint32_t* keep;                 // [n_tracks]
+    uint32_t* rank;                 // [n_tracks]
+    uint32_t* cpos;                 // [n_tracks]
+    uint32_t* totals;               // [2] kept tracks, kept constituents
+    uint32_t* new_track_off;        // [n_events + 1]
+    uint32_t* new_const_off;        // [n_events + 1]
+    uint32_t* lb_flag;
+    uint64_t* lb_agg;
+    uint64_t* lb_inc;
+    // built track columns
+    const uint16_t* outcome;
+    const uint32_t* surface;
+    const float* vec;
+    const float* cov;
+    const float* ndf;
+    const float* chi2;
+    const float* pval;
+    const uint32_t* nholes;
+    // kept track columns
+    uint16_t* o_outcome;
+    uint32_t* o_surface;
+    float* o_vec;
+    float* o_cov;
+    float* o_ndf;
+    float* o_chi2;
+    float* o_pval;
+    uint32_t* o_nholes;
+    uint32_t* o_offsets;
+    uint32_t* o_cons;
+};
+hipError_t launch_votes(const vote_params& V, uint32_t n_cu, hipStream_t stream);
+
 // diagnostics (include/traccc_amd/ckf_debug.h)
 hipError_t launch_debug_kf(uint32_t n, const float* vec, const float* cov, const float2* meas,
                            const float2* var, const uint32_t* info, float* chi2, int32_t* status,

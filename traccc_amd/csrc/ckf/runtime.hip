@@ -154,8 +154,10 @@ void validate_config(const ckf_finding_config& c) {
     if (c.run_pkf != 0u) {
         throw ckf_error(CKF_ERR_UNSUPPORTED, "the progressive Kalman filter is not provided");
     }
-    if (c.max_num_tracks_per_measurement != 0u) {
-        throw ckf_error(CKF_ERR_UNSUPPORTED, "measurement-vote pruning is not provided");
+    if (c.max_num_tracks_per_measurement != 0u &&
+        !(c.min_measurement_voting_fraction >= 0.f && c.min_measurement_voting_fraction <= 1.f)) {
+        throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                        "min_measurement_voting_fraction must lie in [0, 1]");
     }
     if (c.max_num_branches_per_surface == 0u || c.max_track_candidates_per_track == 0u ||
         c.max_track_candidates_per_track > 65535u) {
@@ -243,6 +245,8 @@ struct ckf_algorithm {
     dbuf alive_prefix, dedup_head, dedup_next, lb_flag[4], lb_agg[4], lb_inc[4], ctl;
     dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
         seg_coff, ev_toff, ev_coff;
+    // measurement votes (max_num_tracks_per_measurement > 0)
+    dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     tamd::step_ctl* h_ctl = nullptr;  // pinned mirror
     // captured step chunks: graph[1] with duplicate removal, graph[0] without
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -260,7 +264,8 @@ struct ckf_algorithm {
         dbuf* all[] = {&state[0], &state[1], &flt, &cand_meas, &cand_chi2, &links, &live, &tips,
                        &seed_first, &seed_base, &alive_prefix, &dedup_head, &dedup_next, &ctl,
                        &tip_keep, &tip_rank, &tip_cprefix, &seg_count, &seg_ccount, &seg_base,
-                       &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff};
+                       &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff, &vote_slots,
+                       &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
             lb_flag[k].release();
@@ -536,6 +541,92 @@ void grow(ckf_algorithm& A, uint32_t overflow) {
     if (overflow & 8u) A.tip_cap = up(A.tip_cap, uint64_t(c.n_tips) + c.n_tips_step);
 }
 
+// Measurement-vote pruning of the built tracks in T: the kept tracks move to
+// fresh result blocks, which T takes; the built blocks move to `old`, which
+// the caller keeps until the stream has passed the vote kernels.  The new
+// per-event offsets are left in A.vote_misc after the two totals.
+void run_votes(ckf_algorithm& A, const tamd::ckf_params& P, const ckf_device_event& ev,
+               ckf_device_tracks& T, ckf_device_tracks& K2) {
+    const ckf_finding_config& c = A.cfg;
+    const uint32_t nt = T.n_tracks;
+    const uint32_t nc = T.n_const;
+    const uint32_t K = c.max_num_tracks_per_measurement;
+    const size_t slot_bytes = 8u * size_t(K) * std::max<size_t>(ev.n_meas, 1u);
+    A.vote_slots.ensure(slot_bytes);
+    A.vote_keep.ensure(4u * size_t(nt));
+    A.vote_rank.ensure(4u * size_t(nt));
+    A.vote_cpos.ensure(4u * size_t(nt));
+    A.vote_misc.ensure(4u * (2u + 2u * (size_t(ev.n_events) + 1u)));
+    hip_check(hipMemsetAsync(A.vote_slots.p, 0, slot_bytes, A.stream), "memset");
+    hip_check(hipMemsetAsync(&P.ctl->ticket[1], 0, 4u, A.stream), "memset");
+
+    const size_t n1 = std::max<size_t>(nt, 1);
+    const size_t sizes[10] = {2u * n1, 4u * n1, 24u * n1, 144u * n1, 4u * n1, 4u * n1,
+                              4u * n1, 4u * n1, 4u * (size_t(nt) + 1u),
+                              4u * std::max<size_t>(nc, 1)};
+    void** slots[10] = {reinterpret_cast<void**>(&K2.outcome), reinterpret_cast<void**>(&K2.surface),
+                        reinterpret_cast<void**>(&K2.vec), reinterpret_cast<void**>(&K2.cov),
+                        reinterpret_cast<void**>(&K2.ndf), reinterpret_cast<void**>(&K2.chi2),
+                        reinterpret_cast<void**>(&K2.pval), reinterpret_cast<void**>(&K2.nholes),
+                        reinterpret_cast<void**>(&K2.offsets), reinterpret_cast<void**>(&K2.cons)};
+    for (int i = 0; i < 10; ++i) {
+        *slots[i] = result_cache().get(sizes[i]);
+        K2.track_bytes[i] = sizes[i];
+    }
+
+    tamd::vote_params V{};
+    V.ctl = P.ctl;
+    V.n_tracks = nt;
+    V.n_events = ev.n_events;
+    V.K = K;
+    V.min_fraction = c.min_measurement_voting_fraction;
+    V.track_off = P.event_track_off;
+    V.event_meas_off = P.event_meas_off;
+    V.offsets = T.offsets;
+    V.cons = T.cons;
+    V.key = A.vote_key.as<uint64_t>();
+    V.slots = A.vote_slots.as<uint64_t>();
+    V.keep = A.vote_keep.as<uint32_t>();
+    V.rank = A.vote_rank.as<uint32_t>();
+    V.cpos = A.vote_cpos.as<uint32_t>();
+    V.totals = A.vote_misc.as<uint32_t>();
+    V.new_track_off = V.totals + 2u;
+    V.new_const_off = V.new_track_off + (ev.n_events + 1u);
+    V.lb_flag = A.lb_flag[1].as<uint32_t>();
+    V.lb_agg = A.lb_agg[1].as<uint64_t>();
+    V.lb_inc = A.lb_inc[1].as<uint64_t>();
+    V.outcome = T.outcome;
+    V.surface = T.surface;
+    V.vec = T.vec;
+    V.cov = T.cov;
+    V.ndf = T.ndf;
+    V.chi2 = T.chi2;
+    V.pval = T.pval;
+    V.nholes = T.nholes;
+    V.o_outcome = K2.outcome;
+    V.o_surface = K2.surface;
+    V.o_vec = K2.vec;
+    V.o_cov = K2.cov;
+    V.o_ndf = K2.ndf;
+    V.o_chi2 = K2.chi2;
+    V.o_pval = K2.pval;
+    V.o_nholes = K2.nholes;
+    V.o_offsets = K2.offsets;
+    V.o_cons = K2.cons;
+    hip_check(tamd::launch_votes(V, static_cast<uint32_t>(A.n_cu), A.stream), "votes");
+    std::swap(T.outcome, K2.outcome);
+    std::swap(T.surface, K2.surface);
+    std::swap(T.vec, K2.vec);
+    std::swap(T.cov, K2.cov);
+    std::swap(T.ndf, K2.ndf);
+    std::swap(T.chi2, K2.chi2);
+    std::swap(T.pval, K2.pval);
+    std::swap(T.nholes, K2.nholes);
+    std::swap(T.offsets, K2.offsets);
+    std::swap(T.cons, K2.cons);
+    for (int i = 0; i < 10; ++i) std::swap(T.track_bytes[i], K2.track_bytes[i]);
+}
+
 ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
                                const ckf_device_event& ev) {
     const ckf_finding_config& c = A.cfg;
@@ -620,16 +711,33 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
         P.out_const = T->cons;
         P.out_cap = nt;
         P.out_const_cap = nc;
+        const bool votes = c.max_num_tracks_per_measurement > 0u && nt > 0u;
+        if (votes) {
+            A.vote_key.ensure(8u * size_t(nt));
+            P.vote_key = A.vote_key.as<uint64_t>();
+        }
         hip_check(tamd::launch_build_write(P, shape, A.stream), "build write");
+        const uint32_t* track_off = P.event_track_off;
+        const uint32_t* const_off = P.event_const_off;
+        ckf_device_tracks built;  // blocks the vote kernels read, released after the sync
+        if (votes) {
+            run_votes(A, P, ev, *T, built);
+            track_off = A.vote_misc.as<uint32_t>() + 2u;
+            const_off = track_off + (ev.n_events + 1u);
+        }
         T->ev_track_off.resize(ev.n_events + 1u);
         T->ev_const_off.resize(ev.n_events + 1u);
-        hip_check(hipMemcpyAsync(T->ev_track_off.data(), P.event_track_off,
-                                 4u * (ev.n_events + 1u), hipMemcpyDeviceToHost, A.stream),
+        hip_check(hipMemcpyAsync(T->ev_track_off.data(), track_off, 4u * (ev.n_events + 1u),
+                                 hipMemcpyDeviceToHost, A.stream),
                   "offsets");
-        hip_check(hipMemcpyAsync(T->ev_const_off.data(), P.event_const_off,
-                                 4u * (ev.n_events + 1u), hipMemcpyDeviceToHost, A.stream),
+        hip_check(hipMemcpyAsync(T->ev_const_off.data(), const_off, 4u * (ev.n_events + 1u),
+                                 hipMemcpyDeviceToHost, A.stream),
                   "offsets");
         hip_check(hipStreamSynchronize(A.stream), "build write sync");
+        if (votes) {
+            T->n_tracks = T->ev_track_off[ev.n_events];
+            T->n_const = T->ev_const_off[ev.n_events];
+        }
     } catch (...) {
         delete T;
         throw;
