@@ -1161,6 +1161,143 @@ std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& de
     return result;
 }
 
+// ---------------------------------------------------------------------------
+// Progressive Kalman filter (finding_config.run_pkf): one branch per seed that
+// takes the best measurement of every sensitive surface it reaches.
+//   run_progressive_kalman_filter  core/include/traccc/finding/details/run_progressive_kalman_filter.hpp:55-258
+//   progressive_kalman_filter      core/include/traccc/finding/details/progressive_kalman_filter.hpp:53-209
+//   measurement_updater actor      core/include/traccc/finding/actors/measurement_kalman_updater.hpp:94-260
+//   find_optimal_measurement       core/include/traccc/finding/measurement_selector.hpp:224-264
+// On every surface (the seed's included) the material noise comes first, then
+// the update on the lowest-chi2 measurement (first index on ties, the first
+// one at or below FLT_EPSILON ends the search); a chi2 above chi2_max or a
+// failed update is a hole.  The walk ends (a finished track) on too many
+// holes / consecutive holes, on max_track_candidates_per_track states or when
+// the navigation leaves the detector; it fails on an aborted propagation or a
+// total path beyond path_limit.  Every seed yields one entry; a failed or too
+// short track is an empty entry (as the reference pushes a default track).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoSeed = 0xffffffffu;
+
+template <typename S>
+std::vector<track_out> pkf(const ckf_finding_config& cfg, const ckf_detector& det,
+                           const ckf_bfield& field, const ckf_measurements& meas,
+                           const ckf_bound_params& seeds, run_stats& stats) {
+    const uint32_t n_meas = meas.size;
+    const S mass = static_cast<S>(cfg.ptc_mass);
+    const S abs_q = static_cast<S>(std::fabs(cfg.ptc_charge));
+    const S chi2_max = static_cast<S>(cfg.chi2_max);
+    const S eps = static_cast<S>(std::numeric_limits<float>::epsilon());
+    std::vector<uint32_t> meas_ranges(det.n_surfaces);
+    for (uint32_t s = 0; s < det.n_surfaces; ++s) {
+        if (!det.surfaces[s].sensitive) {
+            meas_ranges[s] = s == 0 ? 0u : meas_ranges[s - 1];
+            continue;
+        }
+        meas_ranges[s] = static_cast<uint32_t>(
+            std::upper_bound(meas.surface_link, meas.surface_link + n_meas,
+                             det.surfaces[s].identifier) - meas.surface_link);
+    }
+    // legs between surfaces: no minimum step and no per-leg step budget (the
+    // PKF propagation has no ckf_aborter); the path limit holds for the walk
+    ckf_finding_config leg = cfg;
+    leg.min_step_length_for_next_surface = 0.f;
+    leg.max_step_counts_for_next_surface = 0x7fffffffu;
+    auto momentum_ok = [&](const bound_state<S>& b, S div) {
+        const S p = abs_q / std::fabs(b.vec.a[4][0]);
+        const S pT = p * std::sin(b.vec.a[3][0]);
+        return pT > static_cast<S>(cfg.min_pT) / div && p > static_cast<S>(cfg.min_p) / div;
+    };
+
+    std::vector<track_out> result(seeds.size);
+    for (uint32_t si = 0; si < seeds.size; ++si) {
+        track_out& t = result[si];
+        t.seed = kNoSeed;
+        bound_state<S> b;
+        b.surface = surface_index(det, seeds.surface_link[si]);
+        for (int k = 0; k < 6; ++k) b.vec.a[k][0] = static_cast<S>(seeds.vector[6 * si + k]);
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) b.cov.a[r][c] = static_cast<S>(seeds.covariance[36 * si + 6 * r + c]);
+        if (!momentum_ok(b, S(10))) continue;
+        std::vector<uint32_t> states;
+        uint32_t holes = 0u, consec = 0u, ndf_sum = 0u;
+        S chi2_sum = S(0), path = S(0);
+        bool good = false;
+        for (;;) {
+            const ckf_surface& sf = det.surfaces[b.surface];
+            add_material<S>(sf, b, mass, abs_q);
+            // find_optimal_measurement
+            const uint32_t lo = b.surface == 0 ? 0u : meas_ranges[b.surface - 1];
+            const uint32_t up = meas_ranges[b.surface];
+            uint32_t best = kNoSeed;
+            S best_chi2 = std::numeric_limits<S>::max();
+            for (uint32_t mi = lo; mi < up; ++mi) {
+                const S chi2 = predicted_chi2<S>(load_meas<S>(meas, mi), b);
+                if (chi2 < best_chi2 && chi2 >= S(0)) {
+                    best = mi;
+                    best_chi2 = chi2;
+                    if (best_chi2 <= eps) break;
+                }
+            }
+            int res = CKF_FIT_ERROR_OTHER;
+            if (best != kNoSeed && best_chi2 <= chi2_max) {
+                const meas_view<S> mv = load_meas<S>(meas, best);
+                if (best_chi2 <= eps && !(sf.thickness_x0 > 0.f)) {
+                    b.cov.a[0][0] = mv.var[0];
+                    b.cov.a[1][1] = mv.var[1];
+                    res = CKF_FIT_SUCCESS;
+                } else {
+                    bound_state<S> f;
+                    res = gain_update<S>(mv, b, f);
+                    if (res == CKF_FIT_SUCCESS) b = f;
+                }
+            }
+            bool exit = false;
+            if (res != CKF_FIT_SUCCESS) {
+                ++holes;
+                ++consec;
+                exit = holes > cfg.max_num_skipping_per_cand || consec > cfg.max_num_consecutive_skipped;
+            } else {
+                states.push_back(best);
+                consec = 0u;
+                ndf_sum += meas.dimensions[best];
+                chi2_sum += best_chi2;
+                exit = states.size() >= cfg.max_track_candidates_per_track;
+            }
+            if (exit) {
+                good = true;
+                break;
+            }
+            bound_state<S> o;
+            S leg_path = S(0);
+            ++stats.props;
+            const int r = propagate_bound<S>(leg, det, field, b, mass, abs_q, o, &leg_path, nullptr);
+            path += leg_path;
+            if (r == 1) {
+                good = true;  // left the detector
+                break;
+            }
+            if (r != 0 || std::fabs(path) > static_cast<S>(cfg.propagation.path_limit)) break;
+            b = o;
+        }
+        stats.steps = std::max<uint32_t>(stats.steps, static_cast<uint32_t>(states.size() + holes));
+        if (!good || !momentum_ok(b, S(1)) || states.size() < cfg.min_track_candidates_per_track ||
+            static_cast<int>(ndf_sum) - 5 < 0) {
+            continue;
+        }
+        const S ndf = static_cast<S>(static_cast<int>(ndf_sum) - 5);
+        t.seed = si;
+        t.ndf = static_cast<double>(ndf);
+        t.chi2 = static_cast<double>(chi2_sum);
+        t.pval = static_cast<double>(prob<S>(chi2_sum, ndf));
+        t.vote_pval = t.pval;
+        t.nholes = holes;
+        t.meas = std::move(states);
+    }
+    stats.tips = seeds.size;
+    return result;
+}
+
 template <typename T>
 T* alloc_n(size_t n) {
     return static_cast<T*>(std::calloc(n ? n : 1, sizeof(T)));
@@ -1186,6 +1323,12 @@ void fill_out(const std::vector<track_out>& tr, const ckf_bound_params& seeds, c
     uint32_t off = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const track_out& t = tr[i];
+        if (t.seed == kNoSeed) {
+            // an empty entry (progressive filter without a track): default parameters
+            out->params_surface_link[i] = ~0ull;
+            out->constituent_offsets[i] = off;
+            continue;
+        }
         out->params_surface_link[i] = seeds.surface_link[t.seed];
         std::memcpy(out->params_vector + 6u * i, seeds.vector + 6u * t.seed, 6u * sizeof(float));
         std::memcpy(out->params_covariance + 36u * i, seeds.covariance + 36u * t.seed, 36u * sizeof(float));
@@ -1213,9 +1356,17 @@ int oracle_ckf_run(const ckf_finding_config* cfg, const ckf_detector* det, const
     if (!cfg || !det || !field || !meas || !seeds || !out) return CKF_ERR_INVALID_ARGUMENT;
     if (cfg->min_track_candidates_per_track == 0u) return CKF_ERR_INVALID_ARGUMENT;
     oracle::run_stats st;
-    std::vector<oracle::track_out> tr =
-        precision == 32 ? oracle::ckf<float>(*cfg, *det, *field, *meas, *seeds, st)
-                        : oracle::ckf<double>(*cfg, *det, *field, *meas, *seeds, st);
+    // the operator's dispatch: core/src/finding/combinatorial_kalman_filter_algorithm.cpp:47-57
+    const bool use_pkf = cfg->run_pkf && (cfg->max_num_branches_per_seed == 1u ||
+                                          cfg->max_num_branches_per_surface == 1u);
+    std::vector<oracle::track_out> tr;
+    if (use_pkf) {
+        tr = precision == 32 ? oracle::pkf<float>(*cfg, *det, *field, *meas, *seeds, st)
+                             : oracle::pkf<double>(*cfg, *det, *field, *meas, *seeds, st);
+    } else {
+        tr = precision == 32 ? oracle::ckf<float>(*cfg, *det, *field, *meas, *seeds, st)
+                             : oracle::ckf<double>(*cfg, *det, *field, *meas, *seeds, st);
+    }
     oracle::fill_out(tr, *seeds, st, out);
     return CKF_OK;
 }

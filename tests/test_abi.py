@@ -80,6 +80,7 @@ def test_config_defaults_and_validation():
     with pytest.raises(ValueError):
         ckf.combinatorial_kalman_filter_algorithm(bad)
     bad = ckf.finding_config()
-    bad.run_pkf = 1
-    with pytest.raises(ckf.CKFError):
+    bad.max_num_tracks_per_measurement = 2
+    bad.min_measurement_voting_fraction = 1.5  # a fraction above one
+    with pytest.raises(ValueError):
         ckf.combinatorial_kalman_filter_algorithm(bad)

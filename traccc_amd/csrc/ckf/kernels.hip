@@ -25,6 +25,7 @@
 #include "device_types.hpp"
 #include "kalman_device.hpp"
 #include "kernels.hpp"
+#include "state_io.hpp"
 #include "wave.hpp"
 
 namespace tamd {
@@ -32,80 +33,6 @@ namespace tamd {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ inline void load_soa(const float* __restrict__ base, uint32_t cap, uint32_t i, float* x,
-                                float* c, uint32_t& sf) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        x[k] = base[static_cast<size_t>(k) * cap + i];
-    }
-#pragma unroll
-    for (int k = 0; k < 21; ++k) {
-        c[k] = base[static_cast<size_t>(6 + k) * cap + i];
-    }
-    sf = __float_as_uint(base[static_cast<size_t>(27) * cap + i]);
-}
-
-__device__ inline void store_soa(float* __restrict__ base, uint32_t cap, uint32_t i, const float* x,
-                                 const float* c, uint32_t sf) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        base[static_cast<size_t>(k) * cap + i] = x[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 21; ++k) {
-        base[static_cast<size_t>(6 + k) * cap + i] = c[k];
-    }
-    base[static_cast<size_t>(27) * cap + i] = __uint_as_float(sf);
-}
-
-__device__ inline void load_seed(const ckf_params& P, uint32_t s, float* x, float* c, uint32_t& sf) {
-    const float* v = P.seed_vec + 6u * static_cast<size_t>(s);
-    const float* m = P.seed_cov + 36u * static_cast<size_t>(s);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        x[k] = v[k];
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-#pragma unroll
-        for (int j = i; j < 6; ++j) {
-            c[sym_idx(i, j)] = m[6 * i + j];
-        }
-    }
-    sf = P.seed_surface[s];
-}
-
-__device__ inline void load_ref(const ckf_params& P, uint32_t ref, uint32_t in_parity, float* x,
-                                float* c, uint32_t& sf) {
-    const uint32_t kind = ref >> kRefKindShift;
-    const uint32_t idx = ref & kRefMask;
-    if (kind == kRefFiltered) {
-        load_soa(P.flt, P.flt_cap, idx, x, c, sf);
-    } else if (kind == kRefInput) {
-        load_soa(P.state[in_parity], P.state_cap, idx, x, c, sf);
-    } else {
-        load_seed(P, idx, x, c, sf);
-    }
-}
-
-// Multiple-scattering noise of the module the state sits on (covariance part
-// of the pointwise material interaction).
-__device__ inline void add_material(const ckf_params& P, uint32_t sf, const float* x, float* c) {
-    const float t = P.sfs[sf].thickness_x0;
-    if (!(t > 0.f)) {
-        return;
-    }
-    const ckf_surface& S = P.sfs[sf];
-    float sp, cp, st, ct;
-    sincosf(x[e_phi], &sp, &cp);
-    sincosf(x[e_theta], &st, &ct);
-    const float cosi = fabsf(cp * st * S.normal[0] + sp * st * S.normal[1] + ct * S.normal[2]);
-    const float t0 = highland_theta0<float>(t / cosi, x[e_qop], P.nav.mass, P.nav.abs_q);
-    const float v = t0 * t0;
-    c[sym_idx(e_phi, e_phi)] += v / (st * st);
-    c[sym_idx(e_theta, e_theta)] += v;
-}
 
 __device__ inline void set_overflow(step_ctl* ctl, uint32_t bit) { atomicOr(&ctl->overflow, bit); }
 
@@ -484,53 +411,20 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                     if (step == 0u) {
                         add_material(P, sf, x, c);
                     }
-                    navigator<float, true> nav;
-                    nav.sfs = P.sfs;
-                    nav.layers = s_layers;
-                    nav.n_layers = P.n_layers;
-                    nav.field = P.field;
-                    nav.cfg = P.nav;
-                    bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
-                    uint32_t end_sf = 0u;
+                    uint32_t n_rk = 0u;
+                    float leg = 0.f;
                     propagated = true;
-                    const int status = nav.propagate(static_cast<int>(sf), end_sf);
+                    const int status = transport_state(P, s_layers, P.nav, x, c, sf, n_rk, leg);
                     const bool success = (status == NAV_FOUND);
                     bool ok = success;
                     if (success) {
-                        float nx[6];
-                        float Jb[6][6];
-                        free_to_bound<float, true>(P.sfs[end_sf], nav.st, nav.J, P.field, P.nav.mass,
-                                                   P.nav.abs_q, nx, Jb);
-                        // C' = Jb C Jb^T
-                        float M[6][6];
-#pragma unroll
-                        for (int r = 0; r < 6; ++r) {
-#pragma unroll
-                            for (int q = 0; q < 6; ++q) {
-                                float acc = 0.f;
-#pragma unroll
-                                for (int k = 0; k < 6; ++k) acc += Jb[r][k] * c[sym_idx(k, q)];
-                                M[r][q] = acc;
-                            }
-                        }
-                        float nc[21];
-#pragma unroll
-                        for (int r = 0; r < 6; ++r) {
-#pragma unroll
-                            for (int q = r; q < 6; ++q) {
-                                float acc = 0.f;
-#pragma unroll
-                                for (int k = 0; k < 6; ++k) acc += M[r][k] * Jb[q][k];
-                                nc[sym_idx(r, q)] = acc;
-                            }
-                        }
-                        add_material(P, end_sf, nx, nc);
-                        const float th = nx[e_theta];
+                        add_material(P, sf, x, c);
+                        const float th = x[e_theta];
                         if (th <= 0.f || th >= 6.28318530717958647692f) ok = false;
-                        if (!isfinite(nx[e_phi])) ok = false;
-                        if (fabsf(nx[e_qop]) == 0.f) ok = false;
+                        if (!isfinite(x[e_phi])) ok = false;
+                        if (fabsf(x[e_qop]) == 0.f) ok = false;
                         if (ok) {
-                            store_soa(P.state[out_par], P.state_cap, i, nx, nc, end_sf);
+                            store_soa(P.state[out_par], P.state_cap, i, x, c, sf);
                             succ = true;
                         }
                     }

@@ -76,6 +76,24 @@ struct vote_params {
 };
 hipError_t launch_votes(const vote_params& V, uint32_t n_cu, hipStream_t stream);
 
+// Progressive Kalman filter: per-seed walk + constituent scan, then the
+// output fill (after the host has sized the result from ctl->n_constituents).
+struct pkf_params {
+    uint32_t* states;    // [n_seeds][max_states] global measurement indices
+    uint32_t* n_states;  // [n_seeds]; 0 = empty entry
+    float* chi2;         // [n_seeds]
+    uint32_t* ndf;       // [n_seeds] sum of measurement dimensions
+    uint32_t* holes;     // [n_seeds]
+    uint32_t* cpos;      // [n_seeds] constituent offset
+    uint32_t max_states;
+    uint32_t* lb_flag;
+    uint64_t* lb_agg;
+    uint64_t* lb_inc;
+};
+hipError_t launch_pkf(const ckf_params& P, const pkf_params& Q, uint32_t n_cu, hipStream_t stream);
+hipError_t launch_pkf_write(const ckf_params& P, const pkf_params& Q, uint32_t n_cu,
+                            hipStream_t stream);
+
 // diagnostics (include/traccc_amd/ckf_debug.h)
 hipError_t launch_debug_kf(uint32_t n, const float* vec, const float* cov, const float2* meas,
                            const float2* var, const uint32_t* info, float* chi2, int32_t* status,

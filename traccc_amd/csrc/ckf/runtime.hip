@@ -151,9 +151,6 @@ void validate_config(const ckf_finding_config& c) {
         throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
                         "a Kalman smoother needs a fitter, which this algorithm does not own");
     }
-    if (c.run_pkf != 0u) {
-        throw ckf_error(CKF_ERR_UNSUPPORTED, "the progressive Kalman filter is not provided");
-    }
     if (c.max_num_tracks_per_measurement != 0u &&
         !(c.min_measurement_voting_fraction >= 0.f && c.min_measurement_voting_fraction <= 1.f)) {
         throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
@@ -247,6 +244,9 @@ struct ckf_algorithm {
         seg_coff, ev_toff, ev_coff;
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
+    // progressive filter (run_pkf)
+    dbuf pkf_states, pkf_n, pkf_chi2, pkf_ndf, pkf_holes, pkf_cpos, pkf_lb_flag, pkf_lb_agg,
+        pkf_lb_inc;
     tamd::step_ctl* h_ctl = nullptr;  // pinned mirror
     // captured step chunks: graph[1] with duplicate removal, graph[0] without
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -265,7 +265,9 @@ struct ckf_algorithm {
                        &seed_first, &seed_base, &alive_prefix, &dedup_head, &dedup_next, &ctl,
                        &tip_keep, &tip_rank, &tip_cprefix, &seg_count, &seg_ccount, &seg_base,
                        &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff, &vote_slots,
-                       &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc};
+                       &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
+                       &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
+                       &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
             lb_flag[k].release();
@@ -627,9 +629,140 @@ void run_votes(ckf_algorithm& A, const tamd::ckf_params& P, const ckf_device_eve
     for (int i = 0; i < 10; ++i) std::swap(T.track_bytes[i], K2.track_bytes[i]);
 }
 
+// The operator's dispatch (core/src/finding/combinatorial_kalman_filter_algorithm.cpp:47-57):
+// the progressive filter runs when asked for and at most one branch is kept.
+bool use_pkf(const ckf_finding_config& c) {
+    return c.run_pkf != 0u &&
+           (c.max_num_branches_per_seed == 1u || c.max_num_branches_per_surface == 1u);
+}
+
+ckf_device_tracks* find_tracks_pkf(ckf_algorithm& A, const ckf_device_detector& D,
+                                   const ckf_device_event& ev) {
+    const ckf_finding_config& c = A.cfg;
+    const uint32_t ns = ev.n_seeds;
+    const size_t n1 = std::max<size_t>(ns, 1u);
+    const uint32_t max_states = c.max_track_candidates_per_track;
+    A.pkf_states.ensure(4u * n1 * max_states);
+    A.pkf_n.ensure(4u * n1);
+    A.pkf_chi2.ensure(4u * n1);
+    A.pkf_ndf.ensure(4u * n1);
+    A.pkf_holes.ensure(4u * n1);
+    A.pkf_cpos.ensure(4u * n1);
+    const uint32_t lb = parts(ns) + 1u;
+    {
+        const size_t old = A.pkf_lb_flag.bytes;
+        A.pkf_lb_flag.ensure(4u * size_t(lb));
+        if (A.pkf_lb_flag.bytes != old) {
+            // fresh descriptors carry epoch 0, which no launch uses
+            hip_check(hipMemsetAsync(A.pkf_lb_flag.p, 0, A.pkf_lb_flag.bytes, A.stream), "memset");
+        }
+    }
+    A.pkf_lb_agg.ensure(8u * size_t(lb));
+    A.pkf_lb_inc.ensure(8u * size_t(lb));
+    A.ctl.ensure(sizeof(tamd::step_ctl));
+    A.ev_toff.ensure(4u * (size_t(ev.n_events) + 1u));
+    A.ev_coff.ensure(4u * (size_t(ev.n_events) + 1u));
+    if (A.h_ctl == nullptr) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
+                                hipHostMallocDefault),
+                  "hipHostMalloc");
+    }
+    A.stats = ckf_run_stats{};
+    tamd::ckf_params P = make_params(A, D, ev);
+    tamd::pkf_params Q{};
+    Q.states = A.pkf_states.as<uint32_t>();
+    Q.n_states = A.pkf_n.as<uint32_t>();
+    Q.chi2 = A.pkf_chi2.as<float>();
+    Q.ndf = A.pkf_ndf.as<uint32_t>();
+    Q.holes = A.pkf_holes.as<uint32_t>();
+    Q.cpos = A.pkf_cpos.as<uint32_t>();
+    Q.max_states = max_states;
+    Q.lb_flag = A.pkf_lb_flag.as<uint32_t>();
+    Q.lb_agg = A.pkf_lb_agg.as<uint64_t>();
+    Q.lb_inc = A.pkf_lb_inc.as<uint64_t>();
+
+    tamd::step_ctl init{};
+    init.epoch = A.epoch_next;
+    std::memcpy(A.h_ctl, &init, sizeof(init));
+    hip_check(hipMemcpyAsync(P.ctl, A.h_ctl, sizeof(init), hipMemcpyHostToDevice, A.stream),
+              "ctl upload");
+    if (ns > 0u) {
+        hip_check(tamd::launch_pkf(P, Q, static_cast<uint32_t>(A.n_cu), A.stream), "pkf");
+    }
+    hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
+                             A.stream),
+              "ctl readback");
+    hip_check(hipStreamSynchronize(A.stream), "pkf sync");
+    A.epoch_next = A.h_ctl->epoch + 2u;
+    const uint32_t nc = ns > 0u ? A.h_ctl->n_constituents : 0u;
+    A.stats.n_steps = 1u;
+    A.stats.n_tips = ns;
+
+    auto* T = new ckf_device_tracks();
+    try {
+        T->n_events = ev.n_events;
+        T->n_tracks = ns;
+        T->n_const = nc;
+        T->ids = D.ids;
+        const size_t sizes[10] = {2u * n1, 4u * n1, 24u * n1, 144u * n1, 4u * n1, 4u * n1,
+                                  4u * n1, 4u * n1, 4u * (size_t(ns) + 1u),
+                                  4u * std::max<size_t>(nc, 1)};
+        void** slots[10] = {reinterpret_cast<void**>(&T->outcome), reinterpret_cast<void**>(&T->surface),
+                            reinterpret_cast<void**>(&T->vec), reinterpret_cast<void**>(&T->cov),
+                            reinterpret_cast<void**>(&T->ndf), reinterpret_cast<void**>(&T->chi2),
+                            reinterpret_cast<void**>(&T->pval), reinterpret_cast<void**>(&T->nholes),
+                            reinterpret_cast<void**>(&T->offsets), reinterpret_cast<void**>(&T->cons)};
+        for (int i = 0; i < 10; ++i) {
+            *slots[i] = result_cache().get(sizes[i]);
+            T->track_bytes[i] = sizes[i];
+        }
+        P.out_outcome = T->outcome;
+        P.out_surface = T->surface;
+        P.out_vec = T->vec;
+        P.out_cov = T->cov;
+        P.out_ndf = T->ndf;
+        P.out_chi2 = T->chi2;
+        P.out_pval = T->pval;
+        P.out_nholes = T->nholes;
+        P.out_offsets = T->offsets;
+        P.out_const = T->cons;
+        P.out_cap = ns;
+        P.out_const_cap = nc;
+        T->ev_track_off.assign(ev.seed_off.begin(), ev.seed_off.end());
+        T->ev_const_off.assign(ev.n_events + 1u, 0u);
+        if (ns > 0u) {
+            hip_check(tamd::launch_pkf_write(P, Q, static_cast<uint32_t>(A.n_cu), A.stream),
+                      "pkf write");
+            hip_check(hipMemcpyAsync(T->ev_const_off.data(), P.event_const_off,
+                                     4u * ev.n_events, hipMemcpyDeviceToHost, A.stream),
+                      "offsets");
+            hip_check(hipStreamSynchronize(A.stream), "pkf write sync");
+        } else {
+            hip_check(hipMemcpyAsync(T->offsets, T->ev_const_off.data(), 4u, hipMemcpyHostToDevice,
+                                     A.stream),
+                      "offsets");
+            hip_check(hipStreamSynchronize(A.stream), "pkf write sync");
+        }
+        // events without seeds start where the next event starts
+        T->ev_const_off[ev.n_events] = nc;
+        for (uint32_t e = ev.n_events; e-- > 0u;) {
+            if (ev.seed_off[e] == ev.seed_off[e + 1u]) {
+                T->ev_const_off[e] = T->ev_const_off[e + 1u];
+            }
+        }
+    } catch (...) {
+        delete T;
+        throw;
+    }
+    return T;
+}
+
 ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
                                const ckf_device_event& ev) {
     const ckf_finding_config& c = A.cfg;
+    if (use_pkf(c)) {
+        return find_tracks_pkf(A, D, ev);
+    }
     // initial capacities; grown and re-run on overflow
     const uint32_t ns = std::max(ev.n_seeds, 1u);
     const uint64_t branchy = std::min<uint64_t>(c.max_num_branches_per_seed,
@@ -1043,7 +1176,10 @@ ckf_status ckf_tracks_download(const ckf_device_tracks* trk, ckf_track_candidate
         if (out->params_surface_link != nullptr && nt > 0) {
             std::vector<uint32_t> sf(nt);
             d2h(sf.data(), trk->surface, 4u * nt);
-            for (size_t i = 0; i < nt; ++i) out->params_surface_link[i] = trk->ids[sf[i]];
+            for (size_t i = 0; i < nt; ++i) {
+                // an empty progressive-filter entry has no surface
+                out->params_surface_link[i] = sf[i] < trk->ids.size() ? trk->ids[sf[i]] : ~0ull;
+            }
         }
         if (out->constituent_type != nullptr) {
             std::fill(out->constituent_type, out->constituent_type + nc, uint16_t(0));
