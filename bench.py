@@ -129,9 +129,17 @@ def main() -> int:
     alg_bytes = {"propagate": prop_bytes, "find": find_bytes}.get(dom, prop_bytes)
     avg_ms = groups[dom] / launches
     achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # measured HBM bytes per launch of that kernel: FETCH_SIZE / WRITE_SIZE
+    # counter runs over this command, scaled by a 1 GiB copy (profiles/)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{a.workload}_10k_muons.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("kernels", {}).get(f"k_{dom}", {}).get(
+                "hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+                "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
                 "bytes_per_launch": int(alg_bytes / launches),
                 "kernel_ms_per_call": {k: round(v, 4) for k, v in groups.items()},
                 "launches_per_call": launches}
