@@ -125,11 +125,13 @@ TAMD_HD bool normalize_angles(float& phi, float& theta) {
     return true;
 }
 
-// Gain-matrix update in place: x (6), c (21 packed).  Returns a
-// ckf_fitter_status; x and c are left untouched on failure.
-TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
-    const uint32_t a = m.a;
-    const uint32_t b = m.b;
+// Gain-matrix update in place for measured bound indices (a, b): x (6),
+// c (21 packed).  Returns a ckf_fitter_status; x and c are left untouched on
+// failure.
+#if defined(__HIPCC__)
+__attribute__((always_inline))
+#endif
+TAMD_HD int gain_update_ab(const meas_data& m, uint32_t a, uint32_t b, float* x, float* c) {
     float pa[6], pb[6];  // columns a and b of P = P H^T
     cov_col(c, a, pa);
     cov_col(c, b, pb);
@@ -213,6 +215,16 @@ TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
         c[i] = nc[i];
     }
     return CKF_FIT_SUCCESS;
+}
+
+// The (loc0, loc1) pixel measurement gets its own copy with the subspace as
+// constants, which folds every column / element select away; any other
+// subspace takes the general copy.
+TAMD_HD int gain_update(const meas_data& m, float* x, float* c) {
+    if (m.a == 0u && m.b == 1u) {
+        return gain_update_ab(m, 0u, 1u, x, c);
+    }
+    return gain_update_ab(m, m.a, m.b, x, c);
 }
 
 // chi2 p-value Q(ndf / 2, chi2 / 2): the reference's split between a power

@@ -46,9 +46,12 @@ __device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.li
 // A team of kTeam lanes handles one input branch: the lanes split the
 // surface's measurements, reduce the best (chi2, index) candidate with
 // shuffles, and run the gain update redundantly (so the team stays uniform);
-// the team leader writes.  A wave holds kWave / kTeam inputs.
-constexpr int kTeam = 16;
+// the team leader writes.  A wave holds kWave / kTeam inputs.  Each lane
+// reads kBatch measurements per round with all loads issued before the chi2
+// arithmetic, so a round costs one memory latency.
+constexpr int kTeam = 4;
 constexpr int kTeamsPerWave = kWave / kTeam;
+constexpr int kBatch = 4;
 
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     step_ctl* ctl = P.ctl;
@@ -105,21 +108,25 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             while (count < N) {
                 float best_c = 0.f;
                 uint32_t best_m = kInvalid;
-                for (uint32_t m = lo + tl; m < up; m += kTeam) {
-                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, m);
-                    const float chi2 = predicted_chi2(md, x, c);
-                    if (!(chi2 <= P.chi2_max && chi2 >= 0.f)) {
-                        continue;  // outside the cut (a NaN never yields a link either)
+                for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
+                    meas_data mb[kBatch];
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u) {
+                        const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+                        mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
                     }
-                    if (rule0 && chi2 != 0.f) {
-                        continue;
-                    }
-                    if (have_last && (chi2 < last_c || (chi2 == last_c && m <= last_m))) {
-                        continue;
-                    }
-                    if (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m)) {
-                        best_c = chi2;
-                        best_m = m;
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u) {
+                        const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+                        const float chi2 = predicted_chi2(mb[u], x, c);
+                        // outside the cut (a NaN never yields a link either)
+                        bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
+                        take = take && !(rule0 && chi2 != 0.f);
+                        take = take && !(have_last && (chi2 < last_c || (chi2 == last_c && m <= last_m)));
+                        if (take && (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m))) {
+                            best_c = chi2;
+                            best_m = m;
+                        }
                     }
                 }
                 // team minimum of (chi2, index); kInvalid sorts last

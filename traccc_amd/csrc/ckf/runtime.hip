@@ -289,9 +289,9 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     const uint32_t N = A.cfg.max_num_branches_per_surface;
     A.flt_cap = static_cast<uint32_t>(std::min<uint64_t>(uint64_t(A.in_cap) * N, 0x3fffffffu));
     A.state_cap = A.flt_cap;
-    // look-back partitions: k_find takes 4 inputs per wave (16-lane teams),
+    // look-back partitions: k_find takes 16 inputs per wave (4-lane teams),
     // the other kernels 64 items per wave
-    A.lb_cap = std::max({(A.in_cap + 3u) / 4u, parts(A.state_cap), parts(A.tip_cap)}) + 1u;
+    A.lb_cap = std::max({(A.in_cap + 15u) / 16u, parts(A.state_cap), parts(A.tip_cap)}) + 1u;
     const size_t st = size_t(tamd::kStateCols) * 4u;
     A.state[0].ensure(st * A.state_cap);
     A.state[1].ensure(st * A.state_cap);
@@ -420,7 +420,7 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
         return std::max(1u, std::min(b, max_blocks));
     };
     tamd::launch_shape s;
-    s.find_blocks = std::max(1u, std::min((((A.in_cap + 3u) / 4u) + 3u) / 4u, max_blocks));
+    s.find_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
     s.prop_blocks = blocks(A.state_cap);
     s.build_blocks = blocks(A.tip_cap);

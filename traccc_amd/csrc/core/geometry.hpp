@@ -85,14 +85,27 @@ TAMD_HD T layer_entry_distance(const ckf_layer& L, const vec3<T>& p,
     const T r1 = static_cast<T>(L.env_r[1]);
     const T z0 = static_cast<T>(L.env_z[0]);
     const T z1 = static_cast<T>(L.env_z[1]);
+    // Envelopes the line cannot reach, before any division: beyond the
+    // widened z range and not moving towards it (z(s) only moves away, every
+    // wall crossing then fails its z test and both z faces lie behind), or
+    // outside the widened outer radius and not moving inwards (r(s) only
+    // grows, so every crossing fails its radius test).  Both give inf, as
+    // the full computation below would.
+    if ((p.z > z1 + margin && d.z >= T(0)) || (p.z < z0 - margin && d.z <= T(0))) {
+        return inf;
+    }
     const T rr = p.x * p.x + p.y * p.y;
+    const T hb0 = p.x * d.x + p.y * d.y;
+    if (rr > (r1 + margin) * (r1 + margin) && hb0 >= T(0)) {
+        return inf;
+    }
     if (rr >= r0 * r0 && rr <= r1 * r1 && p.z >= z0 && p.z <= z1) {
         return T(0);
     }
     T best = inf;
     const T a = d.x * d.x + d.y * d.y;
     if (a > static_cast<T>(1e-12)) {
-        const T hb = p.x * d.x + p.y * d.y;
+        const T hb = hb0;
         const T radii[2] = {r0, r1};
         for (int k = 0; k < 2; ++k) {
             const T c = rr - radii[k] * radii[k];

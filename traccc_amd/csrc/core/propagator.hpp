@@ -395,10 +395,11 @@ struct navigator {
             wp = (nphi - 1) / 2;
         }
         const T inf = static_cast<T>(1e30);
+        // window slots -> module indices (no memory access yet)
+        uint32_t mk[kMaxCandidates];
+        bool okk[kMaxCandidates];
 TAMD_UNROLL
         for (int k = 0; k < kMaxCandidates; ++k) {
-            cs[k] = inf;
-            cm[k] = 0u;
             bool ok = true;
             uint32_t m = 0u;
             if (plane) {
@@ -416,28 +417,35 @@ TAMD_UNROLL
                 ok = ok && il >= 0 && il < nlong;
                 m = L.first_surface + static_cast<uint32_t>(ip * nlong + il);
             }
-            if (!ok || static_cast<int>(m) == exclude) {
-                continue;
-            }
-            const ckf_surface& S = sfs[m];
+            okk[k] = ok && static_cast<int>(m) != exclude;
+            mk[k] = okk[k] ? m : L.first_surface;  // a valid module for the loads below
+            cm[k] = 0u;
+        }
+        // straight-line distance to every slot's plane: the centre and normal
+        // loads of all slots are independent and go out together
+TAMD_UNROLL
+        for (int k = 0; k < kMaxCandidates; ++k) {
+            const ckf_surface& S = sfs[mk[k]];
             const vec3<T> w = sf_w<T>(S);
             const T tw = dot(st.dir, w);
-            if (std::fabs(tw) < static_cast<T>(1e-6)) {
-                continue;
+            const T s = dot(sf_center<T>(S) - st.pos, w) / tw;
+            cs[k] = (okk[k] && std::fabs(tw) >= static_cast<T>(1e-6) && s >= cfg.overstep_tol) ? s : inf;
+        }
+        // mask test (with the distance-scaled tolerance) for the slots left
+TAMD_UNROLL
+        for (int k = 0; k < kMaxCandidates; ++k) {
+            if (cs[k] < inf) {
+                const ckf_surface& S = sfs[mk[k]];
+                const T s = cs[k];
+                const vec3<T> x = st.pos + s * st.dir - sf_center<T>(S);
+                T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
+                tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
+                if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
+                    cm[k] = mk[k];
+                } else {
+                    cs[k] = inf;
+                }
             }
-            const vec3<T> c = sf_center<T>(S);
-            const T s = dot(c - st.pos, w) / tw;
-            if (s < cfg.overstep_tol) {
-                continue;
-            }
-            const vec3<T> x = st.pos + s * st.dir - c;
-            T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
-            tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
-            if (!inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
-                continue;
-            }
-            cs[k] = s;
-            cm[k] = m;
         }
     }
 
