@@ -267,3 +267,30 @@ def test_unsorted_measurements_rejected():
     alg = ckf.combinatorial_kalman_filter_algorithm(S.finding_config_for("C1"))
     with pytest.raises(ckf.CKFError):
         alg(det, fld, ev)
+
+
+@pytest.mark.parametrize("workload,n", [("C2", 3000), ("C4", 2000)])
+def test_propagation_schedules_identical(workload, n):
+    """Every propagation schedule (lanes per link 1, 4, 8, 16) gives the same
+    tracks, bit for bit; the autotuned default is one of them."""
+    det, fld, gun = S.workload(workload, n_particles=n, seed=17)
+    ev = S.generate_event(det, fld, gun)
+    cfg = S.finding_config_for(workload)
+    cfg.max_num_branches_per_surface = 2
+    lib = F.load_library(ckf.LIB)
+    lib.ckf_debug_set_schedule.argtypes = [C.c_void_p, C.c_uint32]
+    lib.ckf_debug_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    ref = None
+    for lanes in (0, 1, 4, 8, 16):
+        alg = ckf.combinatorial_kalman_filter_algorithm(cfg)
+        assert lib.ckf_debug_set_schedule(alg._h, lanes) == 0
+        tc = alg(det, fld, ev)
+        used = C.c_uint32()
+        lib.ckf_debug_get_schedule(alg._h, C.byref(used))
+        assert lanes == 0 or used.value == lanes
+        got = (tc.offsets, tc.constituents, tc.chi2, tc.params_vector)
+        if ref is None:
+            ref = got
+            continue
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), lanes

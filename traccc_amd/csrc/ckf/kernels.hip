@@ -366,6 +366,9 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl);
 // ---------------------------------------------------------------------------
 // propagate_to_next_surface
 // ---------------------------------------------------------------------------
+// kT lanes propagate one link together (team navigator); 64 / kT links per
+// wave partition.  The team leader writes the results.
+template <int kT>
 __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_advance) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -382,17 +385,20 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
     const uint32_t base = ctl->link_base;
     const uint32_t epoch = ctl->epoch;
     const uint32_t tip_base = ctl->n_tips;
-    const uint32_t n_parts = (n + kWave - 1) / kWave;
+    constexpr uint32_t kLinksPerWave = kWave / kT;
+    const uint32_t n_parts = (n + kLinksPerWave - 1) / kLinksPerWave;
     const uint32_t in_par = (step + 1u) & 1u;
     const uint32_t out_par = step & 1u;
     const bool dedup = step >= P.dup_min_len;
+    const uint32_t tl = lane() % kT;
+    const bool leader = (tl == 0u);
 
     for (;;) {
         const uint32_t p = wave_ticket(&ctl->ticket[2]);
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i = p * kWave + lane();
+        const uint32_t i = p * kLinksPerWave + lane() / kT;
         const bool valid = i < n;
         bool succ = false;
         bool propagated = false;
@@ -421,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                     uint32_t n_rk = 0u;
                     float leg = 0.f;
                     propagated = true;
-                    const int status = transport_state(P, s_layers, P.nav, x, c, sf, n_rk, leg);
+                    const int status = transport_state<kT>(P, s_layers, P.nav, x, c, sf, n_rk, leg, tl);
                     const bool success = (status == NAV_FOUND);
                     bool ok = success;
                     if (success) {
@@ -431,7 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                         if (!isfinite(x[e_phi])) ok = false;
                         if (fabsf(x[e_qop]) == 0.f) ok = false;
                         if (ok) {
-                            store_soa(P.state[out_par], P.state_cap, i, x, c, sf);
+                            if (leader) {
+                                store_soa(P.state[out_par], P.state_cap, i, x, c, sf);
+                            }
                             succ = true;
                         }
                     }
@@ -440,10 +448,11 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                 }
             }
         }
-        const uint64_t ms = wave_ballot(succ);
-        const uint64_t m1 = wave_ballot(ntip >= 1u);
-        const uint64_t m2 = wave_ballot(ntip >= 2u);
-        const uint64_t mp = wave_ballot(propagated);
+        // one vote per link: the team leaders'
+        const uint64_t ms = wave_ballot(leader && succ);
+        const uint64_t m1 = wave_ballot(leader && ntip >= 1u);
+        const uint64_t m2 = wave_ballot(leader && ntip >= 2u);
+        const uint64_t mp = wave_ballot(leader && propagated);
         const uint32_t s_tot = static_cast<uint32_t>(__popcll(ms));
         const uint32_t t_tot = static_cast<uint32_t>(__popcll(m1) + __popcll(m2));
         const uint64_t b = wave_lookback(P.lb_flag[2], P.lb_agg[2], P.lb_inc[2], p, epoch,
@@ -452,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
             atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props),
                       static_cast<unsigned long long>(__popcll(mp)));
         }
-        if (succ) {
+        if (leader && succ) {
             const uint32_t pos = lo32(b) + prefix_count(ms);
             if (pos < P.in_cap) {
                 P.live[pos] = i;
@@ -460,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                 set_overflow(ctl, 16u);
             }
         }
-        if (ntip > 0u) {
+        if (leader && ntip > 0u) {
             uint32_t pos = tip_base + hi32(b) + prefix_count(m1) + prefix_count(m2);
             for (uint32_t k = 0; k < ntip; ++k, ++pos) {
                 if (pos < P.tip_cap) {
@@ -827,8 +836,21 @@ hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t 
 
 hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                             bool fused_advance) {
-    hipLaunchKernelGGL(k_propagate, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P,
-                       fused_advance ? 1 : 0);
+    const int fa = fused_advance ? 1 : 0;
+    switch (s.prop_team) {
+        case 16:
+            hipLaunchKernelGGL(k_propagate<16>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_propagate<8>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_propagate<4>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            break;
+        default:
+            hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            break;
+    }
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@ struct launch_shape {
     uint32_t dedup_blocks;
     uint32_t prop_blocks;
     uint32_t build_blocks;
+    uint32_t prop_team;  // lanes per link in k_propagate: 1, 4, 8 or 16
 };
 
 // One CKF step: find_tracks, duplicate removal (active from

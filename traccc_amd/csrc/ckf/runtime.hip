@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -253,6 +254,9 @@ struct ckf_algorithm {
     std::vector<char> graph_key;
     // per-kernel event timing (diagnostics; the step loop then runs uncaptured)
     uint32_t last_steps = 0;  // steps the previous call ran (sizes the first launch)
+    uint32_t prop_team = 1;   // lanes per link in k_propagate (chosen per call)
+    uint32_t schedule_override = 0;   // fixed lanes per link, 0 = autotuned
+    std::map<uint32_t, uint32_t> tuned;  // log2(seeds) -> lanes per link
     bool timing = false;
     double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0;
     uint32_t n_prop_launches = 0;
@@ -290,8 +294,9 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.flt_cap = static_cast<uint32_t>(std::min<uint64_t>(uint64_t(A.in_cap) * N, 0x3fffffffu));
     A.state_cap = A.flt_cap;
     // look-back partitions: k_find takes 16 inputs per wave (4-lane teams),
-    // the other kernels 64 items per wave
-    A.lb_cap = std::max({(A.in_cap + 15u) / 16u, parts(A.state_cap), parts(A.tip_cap)}) + 1u;
+    // k_propagate 64 / team links per wave (teams of up to 16 lanes), the
+    // other kernels 64 items per wave
+    A.lb_cap = std::max({(A.in_cap + 15u) / 16u, (A.state_cap + 3u) / 4u, parts(A.tip_cap)}) + 1u;
     const size_t st = size_t(tamd::kStateCols) * 4u;
     A.state[0].ensure(st * A.state_cap);
     A.state[1].ensure(st * A.state_cap);
@@ -422,7 +427,9 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     tamd::launch_shape s;
     s.find_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
-    s.prop_blocks = blocks(A.state_cap);
+    s.prop_team = A.prop_team;
+    const uint32_t prop_parts = (A.state_cap + 64u / s.prop_team - 1u) / (64u / s.prop_team);
+    s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
     s.build_blocks = blocks(A.tip_cap);
     return s;
 }
@@ -629,6 +636,42 @@ void run_votes(ckf_algorithm& A, const tamd::ckf_params& P, const ckf_device_eve
     for (int i = 0; i < 10; ++i) std::swap(T.track_bytes[i], K2.track_bytes[i]);
 }
 
+// Propagation schedule autotuner: lanes per link in k_propagate.  Every
+// schedule gives the same tracks; which is fastest depends on how many links
+// a step holds against the machine's lanes (one lane per link fills the chip
+// once tens of thousands of links are live; a small event runs its links'
+// navigation on teams of lanes instead).  The first call of each size class
+// (log2 of the seed count) times the step loop once per schedule on its own
+// input and keeps the fastest; later calls of the class reuse it.  A fixed
+// schedule (ckf_debug_set_schedule) bypasses the tuner.
+template <typename Loop>
+uint32_t choose_schedule(ckf_algorithm& A, uint32_t n_seeds, Loop& step_loop) {
+    if (A.schedule_override != 0u) {
+        return A.schedule_override;
+    }
+    uint32_t cls = 0u;
+    while ((1u << cls) < n_seeds && cls < 31u) ++cls;
+    const auto it = A.tuned.find(cls);
+    if (it != A.tuned.end()) {
+        return it->second;
+    }
+    const uint32_t teams[4] = {1u, 4u, 8u, 16u};
+    uint32_t best = 1u;
+    double best_s = 1e300;
+    for (uint32_t t : teams) {
+        A.prop_team = t;
+        const auto t0 = std::chrono::steady_clock::now();
+        step_loop();  // ends with a stream synchronisation
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt < best_s) {
+            best_s = dt;
+            best = t;
+        }
+    }
+    A.tuned[cls] = best;
+    return best;
+}
+
 // The operator's dispatch (core/src/finding/combinatorial_kalman_filter_algorithm.cpp:47-57):
 // the progressive filter runs when asked for and at most one branch is kept.
 bool use_pkf(const ckf_finding_config& c) {
@@ -779,20 +822,24 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
 
     tamd::ckf_params P{};
     tamd::launch_shape shape{};
-    for (int attempt = 0;; ++attempt) {
-        ensure_buffers(A, ev);
-        P = make_params(A, D, ev);
-        shape = make_shape(A);
-        const uint32_t ovf = run_steps(A, P, shape, ev);
-        if (ovf == 0u) {
-            break;
+    auto step_loop = [&]() {
+        for (int attempt = 0;; ++attempt) {
+            ensure_buffers(A, ev);
+            P = make_params(A, D, ev);
+            shape = make_shape(A);
+            const uint32_t ovf = run_steps(A, P, shape, ev);
+            if (ovf == 0u) {
+                break;
+            }
+            if (attempt > 40) {
+                throw ckf_error(CKF_ERR_OUT_OF_MEMORY, "CKF buffers kept overflowing");
+            }
+            grow(A, ovf);
+            A.stats.n_regrows += 1u;
         }
-        if (attempt > 40) {
-            throw ckf_error(CKF_ERR_OUT_OF_MEMORY, "CKF buffers kept overflowing");
-        }
-        grow(A, ovf);
-        A.stats.n_regrows += 1u;
-    }
+    };
+    A.prop_team = choose_schedule(A, ns, step_loop);
+    step_loop();
     const tamd::step_ctl done = *A.h_ctl;
     A.stats.n_steps = done.n_steps_run;
     A.stats.n_links = done.n_links_total;
@@ -1307,6 +1354,26 @@ ckf_status ckf_debug_set_kernel_timing(ckf_algorithm* alg, int enable) {
     return guarded([&] {
         if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
         alg->timing = enable != 0;
+    });
+}
+
+ckf_status ckf_debug_set_schedule(ckf_algorithm* alg, uint32_t lanes_per_link) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        if (lanes_per_link != 0u && lanes_per_link != 1u && lanes_per_link != 4u &&
+            lanes_per_link != 8u && lanes_per_link != 16u) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "lanes per link must be 0, 1, 4, 8 or 16");
+        }
+        alg->schedule_override = lanes_per_link;
+    });
+}
+
+ckf_status ckf_debug_get_schedule(const ckf_algorithm* alg, uint32_t* lanes_per_link) {
+    return guarded([&] {
+        if (alg == nullptr || lanes_per_link == nullptr) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        *lanes_per_link = alg->prop_team;
     });
 }
 

@@ -89,11 +89,16 @@ __device__ inline void add_material(const ckf_params& P, uint32_t sf, const floa
 // (propagate_to_next_surface's stepper, navigator and parameter transport,
 // without the material of the destination).  On NAV_FOUND x, c and sf hold
 // the state on the new surface; otherwise they are unchanged.  n_steps and
-// path receive the RK steps and the path length of the leg.
+// path receive the RK steps and the path length of the leg.  With kT > 1 the
+// kT lanes of a team call this together with the same state (tl: the lane's
+// index in the team) and all get the same result.
+template <int kT = 1>
 __device__ inline int transport_state(const ckf_params& P, const ckf_layer* layers,
                                       const nav_config<float>& cfg, float* x, float* c,
-                                      uint32_t& sf, uint32_t& n_steps, float& path) {
-    navigator<float, true> nav;
+                                      uint32_t& sf, uint32_t& n_steps, float& path,
+                                      uint32_t tl = 0u) {
+    navigator<float, true, kT> nav;
+    nav.tl = tl;
     nav.sfs = P.sfs;
     nav.layers = layers;
     nav.n_layers = P.n_layers;

@@ -144,6 +144,35 @@ TAMD_HD T layer_entry_distance(const ckf_layer& L, const vec3<T>& p,
     return best;
 }
 
+// A lower bound on layer_entry_distance(L, p, d, margin) for an (r, z)
+// envelope, from the slab the line must reach first: an accepted crossing
+// lies within the widened z range and at a radius of at least r0 - margin,
+// and the straight line covers at most its path length in z and in radius.
+// inv_dz is 1 / d.z (inf for d.z == 0).  0 when nothing is known.
+template <typename T>
+TAMD_HD T layer_entry_lower_bound(const ckf_layer& L, const vec3<T>& p, const vec3<T>& d,
+                                  T margin, T inv_dz) {
+    if (L.kind == CKF_LAYER_PLANE) {
+        return T(0);
+    }
+    T lb = T(0);
+    const T zlo = static_cast<T>(L.env_z[0]) - margin;
+    const T zhi = static_cast<T>(L.env_z[1]) + margin;
+    if (p.z < zlo && d.z > T(0)) {
+        lb = (zlo - p.z) * inv_dz;
+    } else if (p.z > zhi && d.z < T(0)) {
+        lb = (zhi - p.z) * inv_dz;
+    }
+    const T rlo = static_cast<T>(L.env_r[0]) - margin;
+    const T rr = p.x * p.x + p.y * p.y;
+    if (rlo > T(0) && rr < rlo * rlo) {
+        // r(s) <= r(0) + s and r(0) < rlo: s >= rlo - r(0) >= (rlo^2 - r(0)^2) / (2 rlo)
+        const T lr = (rlo * rlo - rr) / (T(2) * rlo);
+        lb = lr > lb ? lr : lb;
+    }
+    return lb;
+}
+
 // Module grid bin of a global point: phi bin (wrapped) and long bin (clamped).
 template <typename T>
 TAMD_HD void layer_bin(const ckf_layer& L, const vec3<T>& q, int& iphi,

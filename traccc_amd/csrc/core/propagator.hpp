@@ -340,18 +340,51 @@ constexpr int kMaxCandidates = 9;
 //  5. Mark the layer visited, go to 2.
 // More than max_steps RK steps, a path beyond path_limit, or a start momentum
 // below min_p / min_pT -> NAV_ABORT.
-template <typename T, bool kJac>
+//
+// Team form (kT > 1, device code): kT consecutive lanes of a wave navigate one
+// track together.  They hold the same state and take the same decisions; the
+// layer search (step 2) and the window candidates (step 4) are split over the
+// lanes and joined with shuffle reductions that keep the serial order's
+// tie-breaks (lowest layer index, lowest module index), so every lane of the
+// team reaches the serial navigator's result.  tl is the lane's index in its
+// team.
+template <typename T, bool kJac, int kT = 1>
 struct navigator {
+    static_assert(kT >= 1 && kT <= 64 && (kT & (kT - 1)) == 0, "team size: a power of two");
+    static constexpr int kSlotsPerLane = (kMaxCandidates + kT - 1) / kT;
+
     const ckf_surface* sfs;
     const ckf_layer* layers;
     uint32_t n_layers;
     field_view<T> field;
     nav_config<T> cfg;
+    uint32_t tl = 0u;
 
     free_state<T> st;
     transport_jac<T> J;
     T path;
     uint32_t n_steps;
+
+    // Team-wide minimum of (v, key), the lowest key winning ties; every lane
+    // of the team ends with the winner.
+    TAMD_HD void team_min(T& v, uint32_t& key) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (kT > 1) {
+TAMD_UNROLL
+            for (int d = kT / 2; d >= 1; d >>= 1) {
+                const T ov = __shfl_xor(v, d, kT);
+                const uint32_t ok = __shfl_xor(key, d, kT);
+                if (ov < v || (ov == v && ok < key)) {
+                    v = ov;
+                    key = ok;
+                }
+            }
+        }
+#else
+        (void)v;
+        (void)key;
+#endif
+    }
 
     TAMD_HD T step_limit() const {
         const vec3<T> B = field_at(field, st.pos);
@@ -372,12 +405,15 @@ struct navigator {
         return n_steps <= cfg.max_steps && std::fabs(path) <= cfg.path_limit;
     }
 
-    // Candidate modules of layer `li` in fixed slots (window position, or
-    // module of a plane layer); unused slots hold +inf.  No register array is
-    // indexed with a runtime value.
-    TAMD_HD void collect(uint32_t li, int exclude, T* cs, uint32_t* cm) const {
+    // Candidate modules of layer `li` in window slots (window position, or
+    // module of a plane layer): this lane's slots k = j * kT + tl go to
+    // cs[j] / cm[j]; unused slots hold +inf.  No register array is indexed
+    // with a runtime value.  s_known: the layer's entry distance from the
+    // current state when the caller has it, negative otherwise.
+    TAMD_HD void collect(uint32_t li, int exclude, T s_known, T* cs, uint32_t* cm) const {
         const ckf_layer& L = layers[li];
-        T s_entry = layer_entry_distance(L, st.pos, st.dir, cfg.mask_tol_max);
+        T s_entry = s_known >= T(0) ? s_known
+                                     : layer_entry_distance(L, st.pos, st.dir, cfg.mask_tol_max);
         if (!(s_entry < static_cast<T>(1e29))) {
             s_entry = T(0);
         }
@@ -396,19 +432,20 @@ struct navigator {
         }
         const T inf = static_cast<T>(1e30);
         // window slots -> module indices (no memory access yet)
-        uint32_t mk[kMaxCandidates];
-        bool okk[kMaxCandidates];
+        uint32_t mk[kSlotsPerLane];
+        bool okk[kSlotsPerLane];
 TAMD_UNROLL
-        for (int k = 0; k < kMaxCandidates; ++k) {
-            bool ok = true;
+        for (int j = 0; j < kSlotsPerLane; ++j) {
+            const int k = j * kT + static_cast<int>(tl);
+            bool ok = k < kMaxCandidates;
             uint32_t m = 0u;
             if (plane) {
-                ok = k < nphi * nlong;
+                ok = ok && k < nphi * nlong;
                 m = L.first_surface + static_cast<uint32_t>(k);
             } else {
                 const int dp = k / 3 - 1;
                 const int dl = k % 3 - 1;
-                ok = (dp >= -wp && dp <= wp && dl >= -wl && dl <= wl);
+                ok = ok && (dp >= -wp && dp <= wp && dl >= -wl && dl <= wl);
                 int ip = (ic + dp) % nphi;
                 if (ip < 0) {
                     ip += nphi;
@@ -417,33 +454,33 @@ TAMD_UNROLL
                 ok = ok && il >= 0 && il < nlong;
                 m = L.first_surface + static_cast<uint32_t>(ip * nlong + il);
             }
-            okk[k] = ok && static_cast<int>(m) != exclude;
-            mk[k] = okk[k] ? m : L.first_surface;  // a valid module for the loads below
-            cm[k] = 0u;
+            okk[j] = ok && static_cast<int>(m) != exclude;
+            mk[j] = okk[j] ? m : L.first_surface;  // a valid module for the loads below
+            cm[j] = 0u;
         }
         // straight-line distance to every slot's plane: the centre and normal
         // loads of all slots are independent and go out together
 TAMD_UNROLL
-        for (int k = 0; k < kMaxCandidates; ++k) {
-            const ckf_surface& S = sfs[mk[k]];
+        for (int j = 0; j < kSlotsPerLane; ++j) {
+            const ckf_surface& S = sfs[mk[j]];
             const vec3<T> w = sf_w<T>(S);
             const T tw = dot(st.dir, w);
             const T s = dot(sf_center<T>(S) - st.pos, w) / tw;
-            cs[k] = (okk[k] && std::fabs(tw) >= static_cast<T>(1e-6) && s >= cfg.overstep_tol) ? s : inf;
+            cs[j] = (okk[j] && std::fabs(tw) >= static_cast<T>(1e-6) && s >= cfg.overstep_tol) ? s : inf;
         }
         // mask test (with the distance-scaled tolerance) for the slots left
 TAMD_UNROLL
-        for (int k = 0; k < kMaxCandidates; ++k) {
-            if (cs[k] < inf) {
-                const ckf_surface& S = sfs[mk[k]];
-                const T s = cs[k];
+        for (int j = 0; j < kSlotsPerLane; ++j) {
+            if (cs[j] < inf) {
+                const ckf_surface& S = sfs[mk[j]];
+                const T s = cs[j];
                 const vec3<T> x = st.pos + s * st.dir - sf_center<T>(S);
                 T tol = cfg.mask_tol_min + cfg.mask_tol_scalor * std::fabs(s);
                 tol = tol < cfg.mask_tol_max ? tol : cfg.mask_tol_max;
                 if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), tol)) {
-                    cm[k] = mk[k];
+                    cm[j] = mk[j];
                 } else {
-                    cs[k] = inf;
+                    cs[j] = inf;
                 }
             }
         }
@@ -480,29 +517,34 @@ TAMD_UNROLL
 
     // Visit the candidates of layer `li` in increasing (distance, module).
     // 0: found (out_sf), 1: nothing in this layer, 2: abort.
-    TAMD_HD int process_layer(uint32_t li, int exclude, uint32_t& out_sf) {
-        T cs[kMaxCandidates];
-        uint32_t cm[kMaxCandidates];
-        collect(li, exclude, cs, cm);
+    TAMD_HD int process_layer(uint32_t li, int exclude, T s_known, uint32_t& out_sf) {
+        T cs[kSlotsPerLane];
+        uint32_t cm[kSlotsPerLane];
+        collect(li, exclude, s_known, cs, cm);
         const T inf = static_cast<T>(1e30);
         uint32_t used = 0u;
         for (int tries = 0; tries < kMaxCandidates; ++tries) {
-            int pick = -1;
+            // nearest unvisited candidate, lower module index on ties
             T best = inf;
-            uint32_t bm = 0u;
+            uint32_t bm = 0xffffffffu;
 TAMD_UNROLL
-            for (int k = 0; k < kMaxCandidates; ++k) {
-                const bool avail = ((used >> k) & 1u) == 0u && cs[k] < inf;
-                if (avail && (pick < 0 || cs[k] < best || (cs[k] == best && cm[k] < bm))) {
-                    pick = k;
-                    best = cs[k];
-                    bm = cm[k];
+            for (int j = 0; j < kSlotsPerLane; ++j) {
+                const bool avail = ((used >> j) & 1u) == 0u && cs[j] < inf;
+                if (avail && (cs[j] < best || (cs[j] == best && cm[j] < bm))) {
+                    best = cs[j];
+                    bm = cm[j];
                 }
             }
-            if (pick < 0) {
+            team_min(best, bm);
+            if (!(best < inf)) {
                 return 1;
             }
-            used |= (1u << pick);
+TAMD_UNROLL
+            for (int j = 0; j < kSlotsPerLane; ++j) {
+                if (cs[j] < inf && cm[j] == bm) {
+                    used |= (1u << j);
+                }
+            }
             const ckf_surface& S = sfs[bm];
             {
                 const vec3<T> w = sf_w<T>(S);
@@ -544,7 +586,7 @@ TAMD_UNROLL
         uint64_t visited = 0ull;
         if (start_sf >= 0) {
             const uint32_t l0 = sfs[start_sf].layer;
-            const int r = process_layer(l0, start_sf, out_sf);
+            const int r = process_layer(l0, start_sf, T(-1), out_sf);
             if (r == 0) {
                 return NAV_FOUND;
             }
@@ -556,8 +598,15 @@ TAMD_UNROLL
         for (uint32_t guard = 0; guard <= n_layers; ++guard) {
             int best = -1;
             T bs = static_cast<T>(1e30);
-            for (uint32_t li = 0; li < n_layers; ++li) {
+            const T inv_dz = T(1) / st.dir.z;
+            for (uint32_t li = tl; li < n_layers; li += kT) {
                 if (visited & (1ull << li)) {
+                    continue;
+                }
+                // a layer whose entry cannot come before the best one so far
+                // is not evaluated (10 um of slack covers the bound's rounding)
+                if (layer_entry_lower_bound(layers[li], st.pos, st.dir, cfg.mask_tol_max, inv_dz) >
+                    bs + static_cast<T>(1e-2)) {
                     continue;
                 }
                 const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
@@ -565,6 +614,12 @@ TAMD_UNROLL
                     bs = s;
                     best = static_cast<int>(li);
                 }
+            }
+            {
+                // nearest layer of the team, lowest index on ties
+                uint32_t key = best < 0 ? 0xffffffffu : static_cast<uint32_t>(best);
+                team_min(bs, key);
+                best = key == 0xffffffffu ? -1 : static_cast<int>(key);
             }
             if (best < 0 || !(bs < static_cast<T>(1e29))) {
                 return NAV_EXIT;
@@ -577,7 +632,9 @@ TAMD_UNROLL
                 bs = layer_entry_distance(layers[best], st.pos, st.dir, cfg.mask_tol_max);
             }
             if (bs < static_cast<T>(1e29)) {
-                const int r = process_layer(static_cast<uint32_t>(best), -1, out_sf);
+                // bs is the entry distance from the current state (recomputed
+                // after every approach step)
+                const int r = process_layer(static_cast<uint32_t>(best), -1, bs, out_sf);
                 if (r == 0) {
                     return NAV_FOUND;
                 }
