@@ -50,6 +50,35 @@ def kernel_times(alg, ddet, devents):
             "advance": vals[3].value, "steps": n.value}
 
 
+def batched_line(a, ckf, S, torch):
+    """C3: one operator call over a batch of events (log-uniform 500-50,000
+    muons each, (r, z) solenoid map), inputs resident in HBM."""
+    det, fld, guns = S.batch_workload("C3", n_events=a.batched_events, seed=42)
+    t0 = time.perf_counter()
+    events = S.generate_events(det, fld, guns, threads=min(16, len(os.sched_getaffinity(0))))
+    n_seeds = sum(e.n_seeds for e in events)
+    log(f"[bench] C3 batch: {len(events)} events, {n_seeds} seeds, "
+        f"{sum(e.n_meas for e in events)} measurements (generated in {time.perf_counter() - t0:.1f} s)")
+    cfg = S.finding_config_for("C3")
+    alg = ckf.combinatorial_kalman_filter_algorithm(cfg)
+    ddet = ckf.DeviceDetector(det, fld)
+    devents = ckf.DeviceEvents(ddet, events)
+    for _ in range(2):
+        alg.run_device(ddet, devents)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.batched_steps):
+        out = alg.run_device(ddet, devents)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.batched_steps
+    n_tracks, _ = out.sizes()
+    log(f"[bench] C3 batch: {dt * 1e3:.2f} ms per call -> {n_seeds / dt:.4g} tracks/s")
+    return {"workload": f"C3: {len(events)} events x log-uniform 500-50,000 muons, (r,z) solenoid "
+                        f"map 10 mm, C2 detector, one operator call",
+            "seeds_per_call": n_seeds, "calls": a.batched_steps, "ms_per_call": round(dt * 1e3, 3),
+            "value": round(n_seeds / dt, 1), "unit": "tracks/s", "found_tracks": int(n_tracks)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +87,9 @@ def main() -> int:
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-events-per-core", type=int, default=6)
+    ap.add_argument("--batched-events", type=int, default=256,
+                    help="events of the batched C3 line (0 skips it)")
+    ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -145,6 +177,10 @@ def main() -> int:
                 "launches_per_call": launches}
     log(f"[bench] kernel ms per call: {groups}, steps {launches}")
 
+    batched = None
+    if a.batched_events > 0:
+        batched = batched_line(a, ckf, S, torch)
+
     cpu = None
     if not a.no_cpu_baseline:
         import oracle
@@ -177,6 +213,7 @@ def main() -> int:
                    "max_num_branches_per_surface": cfg.max_num_branches_per_surface},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "efficiency": round(perf.efficiency, 5),
         "found_tracks_per_s": round(len(result) * a.steps / elapsed, 1),
         "gpu_over_cpu": round(value / cpu["value"], 2) if cpu else None,

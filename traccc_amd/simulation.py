@@ -8,6 +8,7 @@ with the stated pT / eta spectra, smearing, noise and dead modules.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import math
 from dataclasses import dataclass, field
 
@@ -335,6 +336,30 @@ def workload(name: str, seed: int = 42, n_particles: int | None = None):
     else:
         raise ValueError(name)
     return det, fld, gun
+
+
+def batch_workload(name: str = "C3", n_events: int = 256, seed: int = 42,
+                   n_min: int = 500, n_max: int = 50000):
+    """(detector, field, [gun spec per event]) of a batched workload: per-event
+    particle counts log-uniform in [n_min, n_max] (C3: 256 events, 500-50,000
+    muons, solenoid map; C5: 64 events of 60,000 pions), event e seeded with
+    seed + e."""
+    det, fld, base = workload(name, seed=seed)
+    rng = np.random.default_rng(seed)
+    if name == "C5":
+        counts = [60000] * n_events
+    else:
+        counts = np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_events)).astype(int).tolist()
+    guns = [dataclasses.replace(base, n_particles=int(n), seed=seed + e) for e, n in enumerate(counts)]
+    return det, fld, guns
+
+
+def generate_events(det: DetectorData, fld: FieldData, specs, threads: int = 8):
+    """generate_event over many specs on a thread pool (the generator runs in
+    C and releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda sp: generate_event(det, fld, sp), specs))
 
 
 def finding_config_for(name: str) -> F.FindingConfig:
