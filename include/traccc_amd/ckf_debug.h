@@ -49,6 +49,11 @@ ckf_status ckf_debug_set_kernel_timing(ckf_algorithm* alg, int enable);
  * default).  *lanes receives the schedule of the last call. */
 ckf_status ckf_debug_set_schedule(ckf_algorithm* alg, uint32_t lanes_per_link);
 ckf_status ckf_debug_get_schedule(const ckf_algorithm* alg, uint32_t* lanes_per_link);
+
+/* One 16x16x16 BF16 matrix-core product D = A B (row-major FP32 host arrays,
+ * inputs rounded to BF16) through the fragment layout the scorer kernels
+ * use; checks that layout on the device. */
+ckf_status ckf_debug_mfma_layout(const float* A, const float* B, float* D);
 ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, double* ms_dedup,
                                   double* ms_propagate, double* ms_other, uint32_t* n_steps);
 

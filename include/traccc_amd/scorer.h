@@ -1,0 +1,93 @@
+/*
+ * traccc_amd -- learned branch scorer for the CKF (C ABI).
+ *
+ * No counterpart in traccc: a small MLP (12 -> 64 -> 64 -> 1, GELU, BF16
+ * weights, FP32 accumulation) that scores a CKF branch candidate -- one
+ * (branch, measurement) pair that passed the chi2 cut -- by its chance to end
+ * in a found track.  It is trained on survive / die labels of branches logged
+ * by the plain CKF itself (ckf_find_tracks_logged) with the project's own
+ * BF16-MFMA forward / backward kernels, and prunes candidates before the
+ * Kalman update (ckf_algorithm_set_scorer).  The plain CKF stays available
+ * and is the judge of finding efficiency.
+ *
+ * Features (in this order, FP32, normalised in the kernel with the scorer's
+ * mean / std):
+ *   0 pull in loc0   1 pull in loc1 (0 for 1-D)   2 predicted chi2
+ *   3 layer index / layer count   4 q/p (1/GeV)   5 theta
+ *   6 holes so far   7 branch depth (CKF step)   8 chi2 sum / ndf sum so far
+ *   9 consecutive holes   10 rank of the candidate among the branch's
+ *   candidates by chi2   11 log(1 + candidates passing the chi2 cut)
+ *
+ * Parameters (flat FP32, ckf_scorer_n_params() = 5057):
+ *   W1[12][64] b1[64] W2[64][64] b2[64] W3[64] b3[1]
+ *   (W[i][j]: input i -> output j).
+ */
+#ifndef TRACCC_AMD_SCORER_H
+#define TRACCC_AMD_SCORER_H
+
+#include <stdint.h>
+
+#include "traccc_amd/ckf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CKF_SCORER_FEATURES 12u
+#define CKF_SCORER_HIDDEN 64u
+
+typedef struct ckf_scorer ckf_scorer;
+
+typedef struct ckf_scorer_train_config {
+    uint32_t epochs;        /* passes over the samples                       */
+    uint32_t batch;         /* samples per optimiser step (multiple of 64)   */
+    float learning_rate;    /* Adam                                          */
+    float beta1, beta2, eps;
+    float weight_decay;     /* decoupled (AdamW)                             */
+    float pos_weight;       /* weight of survive labels in the loss          */
+    uint64_t shuffle_seed;  /* sample order per epoch                        */
+} ckf_scorer_train_config;
+
+typedef struct ckf_scorer_train_stats {
+    uint32_t steps;         /* optimiser steps taken                         */
+    float first_loss;       /* mean loss of the first epoch                  */
+    float last_loss;        /* mean loss of the last epoch                   */
+    double seconds;         /* device time of the training kernels           */
+    double tflops;          /* achieved (algorithmic) TFLOP/s of fwd + bwd    */
+} ckf_scorer_train_stats;
+
+uint32_t ckf_scorer_n_params(void);
+void ckf_scorer_train_config_default(ckf_scorer_train_config* cfg);
+
+/* Random initialisation (scaled uniform, seeded), identity normalisation. */
+ckf_status ckf_scorer_create(uint64_t seed, ckf_scorer** out);
+ckf_status ckf_scorer_destroy(ckf_scorer* s);
+
+/* Host copies of the FP32 master parameters and the feature normalisation
+ * (mean[12], std[12]); either pointer may be NULL. */
+ckf_status ckf_scorer_set_params(ckf_scorer* s, const float* params, const float* mean,
+                                 const float* std_dev);
+ckf_status ckf_scorer_get_params(const ckf_scorer* s, float* params, float* mean, float* std_dev);
+
+/* Device arrays: features [n][12] FP32, labels [n] u8 (1 = survives), scores
+ * [n] FP32 (logits).  Calls are ordered on the stream (hipStream_t as void*,
+ * NULL = default) and return after it has finished. */
+ckf_status ckf_scorer_fit_normalisation(ckf_scorer* s, const float* d_features, uint32_t n,
+                                        void* stream);
+ckf_status ckf_scorer_forward(const ckf_scorer* s, const float* d_features, uint32_t n,
+                              float* d_scores, void* stream);
+/* Gradient of the mean (weighted) binary cross-entropy over the n samples with
+ * respect to the parameters, into the host array grad[n_params]; *loss gets
+ * the mean loss.  For gradient checks; no parameter changes. */
+ckf_status ckf_scorer_gradient(const ckf_scorer* s, const float* d_features,
+                               const uint8_t* d_labels, uint32_t n, float pos_weight,
+                               float* grad, float* loss, void* stream);
+ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_t* d_labels,
+                            uint32_t n, const ckf_scorer_train_config* cfg,
+                            ckf_scorer_train_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRACCC_AMD_SCORER_H */

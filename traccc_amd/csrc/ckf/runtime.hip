@@ -20,43 +20,23 @@
 #include <string>
 #include <vector>
 
+#include "../common/errors.hpp"
 #include "kernels.hpp"
 #include "traccc_amd/ckf.h"
 #include "traccc_amd/ckf_debug.h"
 
+namespace tamd {
+std::string& last_error() {
+    thread_local std::string s;
+    return s;
+}
+}  // namespace tamd
+
 namespace {
 
-thread_local std::string g_last_error;
-
-struct ckf_error : std::runtime_error {
-    ckf_status code;
-    ckf_error(ckf_status c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) {
-        throw ckf_error(e == hipErrorOutOfMemory ? CKF_ERR_OUT_OF_MEMORY : CKF_ERR_DEVICE,
-                        std::string(what) + ": " + hipGetErrorString(e));
-    }
-}
-
-template <typename F>
-ckf_status guarded(F&& f) {
-    try {
-        f();
-        g_last_error.clear();
-        return CKF_OK;
-    } catch (const ckf_error& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return CKF_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return CKF_ERR_INTERNAL;
-    }
-}
+using tamd::ckf_error;
+using tamd::guarded;
+using tamd::hip_check;
 
 // Device allocation that grows by doubling and is reused across calls.
 struct dbuf {
@@ -1066,7 +1046,7 @@ extern "C" {
 
 uint32_t ckf_abi_version(void) { return CKF_ABI_VERSION; }
 
-const char* ckf_last_error(void) { return g_last_error.c_str(); }
+const char* ckf_last_error(void) { return tamd::last_error().c_str(); }
 
 ckf_status ckf_algorithm_create(const ckf_finding_config* cfg, ckf_algorithm** out) {
     return guarded([&] {

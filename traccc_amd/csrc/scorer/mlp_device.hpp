@@ -1,0 +1,189 @@
+// Branch-scorer MLP (12 -> 64 -> 64 -> 1, GELU) on CDNA matrix cores.
+//
+// One wave evaluates a tile of 64 rows (samples).  Layers 1 and 2 run on
+// v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation) over 4 x 4
+// output tiles of 16 x 16; layer 3 (64 -> 1) is a dot product reduced across
+// the 16 lanes that share a row.  Fragment layout of the 16x16x16 MFMA, lane
+// l of 64 (checked on the device by ckf_debug_mfma_layout):
+//   A (M x K):  lane l holds A[l % 16][4 (l / 16) + e],  e = 0..3
+//   B (K x N):  lane l holds B[4 (l / 16) + e][l % 16]
+//   D (M x N):  lane l holds D[4 (l / 16) + e][l % 16]
+// so a D tile read as B (same rows = K, columns = N) needs no data movement;
+// D -> A needs a transpose, which goes through the wave's LDS area.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tamd {
+
+constexpr int kMlpIn = 12;
+constexpr int kMlpInPad = 16;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpHidden + 1;  // 5057
+// parameter offsets in the flat FP32 vector W1[12][64] b1 W2[64][64] b2 W3 b3
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpIn * kMlpHidden;
+constexpr int kOffW2 = kOffB1 + kMlpHidden;
+constexpr int kOffB2 = kOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kOffW3 = kOffB2 + kMlpHidden;
+constexpr int kOffB3 = kOffW3 + kMlpHidden;
+// LDS per wave: X tile [64][16] bf16 + H tile [64][kHStride] bf16
+constexpr int kHStride = 72;  // 144 B rows spread the 16 rows of a fragment over the banks
+constexpr int kMlpLdsPerWave = 64 * kMlpInPad + 64 * kHStride;  // shorts
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Device copy of the weights in the layouts the fragments load directly.
+struct mlp_dev {
+    const short* w1t;      // [64 out][16 in] bf16 (inputs 12..15 zero)
+    const short* w2;       // [64 in][64 out] bf16 (B of dH1 = dZ2 W2^T)
+    const short* w2t;      // [64 out][64 in] bf16 (B of Z2 = H1 W2)
+    const float* b1;       // [64]
+    const float* b2;       // [64]
+    const float* w3;       // [64] BF16-rounded, kept in FP32
+    const float* b3;       // [1]
+    const float* mean;     // [12] feature normalisation
+    const float* inv_std;  // [12]
+};
+
+__device__ inline short f2bf(float f) {
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<short>(u >> 16);
+}
+__device__ inline float bf2f(short h) {
+    return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(h)) << 16);
+}
+
+__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+    return 0.5f * (1.f + erff(z * 0.70710678118654752f)) +
+           z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// Wave-internal LDS hand-over: every lane's earlier LDS writes are visible to
+// every lane's later LDS reads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline bf16x4 load4(const short* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+// Layer-1 and layer-2 pre-activations of a 64-row tile.  On entry each lane
+// holds its row's 12 raw features (lane = row; valid = false gives a zero
+// row).  z1 / z2 are in D layout: z[mt][nt][e] is row 16 mt + 4 (l / 16) + e,
+// column 16 nt + l % 16.  Leaves the normalised X tile in lds_x and H1 =
+// GELU(z1) in lds_h.
+__device__ inline void mlp_tile_layers(const mlp_dev& W, const float* feat, bool valid,
+                                       short* lds_x, short* lds_h, float (&z1)[4][4][4],
+                                       float (&z2)[4][4][4]) {
+    const uint32_t l = __lane_id();
+    const uint32_t r16 = l % 16u;
+    const uint32_t g4 = 4u * (l / 16u);
+#pragma unroll
+    for (int f = 0; f < kMlpInPad; ++f) {
+        float v = 0.f;
+        if (f < kMlpIn && valid) {
+            v = (feat[f] - W.mean[f]) * W.inv_std[f];
+        }
+        lds_x[l * kMlpInPad + f] = f2bf(v);
+    }
+    wave_lds_sync();
+    // layer 1: Z1 = X W1 + b1 (K = 16, one MFMA per tile)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const bf16x4 a = load4(lds_x + (16 * mt + r16) * kMlpInPad + g4);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const bf16x4 b = load4(W.w1t + (16 * nt + r16) * kMlpInPad + g4);
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16(a, b, acc);
+            const float bias = W.b1[16 * nt + r16];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                z1[mt][nt][e] = acc[e] + bias;
+                lds_h[(16 * mt + g4 + e) * kHStride + 16 * nt + r16] = f2bf(gelu(z1[mt][nt][e]));
+            }
+        }
+    }
+    wave_lds_sync();
+    // layer 2: Z2 = H1 W2 + b2 (K = 64, four MFMAs per tile)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        bf16x4 b[4];
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            b[kt] = load4(W.w2t + (16 * nt + r16) * kMlpHidden + 16 * kt + g4);
+        }
+        const float bias = W.b2[16 * nt + r16];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                const bf16x4 a = load4(lds_h + (16 * mt + r16) * kHStride + 16 * kt + g4);
+                acc = mfma16(a, b[kt], acc);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) z2[mt][nt][e] = acc[e] + bias;
+        }
+    }
+}
+
+// Layer 3 from z2: logit of row 16 mt + 4 (l / 16) + e in s[mt][e] (every
+// lane of the 16-lane group holds its group's rows).
+__device__ inline void mlp_tile_output(const mlp_dev& W, const float (&z2)[4][4][4],
+                                       float (&s)[4][4]) {
+    const uint32_t r16 = __lane_id() % 16u;
+    float w3[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) w3[nt] = W.w3[16 * nt + r16];
+    const float b3 = W.b3[0];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float acc = 0.f;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc += gelu(z2[mt][nt][e]) * w3[nt];
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) acc += __shfl_xor(acc, d, 64);
+            s[mt][e] = acc + b3;
+        }
+    }
+}
+
+// Scores of a 64-row tile: the logit of lane l's row returned in lane l.
+__device__ inline float mlp_tile_logit(const mlp_dev& W, const float* feat, bool valid,
+                                       short* lds_x, short* lds_h) {
+    float z1[4][4][4], z2[4][4][4], s[4][4];
+    mlp_tile_layers(W, feat, valid, lds_x, lds_h, z1, z2);
+    mlp_tile_output(W, z2, s);
+    // hand row r's logit (held by lanes of group (r % 16) / 4, tile r / 16)
+    // to lane r through the LDS area
+    const uint32_t l = __lane_id();
+    float* lds_s = reinterpret_cast<float*>(lds_x);  // 64 floats fit in the X tile
+    wave_lds_sync();
+    if ((l % 16u) == 0u) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) lds_s[16 * mt + 4 * (l / 16u) + e] = s[mt][e];
+        }
+    }
+    wave_lds_sync();
+    return lds_s[l];
+}
+
+}  // namespace tamd
