@@ -86,6 +86,27 @@ ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_
                             uint32_t n, const ckf_scorer_train_config* cfg,
                             ckf_scorer_train_stats* stats, void* stream);
 
+/* ---- in the CKF ----------------------------------------------------------- */
+/* Prune, before the Kalman update, every candidate whose logit is below
+ * `threshold`: each input branch lists its max_num_branches_per_surface + 4
+ * best candidates by (chi2, measurement index), the scorer scores the list,
+ * and the update walks the surviving entries in order as the plain CKF walks
+ * all candidates.  scorer == NULL restores the plain CKF.  The scorer must
+ * outlive its use by the algorithm. */
+ckf_status ckf_algorithm_set_scorer(ckf_algorithm* alg, const ckf_scorer* scorer, float threshold);
+
+/* The plain CKF with its branches logged (training data of the scorer):
+ * every measurement link of the run, in link order, gives one sample --
+ * its 12 features at creation into d_features[n][12] and, into d_labels[n],
+ * 1 when the link lies on its seed's best found track candidate (most
+ * measurements, then the lowest chi2 sum), 0 otherwise: the branch that the
+ * seed's best track needs survives, every other branch dies.  Writes at most
+ * `capacity` rows; *n_samples gets the number of samples of the run. */
+ckf_status ckf_find_tracks_logged(ckf_algorithm* alg, const ckf_device_detector* det,
+                                  const ckf_device_event* ev, float* d_features,
+                                  uint8_t* d_labels, uint32_t capacity, uint32_t* n_samples,
+                                  ckf_device_tracks** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,8 @@ def _lib() -> C.CDLL:
                                             C.POINTER(F.BoundParams), C.c_uint32, pvp]
     lib.ckf_device_event_destroy.argtypes = [vp]
     lib.ckf_find_tracks.argtypes = [vp, vp, vp, pvp]
+    lib.ckf_algorithm_set_scorer.argtypes = [vp, vp, C.c_float]
+    lib.ckf_find_tracks_logged.argtypes = [vp, vp, vp, vp, vp, C.c_uint32, F.c_u32_p, pvp]
     lib.ckf_find_tracks_host.argtypes = [vp, C.POINTER(F.Detector), C.POINTER(F.BField),
                                          C.POINTER(F.Measurements), C.POINTER(F.BoundParams), pvp]
     lib.ckf_tracks_size.argtypes = [vp, F.c_u32_p, F.c_u32_p]
@@ -236,6 +238,28 @@ class combinatorial_kalman_filter_algorithm:  # noqa: N801  (traccc's class name
         out = C.c_void_p()
         _check(_lib().ckf_find_tracks(self._h, ddet._h, devents._h, C.byref(out)))
         return DeviceTracks(out)
+
+    def set_scorer(self, scorer, threshold: float = 0.0) -> None:
+        """Prune candidates whose scorer logit is below `threshold` before the
+        Kalman update (scorer None: the plain CKF).  The scorer object must
+        outlive its use."""
+        self._scorer = scorer  # keeps it alive while the algorithm uses it
+        _check(_lib().ckf_algorithm_set_scorer(self._h, scorer.handle if scorer else None,
+                                               float(threshold)))
+
+    def run_device_logged(self, ddet: DeviceDetector, devents: DeviceEvents, capacity: int):
+        """Plain CKF with its branches logged: (tracks, features [n][12],
+        labels [n], n) with features / labels as torch tensors on the GPU."""
+        import torch
+        feats = torch.empty((max(capacity, 1), 12), dtype=torch.float32, device="cuda")
+        labels = torch.empty(max(capacity, 1), dtype=torch.uint8, device="cuda")
+        n = C.c_uint32()
+        out = C.c_void_p()
+        _check(_lib().ckf_find_tracks_logged(self._h, ddet._h, devents._h, feats.data_ptr(),
+                                             labels.data_ptr(), capacity, C.byref(n),
+                                             C.byref(out)))
+        k = min(n.value, capacity)
+        return DeviceTracks(out), feats[:k], labels[:k], n.value
 
     def __call__(self, detector, field, measurements, seeds=None) -> TrackContainer:
         """traccc-shaped operator(det, field, measurements, seeds).  With

@@ -12,6 +12,7 @@
 #include <cstdint>
 
 #include "../core/propagator.hpp"
+#include "../scorer/mlp_view.hpp"
 
 namespace tamd {
 
@@ -146,6 +147,23 @@ struct ckf_params {
     uint32_t out_cap;
     uint32_t out_const_cap;
     uint64_t* vote_key;           // [n]; written when measurement votes run, else null
+    // branch logging (training data of the scorer): features of every
+    // filtered slot, copied to every measurement link; survive marks by build
+    float* feat_slot;             // [flt_cap][12] or null
+    float* log_feat;              // [link_cap][12] or null
+    uint8_t* log_label;           // [link_cap] or null
+    uint64_t* seed_best;          // [n_seeds] best kept tip of each seed (logging)
+    // scorer-pruned finding: per input the list_len best candidates in
+    // (chi2, index) order with their features, scored before the updates
+    uint32_t scored;              // 1: k_find takes its candidates from the list
+    uint32_t list_len;
+    uint32_t* list_m;             // [in_cap][list_len] measurement
+    float* list_c;                // [in_cap][list_len] chi2
+    uint32_t* list_n;             // [in_cap] entries
+    float* list_feat;             // [in_cap][list_len][12]
+    uint8_t* cand_keep;           // [in_cap][list_len] scorer verdict
+    mlp_dev scorer;
+    float score_threshold;        // logit below which a candidate is pruned
 };
 
 }  // namespace tamd

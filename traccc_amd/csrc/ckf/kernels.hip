@@ -24,7 +24,9 @@
 
 #include "device_types.hpp"
 #include "kalman_device.hpp"
+#include "features.hpp"
 #include "kernels.hpp"
+#include "../scorer/mlp_device.hpp"
 #include "state_io.hpp"
 #include "wave.hpp"
 
@@ -53,7 +55,192 @@ constexpr int kTeam = 4;
 constexpr int kTeamsPerWave = kWave / kTeam;
 constexpr int kBatch = 4;
 
+// One pass of a team over the surface's measurements [lo, up): the best
+// (chi2, index) candidate after (last_c, last_m) (every candidate when
+// !have_last) that passes the cut and the step-0 rule.  kInvalid when none.
+// With count_pass, n_pass receives the number of measurements passing the cut
+// (on every lane of the team).
+__device__ inline void scan_next(const ckf_params& P, const float* x, const float* c, uint32_t lo,
+                                 uint32_t up, bool rule0, bool have_last, float last_c,
+                                 uint32_t last_m, uint32_t tl, bool count_pass, float& best_c,
+                                 uint32_t& best_m, uint32_t& n_pass) {
+    best_c = 0.f;
+    best_m = kInvalid;
+    uint32_t np = 0u;
+    for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
+        meas_data mb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const float chi2 = predicted_chi2(mb[u], x, c);
+            // outside the cut (a NaN never yields a link either)
+            bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
+            take = take && !(rule0 && chi2 != 0.f);
+            np += take ? 1u : 0u;
+            take = take && !(have_last && (chi2 < last_c || (chi2 == last_c && m <= last_m)));
+            if (take && (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m))) {
+                best_c = chi2;
+                best_m = m;
+            }
+        }
+    }
+    // team minimum of (chi2, index); kInvalid sorts last
+#pragma unroll
+    for (int d = kTeam / 2; d >= 1; d >>= 1) {
+        const float oc = __shfl_xor(best_c, d, kTeam);
+        const uint32_t om = __shfl_xor(best_m, d, kTeam);
+        const bool take = (om != kInvalid) &&
+                          (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
+        if (take) {
+            best_c = oc;
+            best_m = om;
+        }
+    }
+    if (count_pass) {
+#pragma unroll
+        for (int d = kTeam / 2; d >= 1; d >>= 1) np += __shfl_xor(np, d, kTeam);
+        n_pass = np;
+    }
+}
+
+// The input branch a team works on at this step: its predicted state, its
+// previous link's bookkeeping and the surface's measurement range.
+struct find_input {
+    float x[6], c[21];
+    uint32_t sf, seed, prev_g, skip, consec, ndf_prev, in_slot, lo, up;
+    float chi2_prev;
+    bool rule0;
+};
+
+__device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i,
+                                  uint32_t prev_base, find_input& in) {
+    const uint32_t in_par = (step + 1u) & 1u;
+    in.prev_g = kInvalid;
+    in.skip = in.consec = in.ndf_prev = in.in_slot = 0u;
+    in.chi2_prev = 0.f;
+    if (step == 0u) {
+        in.seed = i;
+        load_seed(P, i, in.x, in.c, in.sf);
+    } else {
+        in.in_slot = P.live[i];
+        in.prev_g = prev_base + in.in_slot;
+        const dlink L = load_link(P, in.prev_g);
+        in.seed = L.seed;
+        in.skip = link_skipped(L);
+        in.consec = link_consec(L);
+        in.chi2_prev = L.chi2_sum;
+        in.ndf_prev = L.ndf_sum;
+        load_soa(P.state[in_par], P.state_cap, in.in_slot, in.x, in.c, in.sf);
+    }
+    const uint32_t ev = P.seed_event[in.seed];
+    const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
+    in.lo = rg[in.sf];
+    in.up = rg[in.sf + 1u];
+    in.rule0 = (step == 0u) && !(P.sfs[in.sf].thickness_x0 > 0.f);
+}
+
+// Scorer-pruned finding, first kernel: per input the list_len best candidates
+// in (chi2, index) order and their features.
+__global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
+    const step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t step = ctl->step;
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t prev_base = ctl->prev_link_base;
+    const uint32_t team = lane() / kTeam;
+    const uint32_t tl = lane() % kTeam;
+    const uint32_t Lc = P.list_len;
+    for (uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * kTeamsPerWave; ;
+         i0 += gridDim.x * (blockDim.x / kWave) * kTeamsPerWave) {
+        if (i0 >= n_in) {
+            break;  // uniform over the wave
+        }
+        const uint32_t i = i0 + team;
+        if (i < n_in) {
+            find_input in;
+            load_input(P, step, i, prev_base, in);
+            bool have_last = false;
+            float last_c = 0.f;
+            uint32_t last_m = 0u, n_pass = 0u, k = 0u;
+            for (; k < Lc; ++k) {
+                float best_c;
+                uint32_t best_m;
+                scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, have_last, last_c, last_m, tl,
+                          k == 0u, best_c, best_m, n_pass);
+                if (best_m == kInvalid) {
+                    break;
+                }
+                have_last = true;
+                last_c = best_c;
+                last_m = best_m;
+                if (tl == 0u) {
+                    const size_t r = static_cast<size_t>(i) * Lc + k;
+                    P.list_m[r] = best_m;
+                    P.list_c[r] = best_c;
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
+                                       in.chi2_prev, in.ndf_prev, best_c, k, n_pass,
+                                       P.list_feat + r * kFeatures);
+                }
+            }
+            if (tl == 0u) {
+                P.list_n[i] = k;
+            }
+        }
+    }
+}
+
+// Scorer-pruned finding, second kernel: the MLP over every listed candidate
+// (64-row tiles on the matrix cores), keep = logit >= threshold.
+__global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
+    __shared__ short lds[(kBlock / kWave) * kMlpLdsPerWave];
+    const step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t rows = n_in * P.list_len;
+    const uint32_t wave = threadIdx.x / kWave;
+    short* lds_x = lds + wave * kMlpLdsPerWave;
+    short* lds_h = lds_x + 64 * kMlpInPad;
+    const uint32_t n_tiles = (rows + 63u) / 64u;
+    for (uint32_t t = blockIdx.x * (blockDim.x / kWave) + wave; t < n_tiles;
+         t += gridDim.x * (blockDim.x / kWave)) {
+        const uint32_t r = t * 64u + lane();
+        bool valid = r < rows;
+        if (valid) {
+            valid = (r % P.list_len) < P.list_n[r / P.list_len];
+        }
+        float f[kMlpIn];
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
+        const float logit = mlp_tile_logit(P.scorer, f, valid, lds_x, lds_h);
+        if (valid) {
+            P.cand_keep[r] = logit >= P.score_threshold ? 1u : 0u;
+        }
+        wave_lds_sync();
+    }
+}
+
+// A team of kTeam lanes handles one input branch.  Plain: the lanes split
+// the surface's measurements (scan_next) and take candidates in (chi2, index)
+// order; logged: the same, with every filtered slot's features kept for the
+// branch log; scored: the candidates come from the scored list, pruned ones
+// skipped.  The gain update runs redundantly on the team (so it stays
+// uniform) and the team leader writes.  A wave holds kWave / kTeam inputs.
+enum find_mode : int { kFindPlain = 0, kFindLogged = 1, kFindScored = 2 };
+
+template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
+    constexpr bool kScored = (kMode == kFindScored);
+    constexpr bool logging = (kMode == kFindLogged);
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
         return;
@@ -64,7 +251,6 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
-    const uint32_t in_par = (step + 1u) & 1u;
     const uint32_t N = P.n_branch_surface;
     const uint32_t team = lane() / kTeam;
     const uint32_t tl = lane() % kTeam;
@@ -81,65 +267,34 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         uint32_t in_slot = 0u;
         float chi2_prev = 0.f;
         if (valid) {
-            float x[6], c[21];
-            uint32_t sf = 0u;
-            if (step == 0u) {
-                seed = i;
-                load_seed(P, i, x, c, sf);
-            } else {
-                in_slot = P.live[i];
-                prev_g = prev_base + in_slot;
-                const dlink L = load_link(P, prev_g);
-                seed = L.seed;
-                skip = link_skipped(L);
-                consec = link_consec(L);
-                chi2_prev = L.chi2_sum;
-                ndf_prev = L.ndf_sum;
-                load_soa(P.state[in_par], P.state_cap, in_slot, x, c, sf);
-            }
-            const uint32_t ev = P.seed_event[seed];
-            const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
-            const uint32_t lo = rg[sf];
-            const uint32_t up = rg[sf + 1u];
-            const bool rule0 = (step == 0u) && !(P.sfs[sf].thickness_x0 > 0.f);
+            find_input in;
+            load_input(P, step, i, prev_base, in);
+            seed = in.seed;
+            prev_g = in.prev_g;
+            skip = in.skip;
+            consec = in.consec;
+            ndf_prev = in.ndf_prev;
+            in_slot = in.in_slot;
+            chi2_prev = in.chi2_prev;
             bool have_last = false;
             float last_c = 0.f;
-            uint32_t last_m = 0u;
+            uint32_t last_m = 0u, n_pass = 0u;
+            uint32_t li = 0u;
+            const uint32_t ln = kScored ? P.list_n[i] : 0u;
             while (count < N) {
                 float best_c = 0.f;
                 uint32_t best_m = kInvalid;
-                for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
-                    meas_data mb[kBatch];
-#pragma unroll
-                    for (int u = 0; u < kBatch; ++u) {
-                        const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
-                        mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
+                if constexpr (kScored) {
+                    const size_t r0 = static_cast<size_t>(i) * P.list_len;
+                    while (li < ln && P.cand_keep[r0 + li] == 0u) ++li;
+                    if (li < ln) {
+                        best_m = P.list_m[r0 + li];
+                        best_c = P.list_c[r0 + li];
+                        ++li;
                     }
-#pragma unroll
-                    for (int u = 0; u < kBatch; ++u) {
-                        const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
-                        const float chi2 = predicted_chi2(mb[u], x, c);
-                        // outside the cut (a NaN never yields a link either)
-                        bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
-                        take = take && !(rule0 && chi2 != 0.f);
-                        take = take && !(have_last && (chi2 < last_c || (chi2 == last_c && m <= last_m)));
-                        if (take && (best_m == kInvalid || chi2 < best_c || (chi2 == best_c && m < best_m))) {
-                            best_c = chi2;
-                            best_m = m;
-                        }
-                    }
-                }
-                // team minimum of (chi2, index); kInvalid sorts last
-#pragma unroll
-                for (int d = kTeam / 2; d >= 1; d >>= 1) {
-                    const float oc = __shfl_xor(best_c, d, kTeam);
-                    const uint32_t om = __shfl_xor(best_m, d, kTeam);
-                    const bool take = (om != kInvalid) &&
-                                      (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
-                    if (take) {
-                        best_c = oc;
-                        best_m = om;
-                    }
+                } else {
+                    scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, have_last, last_c, last_m,
+                              tl, logging && !have_last, best_c, best_m, n_pass);
                 }
                 if (best_m == kInvalid) {
                     break;
@@ -150,11 +305,11 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                 const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
                 float fx[6], fc[21];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) fx[k] = x[k];
+                for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
 #pragma unroll
-                for (int k = 0; k < 21; ++k) fc[k] = c[k];
+                for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
                 int res = CKF_FIT_SUCCESS;
-                if (rule0) {
+                if (in.rule0) {
                     // step 0 on a surface without material: keep the seed,
                     // take the measurement variances
                     fc[sym_idx(0, 0)] = md.v0;
@@ -166,9 +321,14 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                     const uint32_t slot = i * N + count;
                     if (leader) {
                         if (slot < P.flt_cap) {
-                            store_soa(P.flt, P.flt_cap, slot, fx, fc, sf);
+                            store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
                             P.cand_meas[slot] = best_m;
                             P.cand_chi2[slot] = best_c;
+                            if constexpr (logging) {
+                                candidate_features(P, md, in.x, in.c, in.sf, step, skip, consec,
+                                                   chi2_prev, ndf_prev, best_c, count, n_pass,
+                                                   P.feat_slot + static_cast<size_t>(slot) * kFeatures);
+                            }
                         } else {
                             set_overflow(ctl, 4u);
                         }
@@ -232,6 +392,13 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                     const uint32_t g = link_base + first + k;
                     if (g < P.link_cap && first + k < P.state_cap) {
                         P.links[g] = L;
+                        if constexpr (logging) {
+#pragma unroll
+                            for (int q = 0; q < kFeatures; ++q) {
+                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] =
+                                    P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
+                            }
+                        }
                     } else {
                         set_overflow(ctl, 1u);
                     }
@@ -652,6 +819,31 @@ __global__ __launch_bounds__(1024) void k_build_table(ckf_params P) {
     }
 }
 
+// Ranking key of a kept tip among its seed's tips: more measurements first,
+// then the lower chi2 sum, then the lower tip index.
+__device__ inline uint64_t tip_rank_key(uint32_t n_cands, float chi2_sum, uint32_t t) {
+    const uint32_t cb = __float_as_uint(chi2_sum >= 0.f ? chi2_sum : 0.f);  // order-preserving
+    return (static_cast<uint64_t>(n_cands & 0xffu) << 56) |
+           (static_cast<uint64_t>(~cb) << 24) |
+           static_cast<uint64_t>((0xffffffu - (t < 0xffffffu ? t : 0xffffffu)) & 0xffffffu);
+}
+
+// Branch log: each seed's best kept tip (its survive labels go to that
+// track's links only).
+__global__ __launch_bounds__(kBlock) void k_label_best(ckf_params P) {
+    const step_ctl* ctl = P.ctl;
+    const uint32_t n = ctl->n_tips < P.tip_cap ? ctl->n_tips : P.tip_cap;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t kv = P.tip_keep[t];
+        if ((kv >> 31) == 0u) {
+            continue;
+        }
+        const dlink L = load_link(P, P.tips[t]);
+        atomicMax(reinterpret_cast<unsigned long long*>(&P.seed_best[L.seed]),
+                  static_cast<unsigned long long>(tip_rank_key(kv & 0x7fffffffu, L.chi2_sum, t)));
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
     const step_ctl* ctl = P.ctl;
     const uint32_t n = ctl->n_tips < P.tip_cap ? ctl->n_tips : P.tip_cap;
@@ -662,7 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
             continue;
         }
         const uint32_t nc = kv & 0x7fffffffu;
-        dlink L = load_link(P, P.tips[t]);
+        uint32_t g = P.tips[t];
+        dlink L = load_link(P, g);
         const uint32_t ev = P.seed_event[L.seed];
         const size_t seg = static_cast<size_t>(ev) * S + link_step(L);
         const uint32_t pos = P.seg_off[seg] + (P.tip_rank[t] - P.seg_base[seg]);
@@ -671,6 +864,9 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
             continue;
         }
         const uint32_t moff = P.event_meas_off[ev];
+        // logged runs: only the seed's best track marks its links as survivors
+        const bool mark = P.log_label != nullptr &&
+                          P.seed_best[L.seed] == tip_rank_key(nc, L.chi2_sum, t);
         if (P.vote_key != nullptr) {
             // p-value of the tip's running sums ranks the track in measurement votes
             const float vp = prob_chi2(L.chi2_sum, static_cast<float>(L.ndf_sum) - 5.f);
@@ -679,11 +875,20 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
         }
         float ndf = 0.f, chi2 = 0.f;
         for (int k = static_cast<int>(nc) - 1; k >= 0; --k) {
-            while (L.meas == kInvalid && link_step(L) != 0u) L = load_link(P, L.prev);
+            while (L.meas == kInvalid && link_step(L) != 0u) {
+                g = L.prev;
+                L = load_link(P, g);
+            }
             P.out_const[cpos + static_cast<uint32_t>(k)] = L.meas - moff;
             ndf += static_cast<float>(P.meas_info[L.meas] & 0xffu);
             chi2 += L.chi2;
-            if (k > 0) L = load_link(P, L.prev);
+            if (mark) {
+                P.log_label[g] = 1u;  // this measurement link is on the seed's best track
+            }
+            if (k > 0) {
+                g = L.prev;
+                L = load_link(P, g);
+            }
         }
         ndf -= 5.f;
         const uint32_t s = L.seed;
@@ -824,7 +1029,66 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    hipLaunchKernelGGL(k_find, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    if (P.scored) {
+        hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_score, dim3(s.score_blocks), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_find<kFindScored>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    } else if (P.feat_slot != nullptr) {
+        hipLaunchKernelGGL(k_find<kFindLogged>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    } else {
+        hipLaunchKernelGGL(k_find<kFindPlain>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    }
+    return hipGetLastError();
+}
+
+// Branch log compaction: measurement links [0, n) in order, 1024 per block.
+__global__ __launch_bounds__(1024) void k_log_count(ckf_params P, uint32_t n, uint32_t* block_n) {
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    const uint32_t g = blockIdx.x * 1024u + threadIdx.x;
+    if (g < n && P.links[g].meas != kInvalid) atomicAdd(&s_n, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) block_n[blockIdx.x] = s_n;
+}
+
+__global__ __launch_bounds__(1024) void k_log_write(ckf_params P, uint32_t n, const uint32_t* block_off,
+                                                    float* features, uint8_t* labels, uint32_t cap) {
+    __shared__ uint32_t s_w[16];
+    const uint32_t g = blockIdx.x * 1024u + threadIdx.x;
+    const bool is_meas = g < n && P.links[g].meas != kInvalid;
+    const uint64_t m = wave_ballot(is_meas);
+    const uint32_t wv = threadIdx.x / kWave;
+    if (lane() == 0u) s_w[wv] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    uint32_t before = 0u;
+    for (uint32_t w = 0; w < wv; ++w) before += s_w[w];
+    if (is_meas) {
+        const uint32_t pos = block_off[blockIdx.x] + before + prefix_count(m);
+        if (pos < cap) {
+#pragma unroll
+            for (int q = 0; q < kFeatures; ++q) {
+                features[static_cast<size_t>(pos) * kFeatures + q] = P.log_feat[static_cast<size_t>(g) * kFeatures + q];
+            }
+            labels[pos] = P.log_label[g];
+        }
+    }
+}
+
+hipError_t launch_label_best(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    hipLaunchKernelGGL(k_label_best, dim3(s.build_blocks), dim3(kBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_log_count(const ckf_params& P, uint32_t n, uint32_t* block_n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_log_count, dim3((n + 1023u) / 1024u), dim3(1024), 0, stream, P, n, block_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_log_write(const ckf_params& P, uint32_t n, const uint32_t* block_off,
+                            float* features, uint8_t* labels, uint32_t cap, hipStream_t stream) {
+    hipLaunchKernelGGL(k_log_write, dim3((n + 1023u) / 1024u), dim3(1024), 0, stream, P, n,
+                       block_off, features, labels, cap);
     return hipGetLastError();
 }
 

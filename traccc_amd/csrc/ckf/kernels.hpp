@@ -13,6 +13,8 @@ struct launch_shape {
     uint32_t prop_blocks;
     uint32_t build_blocks;
     uint32_t prop_team;  // lanes per link in k_propagate: 1, 4, 8 or 16
+    uint32_t list_blocks;   // scorer-pruned finding: candidate lists
+    uint32_t score_blocks;  // and the MLP over them
 };
 
 // One CKF step: find_tracks, duplicate removal (active from
@@ -33,6 +35,13 @@ hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
 // build_tracks: tip scan, (event, step) segment table, output fill.
 hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+
+// Branch log (scorer training data): measurement links [0, n) in order into
+// caller arrays; block_n / block_off hold one count / offset per 1024 links.
+hipError_t launch_label_best(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_log_count(const ckf_params& P, uint32_t n, uint32_t* block_n, hipStream_t stream);
+hipError_t launch_log_write(const ckf_params& P, uint32_t n, const uint32_t* block_off,
+                            float* features, uint8_t* labels, uint32_t cap, hipStream_t stream);
 
 // Measurement-vote pruning of built tracks (max_num_tracks_per_measurement > 0).
 struct vote_params {

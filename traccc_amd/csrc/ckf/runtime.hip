@@ -24,6 +24,7 @@
 #include "kernels.hpp"
 #include "traccc_amd/ckf.h"
 #include "traccc_amd/ckf_debug.h"
+#include "traccc_amd/scorer.h"
 
 namespace tamd {
 std::string& last_error() {
@@ -225,6 +226,12 @@ struct ckf_algorithm {
         seg_coff, ev_toff, ev_coff;
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
+    // scorer training data (branch log) and scorer-pruned finding
+    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_feat, cand_keep;
+    bool logging = false;
+    const ckf_scorer* scorer = nullptr;
+    float score_threshold = 0.f;
+    uint32_t list_len = 0;
     // progressive filter (run_pkf)
     dbuf pkf_states, pkf_n, pkf_chi2, pkf_ndf, pkf_holes, pkf_cpos, pkf_lb_flag, pkf_lb_agg,
         pkf_lb_inc;
@@ -251,7 +258,9 @@ struct ckf_algorithm {
                        &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff, &vote_slots,
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
-                       &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc};
+                       &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc, &feat_slot, &log_feat,
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_feat,
+                       &cand_keep};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
             lb_flag[k].release();
@@ -321,6 +330,21 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.seg_coff.ensure(4u * seg);
     A.ev_toff.ensure(4u * (size_t(ev.n_events) + 1u));
     A.ev_coff.ensure(4u * (size_t(ev.n_events) + 1u));
+    if (A.logging) {
+        A.feat_slot.ensure(4u * 12u * size_t(A.flt_cap));
+        A.log_feat.ensure(4u * 12u * size_t(A.link_cap));
+        A.log_label.ensure(size_t(A.link_cap));
+        A.seed_best.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
+    }
+    if (A.scorer != nullptr) {
+        A.list_len = std::min(A.cfg.max_num_branches_per_surface + 4u, 16u);
+        const size_t rows = size_t(A.in_cap) * A.list_len;
+        A.list_m.ensure(4u * rows);
+        A.list_c.ensure(4u * rows);
+        A.list_feat.ensure(4u * 12u * rows);
+        A.cand_keep.ensure(rows);
+        A.list_n.ensure(4u * size_t(A.in_cap));
+    }
     if (A.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
                                 hipHostMallocDefault),
@@ -395,6 +419,23 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.event_track_off = A.ev_toff.as<uint32_t>();
     P.event_const_off = A.ev_coff.as<uint32_t>();
     P.event_meas_off = ev.d_meas_off;
+    if (A.logging) {
+        P.feat_slot = A.feat_slot.as<float>();
+        P.log_feat = A.log_feat.as<float>();
+        P.log_label = A.log_label.as<uint8_t>();
+        P.seed_best = A.seed_best.as<uint64_t>();
+    }
+    if (A.scorer != nullptr) {
+        P.scored = 1u;
+        P.list_len = A.list_len;
+        P.list_m = A.list_m.as<uint32_t>();
+        P.list_c = A.list_c.as<float>();
+        P.list_n = A.list_n.as<uint32_t>();
+        P.list_feat = A.list_feat.as<float>();
+        P.cand_keep = A.cand_keep.as<uint8_t>();
+        P.scorer = tamd::scorer_view(A.scorer);
+        P.score_threshold = A.score_threshold;
+    }
     return P;
 }
 
@@ -410,6 +451,10 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     s.prop_team = A.prop_team;
     const uint32_t prop_parts = (A.state_cap + 64u / s.prop_team - 1u) / (64u / s.prop_team);
     s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
+    // scorer-pruned finding: 16 inputs per wave listing, 64 rows per scoring wave
+    s.list_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
+    const uint64_t rows = uint64_t(A.in_cap) * std::max(A.list_len, 1u);
+    s.score_blocks = static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>((rows / 64u + 4u) / 4u, max_blocks)));
     s.build_blocks = blocks(A.tip_cap);
     return s;
 }
@@ -826,7 +871,11 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
     A.stats.n_propagations = done.n_props;
     A.stats.n_tips = done.n_tips;
 
-    // build_tracks: count, size, fill
+    // build_tracks: count, size, fill (the build marks logged links on tracks)
+    if (A.logging) {
+        hip_check(hipMemsetAsync(A.log_label.p, 0, A.log_label.bytes, A.stream), "memset");
+        hip_check(hipMemsetAsync(A.seed_best.p, 0, A.seed_best.bytes, A.stream), "memset");
+    }
     const size_t seg = size_t(ev.n_events) * c.max_track_candidates_per_track;
     hip_check(hipMemsetAsync(P.seg_count, 0, 4u * seg, A.stream), "memset");
     hip_check(hipMemsetAsync(P.seg_ccount, 0, 4u * seg, A.stream), "memset");
@@ -875,6 +924,9 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
         if (votes) {
             A.vote_key.ensure(8u * size_t(nt));
             P.vote_key = A.vote_key.as<uint64_t>();
+        }
+        if (A.logging) {
+            hip_check(tamd::launch_label_best(P, shape, A.stream), "label best");
         }
         hip_check(tamd::launch_build_write(P, shape, A.stream), "build write");
         const uint32_t* track_off = P.event_track_off;
@@ -1131,6 +1183,78 @@ ckf_status ckf_find_tracks(ckf_algorithm* alg, const ckf_device_detector* det,
             throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "event was uploaded for another detector");
         }
         *out = find_tracks(*alg, *det, *ev);
+    });
+}
+
+ckf_status ckf_algorithm_set_scorer(ckf_algorithm* alg, const ckf_scorer* scorer, float threshold) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        if (!(threshold == threshold)) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "threshold is NaN");
+        alg->scorer = scorer;
+        alg->score_threshold = threshold;
+    });
+}
+
+ckf_status ckf_find_tracks_logged(ckf_algorithm* alg, const ckf_device_detector* det,
+                                  const ckf_device_event* ev, float* d_features, uint8_t* d_labels,
+                                  uint32_t capacity, uint32_t* n_samples, ckf_device_tracks** out) {
+    return guarded([&] {
+        if (alg == nullptr || det == nullptr || ev == nullptr || out == nullptr ||
+            n_samples == nullptr || (capacity > 0u && (d_features == nullptr || d_labels == nullptr))) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        if (ev->n_surfaces != det->h_sfs.size()) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "event was uploaded for another detector");
+        }
+        if (use_pkf(alg->cfg) || alg->scorer != nullptr) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                            "branch logging runs the plain CKF (no progressive filter, no scorer)");
+        }
+        struct logging_scope {
+            ckf_algorithm& A;
+            ~logging_scope() { A.logging = false; }
+        } scope{*alg};
+        alg->logging = true;
+        ckf_device_tracks* T = find_tracks(*alg, *det, *ev);
+        try {
+            ckf_algorithm& A = *alg;
+            const uint32_t n = static_cast<uint32_t>(A.stats.n_links);
+            const tamd::ckf_params P = make_params(A, *det, *ev);
+            const uint32_t blocks = (n + 1023u) / 1024u;
+            std::vector<uint32_t> counts(blocks + 1u, 0u);
+            uint32_t total = 0u;
+            if (blocks > 0u) {
+                uint32_t* d_cnt = nullptr;
+                hip_check(hipMalloc(&d_cnt, 4u * (blocks + 1u)), "log buffers");
+                try {
+                    hip_check(tamd::launch_log_count(P, n, d_cnt, A.stream), "log count");
+                    hip_check(hipMemcpyAsync(counts.data(), d_cnt, 4u * blocks, hipMemcpyDeviceToHost,
+                                             A.stream), "log count");
+                    hip_check(hipStreamSynchronize(A.stream), "log count");
+                    for (uint32_t b = 0; b < blocks; ++b) {
+                        const uint32_t c = counts[b];
+                        counts[b] = total;
+                        total += c;
+                    }
+                    hip_check(hipMemcpyAsync(d_cnt, counts.data(), 4u * blocks, hipMemcpyHostToDevice,
+                                             A.stream), "log offsets");
+                    if (capacity > 0u) {
+                        hip_check(tamd::launch_log_write(P, n, d_cnt, d_features, d_labels, capacity,
+                                                         A.stream), "log write");
+                    }
+                    hip_check(hipStreamSynchronize(A.stream), "log write");
+                } catch (...) {
+                    (void)hipFree(d_cnt);
+                    throw;
+                }
+                (void)hipFree(d_cnt);
+            }
+            *n_samples = total;
+        } catch (...) {
+            delete T;
+            throw;
+        }
+        *out = T;
     });
 }
 

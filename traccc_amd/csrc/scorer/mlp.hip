@@ -688,3 +688,5 @@ ckf_status ckf_debug_mfma_layout(const float* A, const float* B, float* D) {
 }
 
 }  // extern "C"
+
+tamd::mlp_dev tamd::scorer_view(const ckf_scorer* s) { return s->view(); }

@@ -16,6 +16,8 @@
 
 #include <cstdint>
 
+#include "mlp_view.hpp"
+
 namespace tamd {
 
 constexpr int kMlpIn = 12;
@@ -36,19 +38,6 @@ constexpr int kMlpLdsPerWave = 64 * kMlpInPad + 64 * kHStride;  // shorts
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// Device copy of the weights in the layouts the fragments load directly.
-struct mlp_dev {
-    const short* w1t;      // [64 out][16 in] bf16 (inputs 12..15 zero)
-    const short* w2;       // [64 in][64 out] bf16 (B of dH1 = dZ2 W2^T)
-    const short* w2t;      // [64 out][64 in] bf16 (B of Z2 = H1 W2)
-    const float* b1;       // [64]
-    const float* b2;       // [64]
-    const float* w3;       // [64] BF16-rounded, kept in FP32
-    const float* b3;       // [1]
-    const float* mean;     // [12] feature normalisation
-    const float* inv_std;  // [12]
-};
 
 __device__ inline short f2bf(float f) {
     // round to nearest even (finite inputs)
