@@ -8,7 +8,9 @@ HBM before the timed region; the output stays in HBM.  tracks/s = seeds
 processed / wall time.
 
 Extra fields: roofline of the dominant kernel (HIP-event timed), finding
-efficiency, and the FP32 CPU oracle timed on the host cores.
+efficiency, the FP32 CPU oracle timed on the host cores, the batched C3 line
+(256 events per call) and the C4 line: plain CKF against the CKF pruned by the
+learned branch scorer, trained here on the GPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -20,6 +22,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -79,6 +83,182 @@ def batched_line(a, ckf, S, torch):
             "value": round(n_seeds / dt, 1), "unit": "tracks/s", "found_tracks": int(n_tracks)}
 
 
+def _batch_efficiency(events, tracks):
+    from traccc_amd import performance as PERF
+    return float(np.mean([PERF.finding_performance(ev, t.offsets, t.constituents).efficiency
+                          for ev, t in ((ev, tracks.event(e)) for e, ev in enumerate(events))]))
+
+
+def _time_calls(alg, ddet, devents, steps, torch):
+    for _ in range(2):
+        alg.run_device(ddet, devents)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = alg.run_device(ddet, devents)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def pruned_line(a, ckf, S, torch, wl="C4", n_particles=20000, n_test=32, n_train=128,
+                n_steps=5, fit=False):
+    """A dense workload (C4: 20,000 pions per event, C2 detector; C5: 60,000
+    pions, 5 barrel layers + 2x6 disks, 2% X0, 3% dead modules, solenoid
+    map): the plain CKF with three branches per surface against the same CKF
+    with the learned branch scorer, on one batch of events.  The scorer is
+    trained here, on the GPU, on survive / die labels that the plain CKF logs
+    on its own training events (a link survives when it lies on its seed's
+    best found track); the threshold is the most aggressive one that keeps
+    validation efficiency within 0.4% absolute of plain; the timed batch is
+    held out.  With fit=True the pruned candidates then go through the Kalman
+    fit + two-filter smoother, timed on its own and as the chained call."""
+    from traccc_amd import scorer as SC
+    threads = min(16, len(os.sched_getaffinity(0)))
+    cfg = S.finding_config_for(wl)
+    cfg.max_num_branches_per_surface = 3
+    det, fld, _ = S.workload(wl, n_particles=10)
+    ddet = ckf.DeviceDetector(det, fld)
+
+    def events(n, seed0):
+        guns = [S.workload(wl, n_particles=n_particles, seed=seed0 + e)[2] for e in range(n)]
+        return S.generate_events(det, fld, guns, threads=threads)
+
+    plain = ckf.combinatorial_kalman_filter_algorithm(cfg)
+    # branch log of the plain CKF over the training events
+    t0 = time.perf_counter()
+    feats, labels = [], []
+    chunk = max(1, 640000 // n_particles)
+    for c in range(0, n_train, chunk):
+        evs = events(min(chunk, n_train - c), 100000 + c)
+        dev = ckf.DeviceEvents(ddet, evs)
+        cap = 16 * n_particles * len(evs)
+        _, f, y, n = plain.run_device_logged(ddet, dev, capacity=cap)
+        if n > cap:  # rerun with room for every sample
+            cap = n
+            _, f, y, n = plain.run_device_logged(ddet, dev, capacity=cap)
+        feats.append(f.clone())
+        labels.append(y.clone())
+        del dev
+    X = torch.cat(feats)
+    Y = torch.cat(labels)
+    del feats, labels
+    t_log = time.perf_counter() - t0
+    sc = SC.Scorer(seed=7)
+    sc.fit_normalisation(X)
+    tcfg = SC.train_config(epochs=a.scorer_epochs, batch=16384, learning_rate=2e-3)
+    st = sc.train(X, Y, tcfg)
+    n_samples = int(X.shape[0])
+    survive = float(Y.float().mean())
+    del X, Y
+    torch.cuda.synchronize()
+    log(f"[bench] {wl} scorer: {n_samples} labels from {n_train} events "
+        f"(survive {survive:.3f}, {t_log:.1f} s), loss {st.first_loss:.4f} -> {st.last_loss:.4f}, "
+        f"{st.steps} Adam steps in {st.seconds:.2f} s, {st.tflops:.2f} TFLOP/s")
+
+    # threshold on validation events
+    val = events(max(2, 80000 // n_particles), 200000)
+    dval = ckf.DeviceEvents(ddet, val)
+    eff_plain_val = _batch_efficiency(val, plain.run_device(ddet, dval).download())
+    pruned = ckf.combinatorial_kalman_filter_algorithm(cfg)
+    chosen, sweep = None, []
+    for thr in (1.0, 0.0, -1.0, -1.5, -1.75, -2.0, -2.25, -2.5, -2.75, -3.0, -3.25, -3.5,
+                -3.75, -4.0, -4.5, -5.0, -6.0):
+        pruned.set_scorer(sc, threshold=thr)
+        eff = _batch_efficiency(val, pruned.run_device(ddet, dval).download())
+        sweep.append({"threshold": thr, "efficiency": round(eff, 5)})
+        if eff >= eff_plain_val - 0.004:
+            chosen = thr
+            break
+    del dval
+    # no threshold within the efficiency bound: the plain CKF runs (threshold null)
+    pruned.set_scorer(sc if chosen is not None else None,
+                      threshold=chosen if chosen is not None else 0.0)
+
+    # timed: the held-out batch, plain and pruned
+    test = events(n_test, 300000)
+    n_seeds = sum(e.n_seeds for e in test)
+    dev = ckf.DeviceEvents(ddet, test)
+    dt_p, out_p = _time_calls(plain, ddet, dev, n_steps, torch)
+    st_p = plain.last_stats()
+    dt_s, out_s = _time_calls(pruned, ddet, dev, n_steps, torch)
+    st_s = pruned.last_stats()
+    tr_p, tr_s = out_p.download(), out_s.download()
+    eff_p, eff_s = _batch_efficiency(test, tr_p), _batch_efficiency(test, tr_s)
+    # scorer kernels: device time over one timed call
+    from traccc_amd import _ffi as F
+    lib = F.load_library(ckf.LIB)
+    lib.ckf_debug_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.ckf_debug_scorer_time.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.ckf_debug_set_kernel_timing(pruned._h, 1)
+    pruned.run_device(ddet, dev)
+    ms_score = C.c_double()
+    lib.ckf_debug_scorer_time(pruned._h, C.byref(ms_score))
+    lib.ckf_debug_set_kernel_timing(pruned._h, 0)
+    n_scored = pruned.last_stats().n_scored
+    mlp_flops = 2 * (12 * 64 + 64 * 64 + 64)  # per scored candidate
+    infer_tflops = n_scored * mlp_flops / (ms_score.value * 1e-3) / 1e12 if ms_score.value else 0.0
+    log(f"[bench] {wl} batch {len(test)} events: plain {dt_p * 1e3:.2f} ms ({n_seeds / dt_p:.4g} "
+        f"tracks/s, eff {eff_p:.4f}), pruned {dt_s * 1e3:.2f} ms ({n_seeds / dt_s:.4g} tracks/s, "
+        f"eff {eff_s:.4f}) at threshold {chosen}; scorer {n_scored} candidates in "
+        f"{ms_score.value:.3f} ms ({infer_tflops:.2f} TFLOP/s)")
+    line = {
+        "workload": f"{wl}: {len(test)} events x {n_particles:,} pions (pT 0.3 GeV + Exp(1 GeV), "
+                    f"|eta|<2.5), 10% noise, 3 branches per surface, one operator call",
+        "seeds_per_call": n_seeds, "calls": n_steps,
+        "plain": {"value": round(n_seeds / dt_p, 1), "unit": "tracks/s",
+                  "ms_per_call": round(dt_p * 1e3, 3), "efficiency": round(eff_p, 5),
+                  "propagations": int(st_p.n_propagations), "links": int(st_p.n_links),
+                  "found_tracks": len(tr_p)},
+        "pruned": {"value": round(n_seeds / dt_s, 1), "unit": "tracks/s",
+                   "ms_per_call": round(dt_s * 1e3, 3), "efficiency": round(eff_s, 5),
+                   "propagations": int(st_s.n_propagations), "links": int(st_s.n_links),
+                   "found_tracks": len(tr_s), "threshold": chosen},
+        "speedup": round(dt_p / dt_s, 3),
+        "efficiency_loss": round(eff_p - eff_s, 5),
+        "threshold_sweep": sweep,
+        "scorer": {
+            "train_labels": n_samples, "train_events": n_train,
+            "survive_fraction": round(survive, 4), "epochs": a.scorer_epochs,
+            "adam_steps": int(st.steps), "train_seconds": round(st.seconds, 3),
+            "loss": [round(st.first_loss, 5), round(st.last_loss, 5)],
+            "train_mfma": {"achieved": round(st.tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                           "frac": round(st.tflops / 2500.0, 6)},
+            "inference_mfma": {"achieved": round(infer_tflops, 3), "peak": 2500.0,
+                               "unit": "TFLOP/s", "frac": round(infer_tflops / 2500.0, 6),
+                               "scored_candidates": int(n_scored),
+                               "list_and_score_ms": round(ms_score.value, 4)},
+        },
+    }
+    if fit:
+        from traccc_amd import fitting as FT
+        fitter = FT.kalman_fitting_algorithm(FT.fitting_config(cfg))
+        fitter.run_device(ddet, dev, out_s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            fitted = fitter.run_device(ddet, dev, out_s)
+        torch.cuda.synchronize()
+        dt_f = (time.perf_counter() - t0) / n_steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            cands = pruned.run_device(ddet, dev)
+            fitted = fitter.run_device(ddet, dev, cands)
+        torch.cuda.synchronize()
+        dt_c = (time.perf_counter() - t0) / n_steps
+        nt, ns = fitted.sizes()
+        outc = fitted.download().fit_outcome
+        ok = float(np.mean(outc == FT.OUTCOME_SUCCESS)) if len(outc) else 0.0
+        log(f"[bench] {wl} fit: {nt} tracks, {ns} states in {dt_f * 1e3:.2f} ms "
+            f"({nt / dt_f:.4g} fitted tracks/s, {ok:.4f} fitted); pruned CKF + fit "
+            f"{dt_c * 1e3:.2f} ms per call -> {n_seeds / dt_c:.4g} tracks/s")
+        line["fit"] = {"tracks": int(nt), "states": int(ns), "ms_per_call": round(dt_f * 1e3, 3),
+                       "fitted_tracks_per_s": round(nt / dt_f, 1), "success_fraction": round(ok, 5)}
+        line["pruned_ckf_plus_fit"] = {"value": round(n_seeds / dt_c, 1), "unit": "tracks/s",
+                                       "ms_per_call": round(dt_c * 1e3, 3)}
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +270,14 @@ def main() -> int:
     ap.add_argument("--batched-events", type=int, default=256,
                     help="events of the batched C3 line (0 skips it)")
     ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--pruned-events", type=int, default=32,
+                    help="events of the timed C4 plain-vs-pruned batch (0 skips the line)")
+    ap.add_argument("--pruned-steps", type=int, default=5)
+    ap.add_argument("--scorer-train-events", type=int, default=128)
+    ap.add_argument("--scorer-epochs", type=int, default=3)
+    ap.add_argument("--c5-events", type=int, default=64,
+                    help="events of the C5 line (60,000 pions each; 0 skips it)")
+    ap.add_argument("--c5-train-events", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -100,7 +288,6 @@ def main() -> int:
               "not part of this build (use --gpus 1)", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
     if a.copy_calibration > 0:
@@ -180,6 +367,14 @@ def main() -> int:
     batched = None
     if a.batched_events > 0:
         batched = batched_line(a, ckf, S, torch)
+    pruned = None
+    if a.pruned_events > 0:
+        pruned = pruned_line(a, ckf, S, torch, "C4", 20000, a.pruned_events,
+                             a.scorer_train_events, a.pruned_steps)
+    c5 = None
+    if a.c5_events > 0:
+        c5 = pruned_line(a, ckf, S, torch, "C5", 60000, a.c5_events, a.c5_train_events,
+                         a.pruned_steps, fit=True)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -214,6 +409,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "mlp_pruned": pruned,
+        "c5_pruned_fit": c5,
         "efficiency": round(perf.efficiency, 5),
         "found_tracks_per_s": round(len(result) * a.steps / elapsed, 1),
         "gpu_over_cpu": round(value / cpu["value"], 2) if cpu else None,

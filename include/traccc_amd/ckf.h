@@ -226,6 +226,7 @@ typedef struct ckf_run_stats {
     uint64_t n_tips;         /* tips before the length filter                  */
     uint32_t n_regrows;      /* buffer regrow reruns (traccc: link buffer resize) */
     uint32_t reserved;
+    uint64_t n_scored;       /* candidates scored by the branch scorer (0 without) */
 } ckf_run_stats;
 
 /* ---- library ------------------------------------------------------------ */

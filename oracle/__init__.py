@@ -44,6 +44,31 @@ class OracleTracks(C.Structure):
     ]
 
 
+class OracleFitted(C.Structure):
+    _d = C.POINTER(C.c_double)
+    _fields_ = [
+        ("n_tracks", C.c_uint32),
+        ("n_states", C.c_uint32),
+        ("outcome", F.c_u16_p),
+        ("params_surface", F.c_u32_p),
+        ("params_vector", _d),
+        ("params_covariance", _d),
+        ("ndf", _d),
+        ("chi2", _d),
+        ("pval", _d),
+        ("nholes", F.c_u32_p),
+        ("state_offsets", F.c_u32_p),
+        ("state_flags", F.c_u8_p),
+        ("filtered_chi2", _d),
+        ("smoothed_chi2", _d),
+        ("backward_chi2", _d),
+        ("filtered_vector", _d),
+        ("filtered_covariance", _d),
+        ("smoothed_vector", _d),
+        ("smoothed_covariance", _d),
+    ]
+
+
 _lib = None
 
 
@@ -75,6 +100,16 @@ def lib() -> C.CDLL:
                                        C.POINTER(F.BField), C.c_uint32, dptr, dptr, C.c_int,
                                        F.c_u32_p, dptr, dptr, dptr, F.c_u32_p]
         L.oracle_propagate.restype = C.c_int
+        L.oracle_fit_run.argtypes = [C.POINTER(F.FittingConfig), C.POINTER(F.Detector),
+                                     C.POINTER(F.BField), C.POINTER(F.Measurements), C.c_uint32,
+                                     F.c_u64_p, F.c_float_p, F.c_float_p, F.c_u32_p, F.c_u32_p,
+                                     C.c_int, C.c_uint32, C.POINTER(OracleFitted), dptr]
+        L.oracle_fit_run.restype = C.c_int
+        L.oracle_fitted_free.argtypes = [C.POINTER(OracleFitted)]
+        L.oracle_propagate_direct.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
+                                              C.POINTER(F.BField), C.c_uint32, C.c_uint32, dptr,
+                                              dptr, dptr, dptr]
+        L.oracle_propagate_direct.restype = C.c_int
         L.oracle_rk_step.argtypes = [C.POINTER(F.BField), C.c_double, C.c_double, C.c_double,
                                      dptr, dptr, dptr]
         _lib = L
@@ -191,6 +226,20 @@ def propagate(cfg, det, fld, start: int, vec, cov, apply_start_material=False):
     return st, es.value, ov, oc.reshape(6, 6), path.value, ns.value
 
 
+def propagate_direct(cfg, det, fld, start: int, target: int, vec, cov):
+    """Returns (status, vec, cov) in FP64 on surface `target`."""
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    c = np.ascontiguousarray(cov, dtype=np.float64).reshape(36)
+    ov = np.zeros(6)
+    oc = np.zeros(36)
+    st = L.oracle_propagate_direct(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), start,
+                                   target, v.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                                   ov.ctypes.data_as(dp), oc.ctypes.data_as(dp))
+    return st, ov, oc.reshape(6, 6)
+
+
 def rk_step(fld, mass, abs_q, h, free_in):
     L = lib()
     dp = C.POINTER(C.c_double)
@@ -200,3 +249,76 @@ def rk_step(fld, mass, abs_q, h, free_in):
     L.oracle_rk_step(C.byref(fld.struct), mass, abs_q, h, fi.ctypes.data_as(dp),
                      fo.ctypes.data_as(dp), D.ctypes.data_as(dp))
     return fo, D.reshape(8, 8)
+
+
+@dataclass
+class Fitted:
+    """Fitted tracks and their states (traccc track_collection +
+    track_state_collection fields), FP64."""
+    outcome: np.ndarray
+    params_surface: np.ndarray
+    params_vector: np.ndarray
+    params_covariance: np.ndarray
+    ndf: np.ndarray
+    chi2: np.ndarray
+    pval: np.ndarray
+    nholes: np.ndarray
+    state_offsets: np.ndarray
+    state_flags: np.ndarray
+    filtered_chi2: np.ndarray
+    smoothed_chi2: np.ndarray
+    backward_chi2: np.ndarray
+    filtered_vector: np.ndarray
+    filtered_covariance: np.ndarray
+    smoothed_vector: np.ndarray
+    smoothed_covariance: np.ndarray
+    seconds: float
+
+    def __len__(self):
+        return len(self.outcome)
+
+
+def fit(fcfg, det, fld, event, seed_surface_link, seed_vector, seed_covariance, offsets,
+        meas_index, precision: int = 64, threads: int = 1) -> Fitted:
+    """Kalman fit + two-filter smoother of candidate tracks of one event."""
+    L = lib()
+    m = event.measurements_struct()
+    n = len(offsets) - 1
+    ssl = np.ascontiguousarray(seed_surface_link, np.uint64)
+    sv = np.ascontiguousarray(seed_vector, np.float32).reshape(-1)
+    sc = np.ascontiguousarray(seed_covariance, np.float32).reshape(-1)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    idx = np.ascontiguousarray(meas_index, np.uint32)
+    if len(idx) == 0:
+        idx = np.zeros(1, np.uint32)
+    out = OracleFitted()
+    sec = C.c_double()
+    rc = L.oracle_fit_run(C.byref(fcfg), C.byref(det.struct), C.byref(fld.struct), C.byref(m), n,
+                          F.ptr(ssl, C.c_uint64), F.ptr(sv, C.c_float), F.ptr(sc, C.c_float),
+                          F.ptr(off, C.c_uint32), F.ptr(idx, C.c_uint32), precision, threads,
+                          C.byref(out), C.byref(sec))
+    if rc != 0:
+        raise ValueError(f"oracle_fit_run failed ({rc})")
+    try:
+        ns = out.n_states
+        d = np.float64
+        r = Fitted(
+            outcome=F.np_from(out.outcome, n, np.uint16),
+            params_surface=F.np_from(out.params_surface, n, np.uint32),
+            params_vector=F.np_from(out.params_vector, 6 * n, d, (n, 6)),
+            params_covariance=F.np_from(out.params_covariance, 36 * n, d, (n, 6, 6)),
+            ndf=F.np_from(out.ndf, n, d), chi2=F.np_from(out.chi2, n, d),
+            pval=F.np_from(out.pval, n, d), nholes=F.np_from(out.nholes, n, np.uint32),
+            state_offsets=F.np_from(out.state_offsets, n + 1, np.uint32),
+            state_flags=F.np_from(out.state_flags, ns, np.uint8),
+            filtered_chi2=F.np_from(out.filtered_chi2, ns, d),
+            smoothed_chi2=F.np_from(out.smoothed_chi2, ns, d),
+            backward_chi2=F.np_from(out.backward_chi2, ns, d),
+            filtered_vector=F.np_from(out.filtered_vector, 6 * ns, d, (ns, 6)),
+            filtered_covariance=F.np_from(out.filtered_covariance, 36 * ns, d, (ns, 6, 6)),
+            smoothed_vector=F.np_from(out.smoothed_vector, 6 * ns, d, (ns, 6)),
+            smoothed_covariance=F.np_from(out.smoothed_covariance, 36 * ns, d, (ns, 6, 6)),
+            seconds=sec.value)
+    finally:
+        L.oracle_fitted_free(C.byref(out))
+    return r

@@ -14,6 +14,7 @@
 // unspecified; the traccc device kernel uses the same lower-index rule,
 // device/common/include/traccc/finding/device/impl/find_tracks.ipp:474-477).
 #include "oracle.h"
+#include "traccc_amd/fitting.h"
 
 #include <algorithm>
 #include <atomic>
@@ -578,15 +579,9 @@ void add_material(const ckf_surface& sf, bound_state<S>& st, S mass, S abs_q) {
     st.cov.a[3][3] += t0 * t0;
 }
 
-// Propagate a bound state to the next sensitive surface.
-// 0 found (out filled), 1 exit, 2 abort.
+// Free state of a propagation engine from bound parameters on surface s0.
 template <typename S>
-int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, const ckf_bfield& field,
-                    const bound_state<S>& in, S mass, S abs_q, bound_state<S>& out, S* path_out,
-                    uint32_t* steps_out) {
-    const ckf_surface& s0 = det.surfaces[in.surface];
-    propagation<S> P{cfg, det, field, true, mass, abs_q, mat<S, 8, 1>::zero(), mat<S, 8, 8>::identity(),
-                     S(0), 0u};
+void load_free(propagation<S>& P, const ckf_surface& s0, const bound_state<S>& in) {
     const S l0 = in.vec.a[0][0], l1 = in.vec.a[1][0], phi = in.vec.a[2][0], th = in.vec.a[3][0];
     for (int i = 0; i < 3; ++i) {
         P.F.a[i][0] = static_cast<S>(s0.center[i]) + l0 * static_cast<S>(s0.axis_u[i]) +
@@ -598,13 +593,14 @@ int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, cons
     P.F.a[6][0] = std::cos(th);
     P.F.a[7][0] = in.vec.a[4][0];
     P.D = mat<S, 8, 8>::identity();
-    uint32_t found = 0u;
-    const int r = P.run(static_cast<int>(in.surface), found);
-    if (path_out) *path_out = P.path;
-    if (steps_out) *steps_out = P.steps;
-    if (r != 0) return r;
+}
 
-    const ckf_surface& s1 = det.surfaces[found];
+// Bound parameters on surface s1 (index `found`) from the engine's free state,
+// with the covariance carried through d(bound_1)/d(bound_0).
+template <typename S>
+void finish_bound(const propagation<S>& P, const ckf_bfield& field, const ckf_surface& s0,
+                  const ckf_surface& s1, uint32_t found, const bound_state<S>& in, S mass, S abs_q,
+                  bound_state<S>& out) {
     const v3<S> rel = P.pos() - fl3<S>(s1.center);
     const v3<S> T = P.dir();
     out.surface = found;
@@ -648,6 +644,39 @@ int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, cons
 
     const mat<S, 6, 6> J = Jfb * corr * P.D * bound_to_free_jac<S>(s0, in.vec);
     out.cov = J * in.cov * transpose(J);
+}
+
+// Propagate a bound state to the next sensitive surface.
+// 0 found (out filled), 1 exit, 2 abort.
+template <typename S>
+int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, const ckf_bfield& field,
+                    const bound_state<S>& in, S mass, S abs_q, bound_state<S>& out, S* path_out,
+                    uint32_t* steps_out) {
+    const ckf_surface& s0 = det.surfaces[in.surface];
+    propagation<S> P{cfg, det, field, true, mass, abs_q, mat<S, 8, 1>::zero(), mat<S, 8, 8>::identity(),
+                     S(0), 0u};
+    load_free<S>(P, s0, in);
+    uint32_t found = 0u;
+    const int r = P.run(static_cast<int>(in.surface), found);
+    if (path_out) *path_out = P.path;
+    if (steps_out) *steps_out = P.steps;
+    if (r != 0) return r;
+    finish_bound<S>(P, field, s0, det.surfaces[found], found, in, mass, abs_q, out);
+    return 0;
+}
+
+// Propagate a bound state straight onto surface `target`, forwards or
+// backwards along the track (the direct navigation of a fit's surface
+// sequence).  0 on the surface (out filled), 2 abort.
+template <typename S>
+int propagate_direct(const ckf_finding_config& cfg, const ckf_detector& det, const ckf_bfield& field,
+                     const bound_state<S>& in, uint32_t target, S mass, S abs_q, bound_state<S>& out) {
+    const ckf_surface& s0 = det.surfaces[in.surface];
+    propagation<S> P{cfg, det, field, true, mass, abs_q, mat<S, 8, 1>::zero(), mat<S, 8, 8>::identity(),
+                     S(0), 0u};
+    load_free<S>(P, s0, in);
+    if (P.onto_plane(det.surfaces[target]) != 0) return 2;
+    finish_bound<S>(P, field, s0, det.surfaces[target], target, in, mass, abs_q, out);
     return 0;
 }
 
@@ -1298,6 +1327,314 @@ std::vector<track_out> pkf(const ckf_finding_config& cfg, const ckf_detector& de
     return result;
 }
 
+// ---------------------------------------------------------------------------
+// Kalman fit with the two-filter smoother (kalman_fitting_algorithm)
+//   kalman_fitter::fit / fit_iteration / filter / smooth
+//                      core/include/traccc/fitting/kalman_filter/kalman_fitter.hpp:177-420
+//   update_statistics / check_fitting_result                        :419-550
+//   kalman_actor       core/include/traccc/fitting/kalman_filter/kalman_actor.hpp (forward
+//                      gain update, hole counting, surface sequence, backward smoothing)
+//   two_filters_smoother core/include/traccc/fitting/kalman_filter/two_filters_smoother.hpp:38-294
+//   inflate_covariance core/include/traccc/edm/track_parameters.hpp:93-106
+// Forward: the covariance-inflated seed goes through every sensitive surface
+// the navigation reaches; the material of a surface comes first, then the
+// gain update with the candidate's measurement of that surface; any other
+// sensitive surface is a hole (kept in the sequence when it has material).
+// Backward: from the last state (filtered, inflated) along the recorded
+// sequence with direct transport; on each state the smoother (information
+// form, as the reference), then the backward filter, then the material.
+// Deliberate deviation, applied identically in the product: before the two
+// estimates are combined, the backward prediction's phi is moved by 2 pi to
+// within pi of the forward one (the reference combines the raw angles, which
+// is wrong for tracks at phi ~ +-pi).
+// A navigation that leaves the detector before the last state leaves states
+// unfiltered; the reference's smoother then fails on the unfiltered state
+// (FAILURE_SMOOTHER), which is the outcome given here directly.
+// ---------------------------------------------------------------------------
+template <typename S, int N>
+bool invert(const mat<S, N, N>& in, mat<S, N, N>& out) {
+    mat<S, N, N> a = in;
+    out = mat<S, N, N>::identity();
+    for (int c = 0; c < N; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < N; ++r)
+            if (std::fabs(a.a[r][c]) > std::fabs(a.a[piv][c])) piv = r;
+        if (!(std::fabs(a.a[piv][c]) > S(0))) return false;
+        if (piv != c) {
+            for (int k = 0; k < N; ++k) {
+                std::swap(a.a[c][k], a.a[piv][k]);
+                std::swap(out.a[c][k], out.a[piv][k]);
+            }
+        }
+        const S d = a.a[c][c];
+        for (int k = 0; k < N; ++k) {
+            a.a[c][k] /= d;
+            out.a[c][k] /= d;
+        }
+        for (int r = 0; r < N; ++r) {
+            if (r == c) continue;
+            const S f = a.a[r][c];
+            if (f == S(0)) continue;
+            for (int k = 0; k < N; ++k) {
+                a.a[r][k] -= f * a.a[c][k];
+                out.a[r][k] -= f * out.a[c][k];
+            }
+        }
+    }
+    return true;
+}
+
+template <typename S>
+void inflate(mat<S, 6, 6>& c, S f) {
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) c.a[i][j] = (i == j) ? c.a[i][i] * f : S(0);
+}
+
+template <typename S>
+bool regularize(mat<S, 6, 6>& c) {
+    for (int i = 0; i < 6; ++i)
+        if (c.a[i][i] < S(-0.01)) return false;
+    for (int i = 0; i < 6; ++i) c.a[i][i] = std::fabs(c.a[i][i]);
+    return true;
+}
+
+// kalman_fitter_status values missing from ckf_fitter_status
+enum : int {
+    kFitSmootherChi2Negative = 4,
+    kFitSmootherChi2NotFinite = 5,
+    kFitSmootherInvalidCov = 6,
+    kFitSmootherSkipped = 7,
+    kFitUpdaterSkipped = 9,
+    kFitSequenceOverflow = 10,
+    kFitPropagationFailure = 11
+};
+
+template <typename S>
+struct fit_state {
+    uint8_t flags = 0u;
+    S filtered_chi2 = S(0), smoothed_chi2 = S(0), backward_chi2 = S(0);
+    bool filtered_ok = false;
+    bound_state<S> filtered{}, smoothed{};
+    uint32_t surface = 0u;
+};
+
+template <typename S>
+struct fit_result {
+    uint16_t outcome = CKF_OUTCOME_UNKNOWN;
+    bound_state<S> params{};
+    S ndf = S(0), chi2 = S(0), pval = S(0);
+    uint32_t nholes = 0u;
+    std::vector<fit_state<S>> states;
+};
+
+// two_filters_smoother::operator() on state st with the backward prediction
+// b; b becomes the backward filtered state.
+template <typename S>
+int smooth_state(fit_state<S>& st, const meas_view<S>& m, bound_state<S>& b) {
+    const mat<S, 6, 6>& Cf = st.filtered.cov;
+    mat<S, 6, 1> xp = b.vec;
+    const S twopi = static_cast<S>(2.0 * kPi);
+    const S dphi = xp.a[2][0] - st.filtered.vec.a[2][0];
+    if (dphi > static_cast<S>(kPi)) xp.a[2][0] -= twopi;
+    if (dphi < -static_cast<S>(kPi)) xp.a[2][0] += twopi;
+    mat<S, 6, 6> Cp_inv, Cf_inv, Cs;
+    if (!invert<S, 6>(b.cov, Cp_inv) || !invert<S, 6>(Cf, Cf_inv)) return CKF_FIT_ERROR_INVERSION;
+    if (!invert<S, 6>(Cp_inv + Cf_inv, Cs)) return CKF_FIT_ERROR_INVERSION;
+    if (!regularize(Cs)) return kFitSmootherInvalidCov;
+    mat<S, 6, 1> xs = Cs * (Cf_inv * st.filtered.vec + Cp_inv * xp);
+    if (!std::isfinite(xs.a[3][0]) || !std::isfinite(xs.a[2][0])) return CKF_FIT_ERROR_INVERSION;
+    if (std::fabs(xs.a[4][0]) == S(0)) return CKF_FIT_ERROR_QOP_ZERO;
+    if (!wrap_angles(xs)) return CKF_FIT_ERROR_THETA_POLE;
+    const mat<S, 2, 6> H = projector(m);
+    mat<S, 2, 2> V = mat<S, 2, 2>::zero();
+    V.a[0][0] = m.var[0];
+    V.a[1][1] = m.var[1];
+    mat<S, 2, 1> ml;
+    ml.a[0][0] = m.pos[0];
+    ml.a[1][0] = m.dims == 2u ? m.pos[1] : S(0);
+    const mat<S, 2, 1> rs = ml - H * xs;
+    const mat<S, 2, 2> Rs = V - H * (Cs * transpose(H));
+    const S chi2_s = (transpose(rs) * masked_inv(Rs, m.dims) * rs).a[0][0];
+    if (chi2_s < S(-10)) return kFitSmootherChi2Negative;
+    if (!std::isfinite(chi2_s)) return kFitSmootherChi2NotFinite;
+    // backward filter (the reference's inline gain update)
+    const mat<S, 6, 2> PHt = b.cov * transpose(H);
+    const mat<S, 6, 2> K = PHt * masked_inv(H * PHt + V, m.dims);
+    mat<S, 6, 1> xb = b.vec + K * (ml - H * b.vec);
+    if (!std::isfinite(xb.a[3][0]) || !std::isfinite(xb.a[2][0])) return CKF_FIT_ERROR_INVERSION;
+    if (std::fabs(xb.a[4][0]) == S(0)) return CKF_FIT_ERROR_QOP_ZERO;
+    if (!wrap_angles(xb)) return CKF_FIT_ERROR_THETA_POLE;
+    const mat<S, 6, 6> IKH = mat<S, 6, 6>::identity() - K * H;
+    mat<S, 6, 6> Cb = IKH * b.cov * transpose(IKH) + K * V * transpose(K);
+    if (!regularize(Cb)) return kFitSmootherInvalidCov;
+    const mat<S, 2, 1> r = ml - H * xb;
+    const mat<S, 2, 2> R = (mat<S, 2, 2>::identity() - H * K) * V;
+    const S chi2_b = (transpose(r) * masked_inv(R, m.dims) * r).a[0][0];
+    if (chi2_b < S(0)) return kFitSmootherChi2Negative;
+    if (!std::isfinite(chi2_b)) return kFitSmootherChi2NotFinite;
+    st.smoothed.surface = st.surface;
+    st.smoothed.vec = xs;
+    st.smoothed.cov = Cs;
+    st.smoothed_chi2 = chi2_s;
+    st.backward_chi2 = chi2_b;
+    st.flags |= CKF_STATE_SMOOTHED;
+    b.vec = xb;
+    b.cov = Cb;
+    return CKF_FIT_SUCCESS;
+}
+
+template <typename S>
+fit_result<S> kalman_fit(const ckf_fitting_config& fc, const ckf_detector& det, const ckf_bfield& field,
+                         const ckf_measurements& meas, const bound_state<S>& seed,
+                         const uint32_t* idx, uint32_t n, const std::vector<uint32_t>& meas_sf) {
+    // navigation settings of the fit (no minimum step: no ckf_aborter)
+    ckf_finding_config nc{};
+    nc.propagation = fc.propagation;
+    nc.min_step_length_for_next_surface = 0.f;
+    nc.max_step_counts_for_next_surface = fc.max_step_counts;
+    nc.min_p = fc.min_p;
+    nc.min_pT = fc.min_pT;
+    nc.ptc_mass = fc.ptc_mass;
+    nc.ptc_charge = fc.ptc_charge;
+    const S mass = static_cast<S>(fc.ptc_mass);
+    const S abs_q = static_cast<S>(std::fabs(fc.ptc_charge));
+    const S infl = static_cast<S>(fc.covariance_inflation_factor);
+    const size_t cap = std::max<size_t>(size_t(n) * fc.surface_sequence_size_factor,
+                                        fc.min_surface_sequence_capacity);
+    fit_result<S> R;
+    R.states.resize(n);
+    for (uint32_t k = 0; k < n; ++k) R.states[k].surface = meas_sf[idx[k]];
+    bound_state<S> start = seed;
+    for (uint32_t it = 0; it < std::max(fc.n_iterations, 1u); ++it) {
+        for (auto& st : R.states) {
+            st.flags = 0u;
+            st.filtered_ok = false;
+        }
+        // ---- forward filter
+        bound_state<S> b = start;
+        inflate(b.cov, infl);
+        uint32_t k = 0u, holes = 0u;
+        std::vector<int64_t> seq;  // >= 0: state index; < 0: -(surface + 1) of a hole
+        int fw = CKF_FIT_SUCCESS;
+        bool complete = false;
+        for (;;) {
+            add_material<S>(det.surfaces[b.surface], b, mass, abs_q);
+            if (b.surface == R.states[k].surface) {
+                const meas_view<S> mv = load_meas<S>(meas, idx[k]);
+                bound_state<S> f;
+                const int r = gain_update<S>(mv, b, f);
+                if (r != CKF_FIT_SUCCESS) {
+                    fw = r;
+                    break;
+                }
+                R.states[k].filtered = f;
+                R.states[k].filtered_ok = true;
+                R.states[k].filtered_chi2 = predicted_chi2<S>(mv, f);
+                b = f;
+                if (seq.size() == cap) {
+                    fw = kFitSequenceOverflow;
+                    break;
+                }
+                seq.push_back(k);
+                if (++k == n) {
+                    complete = true;
+                    break;
+                }
+            } else {
+                bool later = false;
+                for (uint32_t j = k + 1; j < n; ++j) later = later || R.states[j].surface == b.surface;
+                if (later) {
+                    fw = kFitUpdaterSkipped;
+                    break;
+                }
+                ++holes;
+                if (det.surfaces[b.surface].thickness_x0 > 0.f) {
+                    if (seq.size() == cap) {
+                        fw = kFitSequenceOverflow;
+                        break;
+                    }
+                    seq.push_back(-static_cast<int64_t>(b.surface) - 1);
+                }
+            }
+            bound_state<S> o;
+            const int r = propagate_bound<S>(nc, det, field, b, mass, abs_q, o, nullptr, nullptr);
+            if (r == 1) break;  // left the detector: states left unfiltered
+            if (r != 0) {
+                fw = kFitPropagationFailure;
+                break;
+            }
+            b = o;
+        }
+        R.nholes = holes;
+        if (fw != CKF_FIT_SUCCESS) {
+            R.outcome = fw == kFitUpdaterSkipped       ? CKF_OUTCOME_FAILURE_NOT_ALL_FITTED
+                        : fw == kFitPropagationFailure ? CKF_OUTCOME_FAILURE_FORWARD_PROPAGATION
+                        : fw == kFitSequenceOverflow   ? CKF_OUTCOME_FAILURE_SMOOTHER
+                                                       : CKF_OUTCOME_FAILURE_FITTER;
+            return R;
+        }
+        if (!complete) {
+            R.outcome = CKF_OUTCOME_FAILURE_SMOOTHER;
+            return R;
+        }
+        // ---- backward smoother
+        fit_state<S>& last = R.states[n - 1];
+        last.smoothed = last.filtered;
+        bound_state<S> bb = last.filtered;
+        inflate(bb.cov, infl);
+        int bw = CKF_FIT_SUCCESS;
+        for (size_t e = seq.size(); e-- > 0;) {
+            const uint32_t target = seq[e] >= 0 ? R.states[seq[e]].surface
+                                                : static_cast<uint32_t>(-seq[e] - 1);
+            if (target != bb.surface) {
+                bound_state<S> o;
+                if (propagate_direct<S>(nc, det, field, bb, target, mass, abs_q, o) != 0) {
+                    bw = kFitPropagationFailure;
+                    break;
+                }
+                bb = o;
+            }
+            if (seq[e] >= 0) {
+                const uint32_t j = static_cast<uint32_t>(seq[e]);
+                const int r = smooth_state<S>(R.states[j], load_meas<S>(meas, idx[j]), bb);
+                if (r != CKF_FIT_SUCCESS) {
+                    bw = r;
+                    break;
+                }
+            }
+            add_material<S>(det.surfaces[target], bb, mass, abs_q);
+        }
+        if (bw != CKF_FIT_SUCCESS) {
+            R.outcome = bw == kFitSmootherSkipped      ? CKF_OUTCOME_FAILURE_NOT_ALL_SMOOTHED
+                        : bw == kFitPropagationFailure ? CKF_OUTCOME_FAILURE_BACKWARD_PROPAGATION
+                                                       : CKF_OUTCOME_FAILURE_SMOOTHER;
+            return R;
+        }
+        // ---- statistics and outcome
+        S ndf = S(0), chi2 = S(0);
+        bool have_params = false;
+        for (uint32_t j = 0; j < n; ++j) {
+            const fit_state<S>& st = R.states[j];
+            if (st.flags & CKF_STATE_SMOOTHED) {
+                if (!have_params) {
+                    R.params = st.smoothed;
+                    have_params = true;
+                }
+                ndf += static_cast<S>(meas.dimensions[idx[j]]);
+                chi2 += st.backward_chi2;
+            }
+        }
+        ndf -= S(5);
+        R.ndf = ndf;
+        R.chi2 = chi2;
+        R.pval = prob<S>(chi2, ndf);
+        R.outcome = ndf > S(0) ? CKF_OUTCOME_SUCCESS : CKF_OUTCOME_FAILURE_NON_POSITIVE_NDF;
+        // the next iteration starts from the first state's filtered parameters
+        start = R.states[0].filtered;
+    }
+    return R;
+}
+
 template <typename T>
 T* alloc_n(size_t n) {
     return static_cast<T*>(std::calloc(n ? n : 1, sizeof(T)));
@@ -1414,6 +1751,109 @@ int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,
     return CKF_OK;
 }
 
+void oracle_fitted_free(oracle_fitted* f) {
+    if (!f) return;
+    void* ps[] = {f->outcome, f->params_surface, f->params_vector, f->params_covariance, f->ndf,
+                  f->chi2, f->pval, f->nholes, f->state_offsets, f->state_flags, f->filtered_chi2,
+                  f->smoothed_chi2, f->backward_chi2, f->filtered_vector, f->filtered_covariance,
+                  f->smoothed_vector, f->smoothed_covariance};
+    for (void* p : ps) std::free(p);
+    std::memset(f, 0, sizeof(*f));
+}
+
+int oracle_fit_run(const ckf_fitting_config* cfg, const ckf_detector* det, const ckf_bfield* field,
+                   const ckf_measurements* meas, uint32_t n_tracks, const uint64_t* seed_surface_link,
+                   const float* seed_vec, const float* seed_cov, const uint32_t* offsets,
+                   const uint32_t* meas_index, int precision, uint32_t n_threads, oracle_fitted* out,
+                   double* seconds) {
+    using namespace oracle;
+    if (!cfg || !det || !field || !meas || !out || n_threads == 0u ||
+        (n_tracks > 0u && (!seed_surface_link || !seed_vec || !seed_cov || !offsets || !meas_index))) {
+        return CKF_ERR_INVALID_ARGUMENT;
+    }
+    std::memset(out, 0, sizeof(*out));
+    // surface index of every measurement
+    std::vector<uint32_t> meas_sf(meas->size, 0u);
+    for (uint32_t i = 0; i < meas->size; ++i) meas_sf[i] = surface_index(*det, meas->surface_link[i]);
+    const uint32_t ns = n_tracks ? offsets[n_tracks] : 0u;
+    out->n_tracks = n_tracks;
+    out->n_states = ns;
+    out->outcome = alloc_n<uint16_t>(n_tracks);
+    out->params_surface = alloc_n<uint32_t>(n_tracks);
+    out->params_vector = alloc_n<double>(6u * n_tracks);
+    out->params_covariance = alloc_n<double>(36u * n_tracks);
+    out->ndf = alloc_n<double>(n_tracks);
+    out->chi2 = alloc_n<double>(n_tracks);
+    out->pval = alloc_n<double>(n_tracks);
+    out->nholes = alloc_n<uint32_t>(n_tracks);
+    out->state_offsets = alloc_n<uint32_t>(n_tracks + 1u);
+    out->state_flags = alloc_n<uint8_t>(ns);
+    out->filtered_chi2 = alloc_n<double>(ns);
+    out->smoothed_chi2 = alloc_n<double>(ns);
+    out->backward_chi2 = alloc_n<double>(ns);
+    out->filtered_vector = alloc_n<double>(6u * size_t(ns));
+    out->filtered_covariance = alloc_n<double>(36u * size_t(ns));
+    out->smoothed_vector = alloc_n<double>(6u * size_t(ns));
+    out->smoothed_covariance = alloc_n<double>(36u * size_t(ns));
+    for (uint32_t t = 0; t <= n_tracks && n_tracks; ++t) out->state_offsets[t] = offsets[t];
+    auto fit_one = [&](uint32_t t, auto tag) {
+        using S = decltype(tag);
+        bound_state<S> b;
+        b.surface = surface_index(*det, seed_surface_link[t]);
+        for (int k = 0; k < 6; ++k) b.vec.a[k][0] = static_cast<S>(seed_vec[6u * t + k]);
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) b.cov.a[r][c] = static_cast<S>(seed_cov[36u * t + 6 * r + c]);
+        const uint32_t s0 = offsets[t], n = offsets[t + 1] - offsets[t];
+        const fit_result<S> R = kalman_fit<S>(*cfg, *det, *field, *meas, b, meas_index + s0, n, meas_sf);
+        out->outcome[t] = R.outcome;
+        out->params_surface[t] = R.params.surface;
+        for (int k = 0; k < 6; ++k) out->params_vector[6u * t + k] = static_cast<double>(R.params.vec.a[k][0]);
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+                out->params_covariance[36u * t + 6 * r + c] = static_cast<double>(R.params.cov.a[r][c]);
+        out->ndf[t] = static_cast<double>(R.ndf);
+        out->chi2[t] = static_cast<double>(R.chi2);
+        out->pval[t] = static_cast<double>(R.pval);
+        out->nholes[t] = R.nholes;
+        for (uint32_t k = 0; k < n; ++k) {
+            const fit_state<S>& st = R.states[k];
+            const size_t g = s0 + k;
+            out->state_flags[g] = st.flags;
+            out->filtered_chi2[g] = static_cast<double>(st.filtered_chi2);
+            out->smoothed_chi2[g] = static_cast<double>(st.smoothed_chi2);
+            out->backward_chi2[g] = static_cast<double>(st.backward_chi2);
+            for (int q = 0; q < 6; ++q) {
+                out->filtered_vector[6u * g + q] = static_cast<double>(st.filtered.vec.a[q][0]);
+                out->smoothed_vector[6u * g + q] = static_cast<double>(st.smoothed.vec.a[q][0]);
+            }
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    out->filtered_covariance[36u * g + 6 * r + c] = static_cast<double>(st.filtered.cov.a[r][c]);
+                    out->smoothed_covariance[36u * g + 6 * r + c] = static_cast<double>(st.smoothed.cov.a[r][c]);
+                }
+        }
+    };
+    std::atomic<uint32_t> next{0u};
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (uint32_t th = 0; th < n_threads; ++th) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                const uint32_t t0i = next.fetch_add(64u);
+                if (t0i >= n_tracks) break;
+                const uint32_t t1 = std::min(n_tracks, t0i + 64u);
+                for (uint32_t t = t0i; t < t1; ++t) {
+                    if (precision == 32) fit_one(t, 0.f);
+                    else fit_one(t, 0.0);
+                }
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return CKF_OK;
+}
+
 double oracle_prob(double chi2, double ndf) { return oracle::prob<double>(chi2, ndf); }
 float oracle_prob_f(float chi2, float ndf) { return oracle::prob<float>(chi2, ndf); }
 
@@ -1460,6 +1900,26 @@ int oracle_propagate(const ckf_finding_config* cfg, const ckf_detector* det, con
     if (r == 0) {
         add_material<double>(det->surfaces[o.surface], o, mass, aq);
         *end_surface = o.surface;
+        for (int i = 0; i < 6; ++i) out_vec[i] = o.vec.a[i][0];
+        for (int rr = 0; rr < 6; ++rr)
+            for (int c = 0; c < 6; ++c) out_cov[6 * rr + c] = o.cov.a[rr][c];
+    }
+    return r;
+}
+
+int oracle_propagate_direct(const ckf_finding_config* cfg, const ckf_detector* det,
+                            const ckf_bfield* field, uint32_t start, uint32_t target,
+                            const double vec[6], const double cov[36], double out_vec[6],
+                            double out_cov[36]) {
+    using namespace oracle;
+    bound_state<double> in, o;
+    in.surface = start;
+    for (int i = 0; i < 6; ++i) in.vec.a[i][0] = vec[i];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) in.cov.a[r][c] = cov[6 * r + c];
+    const int r = propagate_direct<double>(*cfg, *det, *field, in, target, cfg->ptc_mass,
+                                           std::fabs(cfg->ptc_charge), o);
+    if (r == 0) {
         for (int i = 0; i < 6; ++i) out_vec[i] = o.vec.a[i][0];
         for (int rr = 0; rr < 6; ++rr)
             for (int c = 0; c < 6; ++c) out_cov[6 * rr + c] = o.cov.a[rr][c];

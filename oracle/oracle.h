@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "traccc_amd/ckf.h"
+#include "traccc_amd/fitting.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -54,6 +55,40 @@ int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,
                         const ckf_bound_params* seeds, uint32_t n_events, int precision,
                         uint32_t n_threads, uint64_t* n_tracks_total, double* seconds);
 
+/* ---- Kalman fit + two-filter smoother (kalman_fitting_algorithm) ---- */
+typedef struct oracle_fitted {
+    uint32_t n_tracks;
+    uint32_t n_states;
+    uint16_t* outcome;           /* ckf_fit_outcome */
+    uint32_t* params_surface;    /* surface index */
+    double* params_vector;       /* [n][6]  */
+    double* params_covariance;   /* [n][36] */
+    double* ndf;
+    double* chi2;
+    double* pval;
+    uint32_t* nholes;
+    uint32_t* state_offsets;     /* [n + 1] */
+    uint8_t* state_flags;
+    double* filtered_chi2;
+    double* smoothed_chi2;
+    double* backward_chi2;
+    double* filtered_vector;     /* [n_states][6]  */
+    double* filtered_covariance; /* [n_states][36] */
+    double* smoothed_vector;
+    double* smoothed_covariance;
+} oracle_fitted;
+
+/* Fit n_tracks candidates of one event: seed parameters per track and the
+ * candidate's measurement indices [offsets[t], offsets[t+1]) into `meas`.
+ * precision 64 (oracle) or 32 (timed CPU baseline); tracks spread over
+ * n_threads host threads; *seconds = wall time of the fits. */
+int oracle_fit_run(const ckf_fitting_config* cfg, const ckf_detector* det, const ckf_bfield* field,
+                   const ckf_measurements* meas, uint32_t n_tracks, const uint64_t* seed_surface_link,
+                   const float* seed_vec, const float* seed_cov, const uint32_t* offsets,
+                   const uint32_t* meas_index, int precision, uint32_t n_threads, oracle_fitted* out,
+                   double* seconds);
+void oracle_fitted_free(oracle_fitted* f);
+
 /* ---- unit-level entry points (FP64) ---- */
 double oracle_prob(double chi2, double ndf);
 float oracle_prob_f(float chi2, float ndf);
@@ -71,6 +106,14 @@ int oracle_propagate(const ckf_finding_config* cfg, const ckf_detector* det,
                      const ckf_bfield* field, uint32_t start, const double vec[6],
                      const double cov[36], int apply_start_material, uint32_t* end_surface,
                      double out_vec[6], double out_cov[36], double* path, uint32_t* n_steps);
+
+/* Transport a bound state from surface `start` straight onto surface
+ * `target` (either direction along the track; no navigation).  0 on success,
+ * 2 aborted. */
+int oracle_propagate_direct(const ckf_finding_config* cfg, const ckf_detector* det,
+                            const ckf_bfield* field, uint32_t start, uint32_t target,
+                            const double vec[6], const double cov[36], double out_vec[6],
+                            double out_cov[36]);
 
 /* One RK4 step of a free state [x, y, z, t, Tx, Ty, Tz, q/p] with the
  * transport matrix d(free_out)/d(free_in); for tests. */

@@ -62,6 +62,51 @@ class FindingConfig(C.Structure):
     ]
 
 
+class FittingConfig(C.Structure):
+    """include/traccc_amd/fitting.h ckf_fitting_config."""
+    _fields_ = [
+        ("n_iterations", C.c_uint32),
+        ("propagation", PropagationConfig),
+        ("max_step_counts", C.c_uint32),
+        ("min_p", C.c_float),
+        ("min_pT", C.c_float),
+        ("ptc_pdg", C.c_int32),
+        ("ptc_mass", C.c_float),
+        ("ptc_charge", C.c_float),
+        ("covariance_inflation_factor", C.c_float),
+        ("surface_sequence_size_factor", C.c_uint32),
+        ("min_surface_sequence_capacity", C.c_uint32),
+        ("do_precise_hole_count", C.c_uint32),
+    ]
+
+
+class FittedTracks(C.Structure):
+    """include/traccc_amd/fitting.h ckf_fitted_tracks."""
+    _fields_ = [
+        ("n_tracks", C.c_uint32),
+        ("n_states", C.c_uint32),
+        ("fit_outcome", c_u16_p),
+        ("params_surface_link", c_u64_p),
+        ("params_vector", c_float_p),
+        ("params_covariance", c_float_p),
+        ("ndf", c_float_p),
+        ("chi2", c_float_p),
+        ("pval", c_float_p),
+        ("nholes", c_u32_p),
+        ("state_offsets", c_u32_p),
+        ("state_flags", c_u8_p),
+        ("filtered_chi2", c_float_p),
+        ("smoothed_chi2", c_float_p),
+        ("backward_chi2", c_float_p),
+        ("state_surface_link", c_u64_p),
+        ("filtered_vector", c_float_p),
+        ("filtered_covariance", c_float_p),
+        ("smoothed_vector", c_float_p),
+        ("smoothed_covariance", c_float_p),
+        ("measurement_index", c_u32_p),
+    ]
+
+
 class Surface(C.Structure):
     _fields_ = [
         ("center", C.c_float * 3),
@@ -166,6 +211,7 @@ class RunStats(C.Structure):
         ("n_tips", C.c_uint64),
         ("n_regrows", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("n_scored", C.c_uint64),
     ]
 
 

@@ -29,6 +29,7 @@ HIP_SOURCES = [
     os.path.join(CSRC, "ckf", "votes.hip"),
     os.path.join(CSRC, "ckf", "pkf.hip"),
     os.path.join(CSRC, "scorer", "mlp.hip"),
+    os.path.join(CSRC, "fit", "fit.hip"),
     os.path.join(CSRC, "common", "config.cpp"),
 ]
 SIM_SOURCES = [

@@ -65,9 +65,10 @@ struct step_ctl {
     uint32_t n_constituents;
     uint32_t prev_link_base;  // global index of the previous step's first link
     uint32_t exit_count;      // waves of k_propagate that have finished this step
-    uint32_t pad2;
+    uint32_t n_rows;          // scorer rows listed this step (written by k_find_list)
     uint64_t n_links_total;
     uint64_t n_props;
+    uint64_t n_scored;        // scorer rows of the finished steps
 };
 
 struct ckf_params {
@@ -160,8 +161,9 @@ struct ckf_params {
     uint32_t* list_m;             // [in_cap][list_len] measurement
     float* list_c;                // [in_cap][list_len] chi2
     uint32_t* list_n;             // [in_cap] entries
-    float* list_feat;             // [in_cap][list_len][12]
-    uint8_t* cand_keep;           // [in_cap][list_len] scorer verdict
+    uint32_t* list_row;           // [in_cap] first feature / verdict row of the input
+    float* list_feat;             // [rows][12], rows packed over the step
+    uint8_t* cand_keep;           // [rows] scorer verdict
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
 };

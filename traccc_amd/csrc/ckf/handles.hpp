@@ -1,0 +1,154 @@
+// Handles of the C ABI shared by the CKF and the fitter runtimes: device
+// buffers, the per-call result block cache, and the device-resident
+// detector, events and track candidates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../common/errors.hpp"
+#include "traccc_amd/ckf.h"
+
+namespace tamd {
+
+// Device allocation that grows by doubling and is reused across calls.
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) {
+            return;
+        }
+        if (p != nullptr) {
+            tamd::hip_check(hipFree(p), "hipFree");
+            p = nullptr;
+        }
+        size_t nb = std::max<size_t>(need, bytes * 2);
+        nb = (nb + 255) & ~size_t(255);
+        tamd::hip_check(hipMalloc(&p, nb), "hipMalloc");
+        bytes = nb;
+    }
+    void release() {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Process-wide cache of device blocks for per-call results: freed blocks are
+// kept by power-of-two size class and handed out again, so a steady stream of
+// operator calls does not call hipMalloc / hipFree.
+struct block_cache {
+    std::mutex mu;
+    std::map<size_t, std::vector<void*>> free_blocks;
+    static size_t size_class(size_t bytes) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    void* get(size_t bytes) {
+        const size_t c = size_class(std::max<size_t>(bytes, 1));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = free_blocks.find(c);
+            if (it != free_blocks.end() && !it->second.empty()) {
+                void* p = it->second.back();
+                it->second.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        tamd::hip_check(hipMalloc(&p, c), "hipMalloc");
+        return p;
+    }
+    void put(void* p, size_t bytes) {
+        if (p == nullptr) return;
+        std::lock_guard<std::mutex> lk(mu);
+        free_blocks[size_class(std::max<size_t>(bytes, 1))].push_back(p);
+    }
+};
+
+block_cache& result_cache();
+
+template <typename T>
+T* dev_copy(const T* src, size_t n) {
+    T* d = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    tamd::hip_check(hipMalloc(&d, bytes), "hipMalloc");
+    if (n > 0 && src != nullptr) {
+        tamd::hip_check(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    return d;
+}
+
+}  // namespace tamd
+
+struct ckf_device_detector {
+    std::vector<ckf_surface> h_sfs;
+    std::vector<ckf_layer> h_layers;
+    std::vector<uint64_t> ids;
+    ckf_bfield field{};
+    ckf_surface* d_sfs = nullptr;
+    ckf_layer* d_layers = nullptr;
+    float* d_map = nullptr;
+    ~ckf_device_detector() {
+        if (d_sfs) (void)hipFree(d_sfs);
+        if (d_layers) (void)hipFree(d_layers);
+        if (d_map) (void)hipFree(d_map);
+    }
+};
+
+struct ckf_device_event {
+    uint32_t n_events = 0, n_meas = 0, n_seeds = 0, n_surfaces = 0;
+    std::vector<uint32_t> meas_off, seed_off;  // [n_events + 1]
+    float2* d_pos = nullptr;
+    float2* d_var = nullptr;
+    uint32_t* d_info = nullptr;
+    uint32_t* d_ranges = nullptr;
+    uint32_t* d_seed_event = nullptr;
+    uint32_t* d_seed_surface = nullptr;
+    float* d_seed_vec = nullptr;
+    float* d_seed_cov = nullptr;
+    uint32_t* d_meas_off = nullptr;
+    uint32_t* d_meas_sf = nullptr;  // surface index of every measurement
+    ~ckf_device_event() {
+        void* ps[] = {d_pos, d_var, d_info, d_ranges, d_seed_event, d_seed_surface,
+                      d_seed_vec, d_seed_cov, d_meas_off, d_meas_sf};
+        for (void* p : ps) {
+            if (p) (void)hipFree(p);
+        }
+    }
+};
+
+struct ckf_device_tracks {
+    uint32_t n_events = 0, n_tracks = 0, n_const = 0;
+    std::vector<uint32_t> ev_track_off, ev_const_off;
+    std::vector<uint64_t> ids;
+    uint16_t* outcome = nullptr;
+    uint32_t* surface = nullptr;
+    float* vec = nullptr;
+    float* cov = nullptr;
+    float* ndf = nullptr;
+    float* chi2 = nullptr;
+    float* pval = nullptr;
+    uint32_t* nholes = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* cons = nullptr;
+    size_t track_bytes[10] = {};
+    ~ckf_device_tracks() {
+        void* ps[] = {outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons};
+        for (int i = 0; i < 10; ++i) tamd::result_cache().put(ps[i], track_bytes[i]);
+    }
+};
+

@@ -145,9 +145,11 @@ __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i
 }
 
 // Scorer-pruned finding, first kernel: per input the list_len best candidates
-// in (chi2, index) order and their features.
+// in (chi2, index) order (fixed stride list_len), and their features in rows
+// packed densely over the step (a wave's inputs take one block of rows), so
+// the MLP runs on full 64-row tiles.
 __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
-    const step_ctl* ctl = P.ctl;
+    step_ctl* ctl = P.ctl;
     if (ctl->done) {
         return;
     }
@@ -163,12 +165,14 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
             break;  // uniform over the wave
         }
         const uint32_t i = i0 + team;
-        if (i < n_in) {
-            find_input in;
+        const bool active = i < n_in;
+        find_input in;
+        uint32_t n_pass = 0u, k = 0u;
+        if (active) {
             load_input(P, step, i, prev_base, in);
             bool have_last = false;
             float last_c = 0.f;
-            uint32_t last_m = 0u, n_pass = 0u, k = 0u;
+            uint32_t last_m = 0u;
             for (; k < Lc; ++k) {
                 float best_c;
                 uint32_t best_m;
@@ -184,14 +188,28 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                     const size_t r = static_cast<size_t>(i) * Lc + k;
                     P.list_m[r] = best_m;
                     P.list_c[r] = best_c;
-                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
-                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, best_c, k, n_pass,
-                                       P.list_feat + r * kFeatures);
                 }
             }
-            if (tl == 0u) {
-                P.list_n[i] = k;
+        }
+        // rows of this wave's inputs: one atomic per wave, leaders' prefix
+        const uint32_t cnt = (active && tl == 0u) ? k : 0u;
+        const uint32_t incl = wave_inclusive_scan(cnt);
+        const uint32_t total = wave_broadcast(incl, kWave - 1);
+        uint32_t base = 0u;
+        if (lane() == 0u && total != 0u) {
+            base = atomicAdd(&ctl->n_rows, total);
+        }
+        base = wave_broadcast(base, 0);
+        if (active && tl == 0u) {
+            const uint32_t row0 = base + incl - cnt;
+            P.list_n[i] = k;
+            P.list_row[i] = row0;
+            for (uint32_t q = 0u; q < k; ++q) {
+                const size_t r = static_cast<size_t>(i) * Lc + q;
+                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, P.list_m[r]);
+                candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
+                                   in.chi2_prev, in.ndf_prev, P.list_c[r], q, n_pass,
+                                   P.list_feat + static_cast<size_t>(row0 + q) * kFeatures);
             }
         }
     }
@@ -206,7 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
         return;
     }
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
-    const uint32_t rows = n_in * P.list_len;
+    const uint32_t cap = n_in * P.list_len;
+    const uint32_t rows = ctl->n_rows < cap ? ctl->n_rows : cap;
     const uint32_t wave = threadIdx.x / kWave;
     short* lds_x = lds + wave * kMlpLdsPerWave;
     short* lds_h = lds_x + 64 * kMlpInPad;
@@ -214,10 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
     for (uint32_t t = blockIdx.x * (blockDim.x / kWave) + wave; t < n_tiles;
          t += gridDim.x * (blockDim.x / kWave)) {
         const uint32_t r = t * 64u + lane();
-        bool valid = r < rows;
-        if (valid) {
-            valid = (r % P.list_len) < P.list_n[r / P.list_len];
-        }
+        const bool valid = r < rows;
         float f[kMlpIn];
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
@@ -281,12 +297,13 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             uint32_t last_m = 0u, n_pass = 0u;
             uint32_t li = 0u;
             const uint32_t ln = kScored ? P.list_n[i] : 0u;
+            const uint32_t row0 = kScored ? P.list_row[i] : 0u;
             while (count < N) {
                 float best_c = 0.f;
                 uint32_t best_m = kInvalid;
                 if constexpr (kScored) {
                     const size_t r0 = static_cast<size_t>(i) * P.list_len;
-                    while (li < ln && P.cand_keep[r0 + li] == 0u) ++li;
+                    while (li < ln && P.cand_keep[row0 + li] == 0u) ++li;
                     if (li < ln) {
                         best_m = P.list_m[r0 + li];
                         best_c = P.list_c[r0 + li];
@@ -683,6 +700,8 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->n_in = ctl->n_live;
     ctl->n_live = 0u;
     ctl->n_links = 0u;
+    ctl->n_scored += ctl->n_rows;
+    ctl->n_rows = 0u;
     ctl->step += 1u;
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
@@ -1028,10 +1047,16 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.scored) {
         hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_score, dim3(s.score_blocks), dim3(kBlock), 0, stream, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    if (P.scored) {
         hipLaunchKernelGGL(k_find<kFindScored>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     } else if (P.feat_slot != nullptr) {
         hipLaunchKernelGGL(k_find<kFindLogged>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
@@ -1039,6 +1064,14 @@ hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t s
         hipLaunchKernelGGL(k_find<kFindPlain>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    const hipError_t e = launch_score_lists(P, s, stream);
+    if (e != hipSuccess) {
+        return e;
+    }
+    return launch_find_update(P, s, stream);
 }
 
 // Branch log compaction: measurement links [0, n) in order, 1024 per block.

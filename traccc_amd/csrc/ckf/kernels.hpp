@@ -27,6 +27,10 @@ hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream
                            bool with_dedup);
 // the same step, kernel group by kernel group (timed runs)
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+// launch_find = the scorer's list + score kernels (scored runs only), then the
+// find / update kernel
+hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                             bool fused_advance);

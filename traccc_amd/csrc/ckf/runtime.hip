@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../common/errors.hpp"
+#include "handles.hpp"
 #include "kernels.hpp"
 #include "traccc_amd/ckf.h"
 #include "traccc_amd/ckf_debug.h"
@@ -31,6 +32,11 @@ std::string& last_error() {
     thread_local std::string s;
     return s;
 }
+
+block_cache& result_cache() {
+    static block_cache* c = new block_cache();  // lives until the process ends
+    return *c;
+}
 }  // namespace tamd
 
 namespace {
@@ -38,85 +44,9 @@ namespace {
 using tamd::ckf_error;
 using tamd::guarded;
 using tamd::hip_check;
-
-// Device allocation that grows by doubling and is reused across calls.
-struct dbuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t need) {
-        if (need <= bytes) {
-            return;
-        }
-        if (p != nullptr) {
-            hip_check(hipFree(p), "hipFree");
-            p = nullptr;
-        }
-        size_t nb = std::max<size_t>(need, bytes * 2);
-        nb = (nb + 255) & ~size_t(255);
-        hip_check(hipMalloc(&p, nb), "hipMalloc");
-        bytes = nb;
-    }
-    void release() {
-        if (p != nullptr) {
-            (void)hipFree(p);
-        }
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-// Process-wide cache of device blocks for per-call results: freed blocks are
-// kept by power-of-two size class and handed out again, so a steady stream of
-// operator calls does not call hipMalloc / hipFree.
-struct block_cache {
-    std::mutex mu;
-    std::map<size_t, std::vector<void*>> free_blocks;
-    static size_t size_class(size_t bytes) {
-        size_t c = 256;
-        while (c < bytes) c <<= 1;
-        return c;
-    }
-    void* get(size_t bytes) {
-        const size_t c = size_class(std::max<size_t>(bytes, 1));
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            auto it = free_blocks.find(c);
-            if (it != free_blocks.end() && !it->second.empty()) {
-                void* p = it->second.back();
-                it->second.pop_back();
-                return p;
-            }
-        }
-        void* p = nullptr;
-        hip_check(hipMalloc(&p, c), "hipMalloc");
-        return p;
-    }
-    void put(void* p, size_t bytes) {
-        if (p == nullptr) return;
-        std::lock_guard<std::mutex> lk(mu);
-        free_blocks[size_class(std::max<size_t>(bytes, 1))].push_back(p);
-    }
-};
-
-block_cache& result_cache() {
-    static block_cache* c = new block_cache();  // lives until the process ends
-    return *c;
-}
-
-template <typename T>
-T* dev_copy(const T* src, size_t n) {
-    T* d = nullptr;
-    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    hip_check(hipMalloc(&d, bytes), "hipMalloc");
-    if (n > 0 && src != nullptr) {
-        hip_check(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    }
-    return d;
-}
+using tamd::dbuf;
+using tamd::dev_copy;
+using tamd::result_cache;
 
 void validate_config(const ckf_finding_config& c) {
     // traccc: core/src/finding/combinatorial_kalman_filter_algorithm.cpp:27-31
@@ -152,63 +82,6 @@ void validate_config(const ckf_finding_config& c) {
 
 }  // namespace
 
-struct ckf_device_detector {
-    std::vector<ckf_surface> h_sfs;
-    std::vector<ckf_layer> h_layers;
-    std::vector<uint64_t> ids;
-    ckf_bfield field{};
-    ckf_surface* d_sfs = nullptr;
-    ckf_layer* d_layers = nullptr;
-    float* d_map = nullptr;
-    ~ckf_device_detector() {
-        if (d_sfs) (void)hipFree(d_sfs);
-        if (d_layers) (void)hipFree(d_layers);
-        if (d_map) (void)hipFree(d_map);
-    }
-};
-
-struct ckf_device_event {
-    uint32_t n_events = 0, n_meas = 0, n_seeds = 0, n_surfaces = 0;
-    std::vector<uint32_t> meas_off, seed_off;  // [n_events + 1]
-    float2* d_pos = nullptr;
-    float2* d_var = nullptr;
-    uint32_t* d_info = nullptr;
-    uint32_t* d_ranges = nullptr;
-    uint32_t* d_seed_event = nullptr;
-    uint32_t* d_seed_surface = nullptr;
-    float* d_seed_vec = nullptr;
-    float* d_seed_cov = nullptr;
-    uint32_t* d_meas_off = nullptr;
-    ~ckf_device_event() {
-        void* ps[] = {d_pos, d_var, d_info, d_ranges, d_seed_event, d_seed_surface,
-                      d_seed_vec, d_seed_cov, d_meas_off};
-        for (void* p : ps) {
-            if (p) (void)hipFree(p);
-        }
-    }
-};
-
-struct ckf_device_tracks {
-    uint32_t n_events = 0, n_tracks = 0, n_const = 0;
-    std::vector<uint32_t> ev_track_off, ev_const_off;
-    std::vector<uint64_t> ids;
-    uint16_t* outcome = nullptr;
-    uint32_t* surface = nullptr;
-    float* vec = nullptr;
-    float* cov = nullptr;
-    float* ndf = nullptr;
-    float* chi2 = nullptr;
-    float* pval = nullptr;
-    uint32_t* nholes = nullptr;
-    uint32_t* offsets = nullptr;
-    uint32_t* cons = nullptr;
-    size_t track_bytes[10] = {};
-    ~ckf_device_tracks() {
-        void* ps[] = {outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons};
-        for (int i = 0; i < 10; ++i) result_cache().put(ps[i], track_bytes[i]);
-    }
-};
-
 struct ckf_algorithm {
     ckf_finding_config cfg{};
     hipStream_t stream = nullptr;
@@ -227,7 +100,8 @@ struct ckf_algorithm {
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
-    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_feat, cand_keep;
+    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row, list_feat,
+        cand_keep;
     bool logging = false;
     const ckf_scorer* scorer = nullptr;
     float score_threshold = 0.f;
@@ -245,7 +119,7 @@ struct ckf_algorithm {
     uint32_t schedule_override = 0;   // fixed lanes per link, 0 = autotuned
     std::map<uint32_t, uint32_t> tuned;  // log2(seeds) -> lanes per link
     bool timing = false;
-    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0;
+    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0, t_score = 0.0;
     uint32_t n_prop_launches = 0;
 
     ~ckf_algorithm() {
@@ -259,7 +133,7 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_feat,
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat,
                        &cand_keep};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
@@ -344,6 +218,7 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         A.list_feat.ensure(4u * 12u * rows);
         A.cand_keep.ensure(rows);
         A.list_n.ensure(4u * size_t(A.in_cap));
+        A.list_row.ensure(4u * size_t(A.in_cap));
     }
     if (A.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
@@ -431,6 +306,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
         P.list_m = A.list_m.as<uint32_t>();
         P.list_c = A.list_c.as<float>();
         P.list_n = A.list_n.as<uint32_t>();
+        P.list_row = A.list_row.as<uint32_t>();
         P.list_feat = A.list_feat.as<float>();
         P.cand_keep = A.cand_keep.as<uint8_t>();
         P.scorer = tamd::scorer_view(A.scorer);
@@ -453,7 +329,8 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
     // scorer-pruned finding: 16 inputs per wave listing, 64 rows per scoring wave
     s.list_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
-    const uint64_t rows = uint64_t(A.in_cap) * std::max(A.list_len, 1u);
+    // packed rows: about one per input (the grid strides over any excess)
+    const uint64_t rows = 2u * uint64_t(A.in_cap);
     s.score_blocks = static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>((rows / 64u + 4u) / 4u, max_blocks)));
     s.build_blocks = blocks(A.tip_cap);
     return s;
@@ -461,13 +338,15 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
 
 // Step loop with HIP events around every kernel group, one step at a time.
 uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::launch_shape& s) {
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
-    A.t_find = A.t_dedup = A.t_prop = A.t_other = 0.0;
+    A.t_find = A.t_dedup = A.t_prop = A.t_other = A.t_score = 0.0;
     A.n_prop_launches = 0u;
     for (uint32_t k = 0; k < A.cfg.max_track_candidates_per_track + 1u; ++k) {
+        hip_check(hipEventRecord(ev[5], A.stream), "record");
+        hip_check(tamd::launch_score_lists(P, s, A.stream), "score lists");
         hip_check(hipEventRecord(ev[0], A.stream), "record");
-        hip_check(tamd::launch_find(P, s, A.stream), "find");
+        hip_check(tamd::launch_find_update(P, s, A.stream), "find");
         hip_check(hipEventRecord(ev[1], A.stream), "record");
         hip_check(tamd::launch_dedup(P, s, A.stream), "dedup");
         hip_check(hipEventRecord(ev[2], A.stream), "record");
@@ -479,8 +358,10 @@ uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd
                                  A.stream),
                   "ctl readback");
         hip_check(hipStreamSynchronize(A.stream), "sync");
-        float ms[4];
+        float ms[4], ms_score = 0.f;
         for (int j = 0; j < 4; ++j) hip_check(hipEventElapsedTime(&ms[j], ev[j], ev[j + 1]), "elapsed");
+        hip_check(hipEventElapsedTime(&ms_score, ev[5], ev[0]), "elapsed");
+        A.t_score += ms_score;
         A.t_find += ms[0];
         A.t_dedup += ms[1];
         A.t_prop += ms[2];
@@ -870,6 +751,7 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
     A.stats.n_links = done.n_links_total;
     A.stats.n_propagations = done.n_props;
     A.stats.n_tips = done.n_tips;
+    A.stats.n_scored = done.n_scored;
 
     // build_tracks: count, size, fill (the build marks logged links on tracks)
     if (A.logging) {
@@ -978,7 +860,7 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
         E->n_meas = nm;
         E->n_seeds = ns;
         std::vector<float2> pos(nm), var(nm);
-        std::vector<uint32_t> info(nm), ranges(size_t(n_events) * (S + 1u));
+        std::vector<uint32_t> info(nm), ranges(size_t(n_events) * (S + 1u)), meas_sf(nm, 0u);
         std::vector<uint32_t> seed_event(ns), seed_surface(ns);
         std::vector<float> seed_vec(size_t(ns) * 6u), seed_cov(size_t(ns) * 36u);
         for (uint32_t e = 0; e < n_events; ++e) {
@@ -1014,6 +896,7 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
                 const uint64_t* b = M.surface_link;
                 const uint32_t ub = static_cast<uint32_t>(std::upper_bound(b, b + M.size, id) - b);
                 row[s + 1u] = off + ub;
+                for (uint32_t m = row[s]; m < row[s + 1u]; ++m) meas_sf[m] = s;
             }
             const ckf_bound_params& B = seeds[e];
             if (B.size > 0u && (B.surface_link == nullptr || B.vector == nullptr ||
@@ -1041,6 +924,7 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
         E->d_seed_vec = dev_copy(seed_vec.data(), seed_vec.size());
         E->d_seed_cov = dev_copy(seed_cov.data(), seed_cov.size());
         E->d_meas_off = dev_copy(E->meas_off.data(), n_events);
+        E->d_meas_sf = dev_copy(meas_sf.data(), nm);
     } catch (...) {
         delete E;
         throw;
@@ -1490,6 +1374,13 @@ ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, dou
         if (ms_propagate) *ms_propagate = alg->t_prop;
         if (ms_other) *ms_other = alg->t_other;
         if (n_steps) *n_steps = alg->n_prop_launches;
+    });
+}
+
+ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list_score) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        if (ms_list_score) *ms_list_score = alg->t_score;
     });
 }
 

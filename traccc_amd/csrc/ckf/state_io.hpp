@@ -67,55 +67,37 @@ __device__ inline void load_ref(const ckf_params& P, uint32_t ref, uint32_t in_p
     }
 }
 
-// Multiple-scattering noise of the module the state sits on (covariance part
+// Multiple-scattering noise of module S for a state on it (covariance part
 // of the pointwise material interaction).
-__device__ inline void add_material(const ckf_params& P, uint32_t sf, const float* x, float* c) {
-    const float t = P.sfs[sf].thickness_x0;
+__device__ inline void add_material_sf(const ckf_surface& S, float mass, float abs_q,
+                                       const float* x, float* c) {
+    const float t = S.thickness_x0;
     if (!(t > 0.f)) {
         return;
     }
-    const ckf_surface& S = P.sfs[sf];
     float sp, cp, st, ct;
     sincosf(x[e_phi], &sp, &cp);
     sincosf(x[e_theta], &st, &ct);
     const float cosi = fabsf(cp * st * S.normal[0] + sp * st * S.normal[1] + ct * S.normal[2]);
-    const float t0 = highland_theta0<float>(t / cosi, x[e_qop], P.nav.mass, P.nav.abs_q);
+    const float t0 = highland_theta0<float>(t / cosi, x[e_qop], mass, abs_q);
     const float v = t0 * t0;
     c[sym_idx(e_phi, e_phi)] += v / (st * st);
     c[sym_idx(e_theta, e_theta)] += v;
 }
 
-// Transport (x, c) from surface sf to the next sensitive surface
-// (propagate_to_next_surface's stepper, navigator and parameter transport,
-// without the material of the destination).  On NAV_FOUND x, c and sf hold
-// the state on the new surface; otherwise they are unchanged.  n_steps and
-// path receive the RK steps and the path length of the leg.  With kT > 1 the
-// kT lanes of a team call this together with the same state (tl: the lane's
-// index in the team) and all get the same result.
-template <int kT = 1>
-__device__ inline int transport_state(const ckf_params& P, const ckf_layer* layers,
-                                      const nav_config<float>& cfg, float* x, float* c,
-                                      uint32_t& sf, uint32_t& n_steps, float& path,
-                                      uint32_t tl = 0u) {
-    navigator<float, true, kT> nav;
-    nav.tl = tl;
-    nav.sfs = P.sfs;
-    nav.layers = layers;
-    nav.n_layers = P.n_layers;
-    nav.field = P.field;
-    nav.cfg = cfg;
-    bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
-    uint32_t end_sf = 0u;
-    const int status = nav.propagate(static_cast<int>(sf), end_sf);
-    n_steps = nav.n_steps;
-    path = nav.path;
-    if (status != NAV_FOUND) {
-        return status;
-    }
-    float nx[6];
-    float Jb[6][6];
-    free_to_bound<float, true>(P.sfs[end_sf], nav.st, nav.J, P.field, cfg.mass, cfg.abs_q, nx, Jb);
-    // C' = Jb C Jb^T
+__device__ inline void add_material(const ckf_params& P, uint32_t sf, const float* x, float* c) {
+    add_material_sf(P.sfs[sf], P.nav.mass, P.nav.abs_q, x, c);
+}
+
+// Geometry and field a transport needs.
+struct geo_view {
+    const ckf_surface* sfs;
+    uint32_t n_layers;
+    field_view<float> field;
+};
+
+// c <- Jb c Jb^T on the packed symmetric covariance.
+__device__ inline void transport_cov(const float (&Jb)[6][6], float* c) {
     float M[6][6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
@@ -137,10 +119,81 @@ __device__ inline int transport_state(const ckf_params& P, const ckf_layer* laye
             c[sym_idx(r, q)] = acc;
         }
     }
+}
+
+// Transport (x, c) from surface sf to the next sensitive surface
+// (propagate_to_next_surface's stepper, navigator and parameter transport,
+// without the material of the destination).  On NAV_FOUND x, c and sf hold
+// the state on the new surface; otherwise they are unchanged.  n_steps and
+// path receive the RK steps and the path length of the leg.  With kT > 1 the
+// kT lanes of a team call this together with the same state (tl: the lane's
+// index in the team) and all get the same result.
+template <int kT = 1>
+__device__ inline int transport_bound(const geo_view& G, const ckf_layer* layers,
+                                      const nav_config<float>& cfg, float* x, float* c,
+                                      uint32_t& sf, uint32_t& n_steps, float& path,
+                                      uint32_t tl = 0u) {
+    navigator<float, true, kT> nav;
+    nav.tl = tl;
+    nav.sfs = G.sfs;
+    nav.layers = layers;
+    nav.n_layers = G.n_layers;
+    nav.field = G.field;
+    nav.cfg = cfg;
+    bound_to_free<float, true>(G.sfs[sf], x, nav.st, nav.J);
+    uint32_t end_sf = 0u;
+    const int status = nav.propagate(static_cast<int>(sf), end_sf);
+    n_steps = nav.n_steps;
+    path = nav.path;
+    if (status != NAV_FOUND) {
+        return status;
+    }
+    float nx[6];
+    float Jb[6][6];
+    free_to_bound<float, true>(G.sfs[end_sf], nav.st, nav.J, G.field, cfg.mass, cfg.abs_q, nx, Jb);
+    transport_cov(Jb, c);
 #pragma unroll
     for (int k = 0; k < 6; ++k) x[k] = nx[k];
     sf = end_sf;
     return NAV_FOUND;
+}
+
+template <int kT = 1>
+__device__ inline int transport_state(const ckf_params& P, const ckf_layer* layers,
+                                      const nav_config<float>& cfg, float* x, float* c,
+                                      uint32_t& sf, uint32_t& n_steps, float& path,
+                                      uint32_t tl = 0u) {
+    const geo_view G{P.sfs, P.n_layers, P.field};
+    return transport_bound<kT>(G, layers, cfg, x, c, sf, n_steps, path, tl);
+}
+
+// Transport (x, c) from surface sf straight onto surface `target` (forwards
+// or backwards along the track, no navigation: the surface sequence of a
+// fit).  0 on success (x, c, sf updated), NAV_ABORT when the step or path
+// limit is hit or the plane is parallel to the track (x, c, sf unchanged).
+__device__ inline int transport_to_surface(const geo_view& G, const nav_config<float>& cfg,
+                                           float* x, float* c, uint32_t& sf, uint32_t target) {
+    navigator<float, true, 1> nav;
+    nav.tl = 0u;
+    nav.sfs = G.sfs;
+    nav.layers = nullptr;
+    nav.n_layers = G.n_layers;
+    nav.field = G.field;
+    nav.cfg = cfg;
+    nav.path = 0.f;
+    nav.n_steps = 0u;
+    bound_to_free<float, true>(G.sfs[sf], x, nav.st, nav.J);
+    if (nav.step_to_plane(G.sfs[target]) != 0) {
+        return NAV_ABORT;
+    }
+    float nx[6];
+    float Jb[6][6];
+    free_to_bound<float, true>(G.sfs[target], nav.st, nav.J, G.field, cfg.mass, cfg.abs_q, nx, Jb);
+    transport_cov(Jb, c);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = nx[k];
+    sf = target;
+    return 0;
 }
 
 }  // namespace tamd

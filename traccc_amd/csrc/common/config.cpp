@@ -8,6 +8,7 @@
 #include <limits>
 
 #include "traccc_amd/ckf.h"
+#include "traccc_amd/fitting.h"
 
 extern "C" void ckf_finding_config_default(ckf_finding_config* c) {
     if (c == nullptr) {
@@ -45,4 +46,28 @@ extern "C" void ckf_finding_config_default(ckf_finding_config* c) {
     c->ptc_charge = 1.f;
     c->duplicate_removal_minimum_length = 5u;
     c->initial_links_per_seed = 100u;
+}
+
+// fitting_config defaults: traccc::fitting_config
+// (core/include/traccc/fitting/fitting_config.hpp:20-45); the propagation,
+// momentum cuts and particle hypothesis default to finding_config's.
+extern "C" void ckf_fitting_config_default(ckf_fitting_config* cfg) {
+    if (cfg == nullptr) {
+        return;
+    }
+    ckf_finding_config f{};
+    ckf_finding_config_default(&f);
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->n_iterations = 1u;
+    cfg->propagation = f.propagation;
+    cfg->max_step_counts = f.max_step_counts_for_next_surface;
+    cfg->min_p = f.min_p;
+    cfg->min_pT = f.min_pT;
+    cfg->ptc_pdg = f.ptc_pdg;
+    cfg->ptc_mass = f.ptc_mass;
+    cfg->ptc_charge = f.ptc_charge;
+    cfg->covariance_inflation_factor = 1e3f;
+    cfg->surface_sequence_size_factor = 5u;
+    cfg->min_surface_sequence_capacity = 100u;
+    cfg->do_precise_hole_count = 0u;
 }
