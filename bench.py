@@ -188,19 +188,31 @@ def pruned_line(a, ckf, S, torch, wl="C4", n_particles=20000, n_test=32, n_train
     from traccc_amd import _ffi as F
     lib = F.load_library(ckf.LIB)
     lib.ckf_debug_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
-    lib.ckf_debug_scorer_time.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.ckf_debug_scorer_time.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+    lib.ckf_debug_kernel_times.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4 + [
+        C.POINTER(C.c_uint32)]
     lib.ckf_debug_set_kernel_timing(pruned._h, 1)
     pruned.run_device(ddet, dev)
-    ms_score = C.c_double()
-    lib.ckf_debug_scorer_time(pruned._h, C.byref(ms_score))
+    ms_list, ms_score = C.c_double(), C.c_double()
+    lib.ckf_debug_scorer_time(pruned._h, C.byref(ms_list), C.byref(ms_score))
+    kt = [C.c_double() for _ in range(4)]
+    nst = C.c_uint32()
+    lib.ckf_debug_kernel_times(pruned._h, *[C.byref(v) for v in kt], C.byref(nst))
     lib.ckf_debug_set_kernel_timing(pruned._h, 0)
+    pruned_kernel_ms = {"list": round(ms_list.value, 3), "score": round(ms_score.value, 3),
+                        "find_update": round(kt[0].value, 3), "dedup": round(kt[1].value, 3),
+                        "propagate": round(kt[2].value, 3), "advance": round(kt[3].value, 3)}
     n_scored = pruned.last_stats().n_scored
+    kp = kernel_times(plain, ddet, dev)
+    plain_kernel_ms = {k: round(kp[k], 3) for k in ("find", "dedup", "propagate", "advance")}
     mlp_flops = 2 * (12 * 64 + 64 * 64 + 64)  # per scored candidate
     infer_tflops = n_scored * mlp_flops / (ms_score.value * 1e-3) / 1e12 if ms_score.value else 0.0
     log(f"[bench] {wl} batch {len(test)} events: plain {dt_p * 1e3:.2f} ms ({n_seeds / dt_p:.4g} "
         f"tracks/s, eff {eff_p:.4f}), pruned {dt_s * 1e3:.2f} ms ({n_seeds / dt_s:.4g} tracks/s, "
         f"eff {eff_s:.4f}) at threshold {chosen}; scorer {n_scored} candidates in "
-        f"{ms_score.value:.3f} ms ({infer_tflops:.2f} TFLOP/s)")
+        f"{ms_score.value:.3f} ms ({infer_tflops:.2f} TFLOP/s); pruned kernel ms {pruned_kernel_ms}; "
+        f"plain kernel ms {plain_kernel_ms}")
     line = {
         "workload": f"{wl}: {len(test)} events x {n_particles:,} pions (pT 0.3 GeV + Exp(1 GeV), "
                     f"|eta|<2.5), 10% noise, 3 branches per surface, one operator call",
@@ -226,8 +238,10 @@ def pruned_line(a, ckf, S, torch, wl="C4", n_particles=20000, n_test=32, n_train
             "inference_mfma": {"achieved": round(infer_tflops, 3), "peak": 2500.0,
                                "unit": "TFLOP/s", "frac": round(infer_tflops / 2500.0, 6),
                                "scored_candidates": int(n_scored),
-                               "list_and_score_ms": round(ms_score.value, 4)},
+                               "score_kernel_ms": round(ms_score.value, 4)},
         },
+        "pruned_kernel_ms_per_call": pruned_kernel_ms,
+        "plain_kernel_ms_per_call": plain_kernel_ms,
     }
     if fit:
         from traccc_amd import fitting as FT

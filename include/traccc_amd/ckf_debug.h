@@ -56,9 +56,10 @@ ckf_status ckf_debug_get_schedule(const ckf_algorithm* alg, uint32_t* lanes_per_
 ckf_status ckf_debug_mfma_layout(const float* A, const float* B, float* D);
 ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, double* ms_dedup,
                                   double* ms_propagate, double* ms_other, uint32_t* n_steps);
-/* Device time of the branch scorer's candidate-list and MLP kernels in the
- * last timed call (0 without a scorer); not included in ms_find. */
-ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list_score);
+/* Device time of the branch scorer's candidate-list kernel and of its MLP
+ * kernel in the last timed call (0 without a scorer); not included in
+ * ms_find. */
+ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list, double* ms_score);
 
 #ifdef __cplusplus
 }

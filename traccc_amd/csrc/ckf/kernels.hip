@@ -53,6 +53,8 @@ __device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.li
 // arithmetic, so a round costs one memory latency.
 constexpr int kTeam = 4;
 constexpr int kTeamsPerWave = kWave / kTeam;
+// scorer lists: up to kTeam * kListSlots candidates per input
+constexpr int kListSlots = 4;
 constexpr int kBatch = 4;
 
 // One pass of a team over the surface's measurements [lo, up): the best
@@ -117,6 +119,125 @@ struct find_input {
     bool rule0;
 };
 
+// Candidates of one input in (chi2, index) order from a single pass over the
+// surface's measurements: every lane keeps the kLaneKeep best passing
+// candidates of its share in registers (sorted), and next() merges the
+// team's lists with one team minimum per candidate.  When some lane of the
+// team saw more passing candidates than it keeps, the stream falls back to
+// one scan_next pass per candidate (the order is the same either way).
+constexpr int kLaneKeep = 4;
+
+struct cand_stream {
+    float lc[kLaneKeep];
+    uint32_t lm[kLaneKeep];
+    uint32_t head;      // next own list entry
+    uint32_t n_pass;    // team: measurements passing the cut
+    bool fallback;      // team-uniform
+    bool have_last;
+    float last_c;
+    uint32_t last_m;
+};
+
+__device__ inline void stream_init(const ckf_params& P, const find_input& in, uint32_t tl,
+                                   cand_stream& S) {
+    const float inf = __int_as_float(0x7f800000);
+#pragma unroll
+    for (int j = 0; j < kLaneKeep; ++j) {
+        S.lc[j] = inf;
+        S.lm[j] = kInvalid;
+    }
+    uint32_t np = 0u;
+    for (uint32_t m0 = in.lo + tl; m0 < in.up; m0 += kTeam * kBatch) {
+        meas_data mb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < in.up ? m : in.lo);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const float chi2 = predicted_chi2(mb[u], in.x, in.c);
+            bool take = m < in.up && chi2 <= P.chi2_max && chi2 >= 0.f;
+            take = take && !(in.rule0 && chi2 != 0.f);
+            if (take) {
+                ++np;
+                // insertion into the list sorted by (chi2, index)
+                float cc = chi2;
+                uint32_t cm = m;
+#pragma unroll
+                for (int j = 0; j < kLaneKeep; ++j) {
+                    if (cc < S.lc[j] || (cc == S.lc[j] && cm < S.lm[j])) {
+                        const float tc = S.lc[j];
+                        const uint32_t tm = S.lm[j];
+                        S.lc[j] = cc;
+                        S.lm[j] = cm;
+                        cc = tc;
+                        cm = tm;
+                    }
+                }
+            }
+        }
+    }
+    uint32_t over = np > static_cast<uint32_t>(kLaneKeep) ? 1u : 0u;
+#pragma unroll
+    for (int d = kTeam / 2; d >= 1; d >>= 1) {
+        np += __shfl_xor(np, d, kTeam);
+        over |= __shfl_xor(over, d, kTeam);
+    }
+    S.n_pass = np;
+    S.fallback = over != 0u;
+    S.head = 0u;
+    S.have_last = false;
+    S.last_c = 0.f;
+    S.last_m = 0u;
+}
+
+__device__ inline void stream_next(const ckf_params& P, const find_input& in, uint32_t tl,
+                                   cand_stream& S, float& best_c, uint32_t& best_m) {
+    if (S.fallback) {
+        uint32_t unused = 0u;
+        scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, S.have_last, S.last_c, S.last_m, tl,
+                  false, best_c, best_m, unused);
+    } else {
+        // own head: a select chain keeps the list in registers
+        float c = S.lc[kLaneKeep - 1];
+        uint32_t m = S.lm[kLaneKeep - 1];
+#pragma unroll
+        for (int j = kLaneKeep - 2; j >= 0; --j) {
+            if (S.head == static_cast<uint32_t>(j)) {
+                c = S.lc[j];
+                m = S.lm[j];
+            }
+        }
+        if (S.head >= static_cast<uint32_t>(kLaneKeep)) {
+            m = kInvalid;
+        }
+        best_c = c;
+        best_m = m;
+        // team minimum of (chi2, index); kInvalid sorts last
+#pragma unroll
+        for (int d = kTeam / 2; d >= 1; d >>= 1) {
+            const float oc = __shfl_xor(best_c, d, kTeam);
+            const uint32_t om = __shfl_xor(best_m, d, kTeam);
+            const bool take = (om != kInvalid) &&
+                              (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
+            if (take) {
+                best_c = oc;
+                best_m = om;
+            }
+        }
+        if (best_m != kInvalid && best_m == m) {
+            ++S.head;  // the owner moves to its next entry
+        }
+    }
+    if (best_m != kInvalid) {
+        S.have_last = true;
+        S.last_c = best_c;
+        S.last_m = best_m;
+    }
+}
+
 __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i,
                                   uint32_t prev_base, find_input& in) {
     const uint32_t in_par = (step + 1u) & 1u;
@@ -168,26 +289,27 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         const bool active = i < n_in;
         find_input in;
         uint32_t n_pass = 0u, k = 0u;
+        // candidate q of the list is held by team lane q % kTeam in slot q / kTeam
+        uint32_t sm[kListSlots];
+        float sc[kListSlots];
         if (active) {
             load_input(P, step, i, prev_base, in);
-            bool have_last = false;
-            float last_c = 0.f;
-            uint32_t last_m = 0u;
+            cand_stream cs;
+            stream_init(P, in, tl, cs);
+            n_pass = cs.n_pass;
             for (; k < Lc; ++k) {
                 float best_c;
                 uint32_t best_m;
-                scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, have_last, last_c, last_m, tl,
-                          k == 0u, best_c, best_m, n_pass);
+                stream_next(P, in, tl, cs, best_c, best_m);
                 if (best_m == kInvalid) {
                     break;
                 }
-                have_last = true;
-                last_c = best_c;
-                last_m = best_m;
-                if (tl == 0u) {
-                    const size_t r = static_cast<size_t>(i) * Lc + k;
-                    P.list_m[r] = best_m;
-                    P.list_c[r] = best_c;
+#pragma unroll
+                for (int j = 0; j < kListSlots; ++j) {
+                    if (k == static_cast<uint32_t>(j * kTeam) + tl) {
+                        sm[j] = best_m;
+                        sc[j] = best_c;
+                    }
                 }
             }
         }
@@ -200,16 +322,25 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
             base = atomicAdd(&ctl->n_rows, total);
         }
         base = wave_broadcast(base, 0);
-        if (active && tl == 0u) {
-            const uint32_t row0 = base + incl - cnt;
-            P.list_n[i] = k;
-            P.list_row[i] = row0;
-            for (uint32_t q = 0u; q < k; ++q) {
-                const size_t r = static_cast<size_t>(i) * Lc + q;
-                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, P.list_m[r]);
-                candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                   in.chi2_prev, in.ndf_prev, P.list_c[r], q, n_pass,
-                                   P.list_feat + static_cast<size_t>(row0 + q) * kFeatures);
+        const uint32_t row0 = __shfl(base + incl - cnt, static_cast<int>(lane() & ~(kTeam - 1u)), kWave);
+        if (active) {
+            if (tl == 0u) {
+                P.list_n[i] = k;
+                P.list_row[i] = row0;
+            }
+            // the team's lanes write the list and the features of their candidates
+#pragma unroll
+            for (int j = 0; j < kListSlots; ++j) {
+                const uint32_t q = static_cast<uint32_t>(j * kTeam) + tl;
+                if (q < k) {
+                    const size_t r = static_cast<size_t>(i) * Lc + q;
+                    P.list_m[r] = sm[j];
+                    P.list_c[r] = sc[j];
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
+                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
+                                       P.list_feat + static_cast<size_t>(row0 + q) * kFeatures);
+                }
             }
         }
     }
@@ -249,14 +380,18 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
 // the surface's measurements (scan_next) and take candidates in (chi2, index)
 // order; logged: the same, with every filtered slot's features kept for the
 // branch log; scored: the candidates come from the scored list, pruned ones
-// skipped.  The gain update runs redundantly on the team (so it stays
-// uniform) and the team leader writes.  A wave holds kWave / kTeam inputs.
+// skipped -- there is no scan left to split, so a scored team is one lane and
+// a wave holds 64 inputs.  The gain update runs redundantly on the team (so
+// it stays uniform) and the team leader writes.  A wave holds kWave / kT
+// inputs.
 enum find_mode : int { kFindPlain = 0, kFindLogged = 1, kFindScored = 2 };
 
 template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     constexpr bool kScored = (kMode == kFindScored);
     constexpr bool logging = (kMode == kFindLogged);
+    constexpr int kT = kScored ? 1 : kTeam;
+    constexpr uint32_t kPerWave = kWave / kT;
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
         return;
@@ -266,10 +401,10 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t link_base = ctl->link_base;
     const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
+    const uint32_t n_parts = (n_in + kPerWave - 1) / kPerWave;
     const uint32_t N = P.n_branch_surface;
-    const uint32_t team = lane() / kTeam;
-    const uint32_t tl = lane() % kTeam;
+    const uint32_t team = lane() / kT;
+    const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
     for (;;) {
@@ -277,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i = p * kTeamsPerWave + team;
+        const uint32_t i = p * kPerWave + team;
         const bool valid = i < n_in;
         uint32_t count = 0u, seed = 0u, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
         uint32_t in_slot = 0u;
@@ -292,65 +427,92 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             ndf_prev = in.ndf_prev;
             in_slot = in.in_slot;
             chi2_prev = in.chi2_prev;
-            bool have_last = false;
-            float last_c = 0.f;
-            uint32_t last_m = 0u, n_pass = 0u;
+            uint32_t n_pass = 0u;
             uint32_t li = 0u;
             const uint32_t ln = kScored ? P.list_n[i] : 0u;
             const uint32_t row0 = kScored ? P.list_row[i] : 0u;
-            while (count < N) {
-                float best_c = 0.f;
-                uint32_t best_m = kInvalid;
-                if constexpr (kScored) {
-                    const size_t r0 = static_cast<size_t>(i) * P.list_len;
+            if constexpr (kScored) {
+                // one lane per input: the kept list entries in order
+                while (count < N) {
                     while (li < ln && P.cand_keep[row0 + li] == 0u) ++li;
-                    if (li < ln) {
-                        best_m = P.list_m[r0 + li];
-                        best_c = P.list_c[r0 + li];
-                        ++li;
+                    if (li >= ln) {
+                        break;
                     }
-                } else {
-                    scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, have_last, last_c, last_m,
-                              tl, logging && !have_last, best_c, best_m, n_pass);
-                }
-                if (best_m == kInvalid) {
-                    break;
-                }
-                have_last = true;
-                last_c = best_c;
-                last_m = best_m;
-                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
-                float fx[6], fc[21];
+                    const size_t r = static_cast<size_t>(i) * P.list_len + li;
+                    const uint32_t best_m = P.list_m[r];
+                    const float best_c = P.list_c[r];
+                    ++li;
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                    float fx[6], fc[21];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
+                    for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
 #pragma unroll
-                for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
-                int res = CKF_FIT_SUCCESS;
-                if (in.rule0) {
-                    // step 0 on a surface without material: keep the seed,
-                    // take the measurement variances
-                    fc[sym_idx(0, 0)] = md.v0;
-                    fc[sym_idx(1, 1)] = md.v1;
-                } else {
-                    res = gain_update(md, fx, fc);
-                }
-                if (res == CKF_FIT_SUCCESS) {
-                    const uint32_t slot = i * N + count;
-                    if (leader) {
+                    for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
+                    int res = CKF_FIT_SUCCESS;
+                    if (in.rule0) {
+                        fc[sym_idx(0, 0)] = md.v0;
+                        fc[sym_idx(1, 1)] = md.v1;
+                    } else {
+                        res = gain_update(md, fx, fc);
+                    }
+                    if (res == CKF_FIT_SUCCESS) {
+                        const uint32_t slot = i * N + count;
                         if (slot < P.flt_cap) {
                             store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
                             P.cand_meas[slot] = best_m;
                             P.cand_chi2[slot] = best_c;
-                            if constexpr (logging) {
-                                candidate_features(P, md, in.x, in.c, in.sf, step, skip, consec,
-                                                   chi2_prev, ndf_prev, best_c, count, n_pass,
-                                                   P.feat_slot + static_cast<size_t>(slot) * kFeatures);
-                            }
                         } else {
                             set_overflow(ctl, 4u);
                         }
+                        ++count;
                     }
-                    ++count;
+                }
+            } else {
+                // candidates in stream order; the gain update runs on every
+                // lane of the team (uniform), the leader writes
+                cand_stream cs;
+                stream_init(P, in, tl, cs);
+                n_pass = cs.n_pass;
+                while (count < N) {
+                    float best_c = 0.f;
+                    uint32_t best_m = kInvalid;
+                    stream_next(P, in, tl, cs, best_c, best_m);
+                    if (best_m == kInvalid) {
+                        break;
+                    }
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                    float fx[6], fc[21];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
+#pragma unroll
+                    for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
+                    int res = CKF_FIT_SUCCESS;
+                    if (in.rule0) {
+                        // step 0 on a surface without material: keep the
+                        // seed, take the measurement variances
+                        fc[sym_idx(0, 0)] = md.v0;
+                        fc[sym_idx(1, 1)] = md.v1;
+                    } else {
+                        res = gain_update(md, fx, fc);
+                    }
+                    if (res == CKF_FIT_SUCCESS) {
+                        const uint32_t slot = i * N + count;
+                        if (leader) {
+                            if (slot < P.flt_cap) {
+                                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
+                                P.cand_meas[slot] = best_m;
+                                P.cand_chi2[slot] = best_c;
+                                if constexpr (logging) {
+                                    candidate_features(P, md, in.x, in.c, in.sf, step, skip, consec,
+                                                       chi2_prev, ndf_prev, best_c, count, n_pass,
+                                                       P.feat_slot + static_cast<size_t>(slot) * kFeatures);
+                                }
+                            } else {
+                                set_overflow(ctl, 4u);
+                            }
+                        }
+                        ++count;
+                    }
                 }
             }
         }
@@ -364,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             ctl->n_links = lo32(base) + wtotal;
         }
         // head of the seed's contiguous run of inputs
-        uint32_t prev_seed = __shfl(seed, (lane() + kWave - kTeam) % kWave, kWave);
+        uint32_t prev_seed = __shfl(seed, (lane() + kWave - kT) % kWave, kWave);
         if (valid && team == 0u && i > 0u) {
             prev_seed = (step == 0u) ? i - 1u : load_link(P, prev_base + P.live[i - 1u]).seed;
         }
@@ -1047,12 +1209,26 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.scored) {
         hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_score(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    if (P.scored) {
         hipLaunchKernelGGL(k_score, dim3(s.score_blocks), dim3(kBlock), 0, stream, P);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+    const hipError_t e = launch_list(P, s, stream);
+    if (e != hipSuccess) {
+        return e;
+    }
+    return launch_score(P, s, stream);
 }
 
 hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream) {

@@ -30,6 +30,8 @@ hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t s
 // launch_find = the scorer's list + score kernels (scored runs only), then the
 // find / update kernel
 hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_score(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,

@@ -119,7 +119,7 @@ struct ckf_algorithm {
     uint32_t schedule_override = 0;   // fixed lanes per link, 0 = autotuned
     std::map<uint32_t, uint32_t> tuned;  // log2(seeds) -> lanes per link
     bool timing = false;
-    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0, t_score = 0.0;
+    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0, t_score = 0.0, t_list = 0.0;
     uint32_t n_prop_launches = 0;
 
     ~ckf_algorithm() {
@@ -338,13 +338,15 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
 
 // Step loop with HIP events around every kernel group, one step at a time.
 uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::launch_shape& s) {
-    hipEvent_t ev[6];
+    hipEvent_t ev[7];
     for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
-    A.t_find = A.t_dedup = A.t_prop = A.t_other = A.t_score = 0.0;
+    A.t_find = A.t_dedup = A.t_prop = A.t_other = A.t_score = A.t_list = 0.0;
     A.n_prop_launches = 0u;
     for (uint32_t k = 0; k < A.cfg.max_track_candidates_per_track + 1u; ++k) {
         hip_check(hipEventRecord(ev[5], A.stream), "record");
-        hip_check(tamd::launch_score_lists(P, s, A.stream), "score lists");
+        hip_check(tamd::launch_list(P, s, A.stream), "list");
+        hip_check(hipEventRecord(ev[6], A.stream), "record");
+        hip_check(tamd::launch_score(P, s, A.stream), "score");
         hip_check(hipEventRecord(ev[0], A.stream), "record");
         hip_check(tamd::launch_find_update(P, s, A.stream), "find");
         hip_check(hipEventRecord(ev[1], A.stream), "record");
@@ -358,9 +360,11 @@ uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd
                                  A.stream),
                   "ctl readback");
         hip_check(hipStreamSynchronize(A.stream), "sync");
-        float ms[4], ms_score = 0.f;
+        float ms[4], ms_list = 0.f, ms_score = 0.f;
         for (int j = 0; j < 4; ++j) hip_check(hipEventElapsedTime(&ms[j], ev[j], ev[j + 1]), "elapsed");
-        hip_check(hipEventElapsedTime(&ms_score, ev[5], ev[0]), "elapsed");
+        hip_check(hipEventElapsedTime(&ms_list, ev[5], ev[6]), "elapsed");
+        hip_check(hipEventElapsedTime(&ms_score, ev[6], ev[0]), "elapsed");
+        A.t_list += ms_list;
         A.t_score += ms_score;
         A.t_find += ms[0];
         A.t_dedup += ms[1];
@@ -1377,10 +1381,11 @@ ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, dou
     });
 }
 
-ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list_score) {
+ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list, double* ms_score) {
     return guarded([&] {
         if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
-        if (ms_list_score) *ms_list_score = alg->t_score;
+        if (ms_list) *ms_list = alg->t_list;
+        if (ms_score) *ms_score = alg->t_score;
     });
 }
 
