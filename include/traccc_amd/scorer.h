@@ -15,8 +15,11 @@
  *   0 pull in loc0   1 pull in loc1 (0 for 1-D)   2 predicted chi2
  *   3 layer index / layer count   4 q/p (1/GeV)   5 theta
  *   6 holes so far   7 branch depth (CKF step)   8 chi2 sum / ndf sum so far
- *   9 consecutive holes   10 rank of the candidate among the branch's
- *   candidates by chi2   11 log(1 + candidates passing the chi2 cut)
+ *   9 standing: the branch's running chi2 / ndf with this candidate minus
+ *     the lowest such value over the rank-0 candidates of all branches of
+ *     the seed at this step (0 for the seed's best branch)
+ *   10 rank of the candidate among the branch's candidates by chi2
+ *   11 log(1 + candidates passing the chi2 cut)
  *
  * Parameters (flat FP32, ckf_scorer_n_params() = 5057):
  *   W1[12][64] b1[64] W2[64][64] b2[64] W3[64] b3[1]

@@ -63,8 +63,15 @@ def test_branch_log_closed_forms():
     assert np.all((f[:, 2] >= 0) & (f[:, 2] <= cfg.chi2_max))          # chi2
     assert np.all((f[:, 7] >= 0) & (f[:, 7] < st.n_steps))             # depth
     assert np.all(f[:, 10] == 0)                                        # rank (one branch)
+    assert np.all(f[:, 9] == 0)         # standing: a seed's only branch is its best
     assert np.all(f[:, 11] >= np.log(2.0) - 1e-6)                       # >= 1 candidate
     assert np.all(f[:, 6] <= cfg.max_num_skipping_per_cand)            # holes
+    # three branches per surface: a branch never stands below its seed's best
+    cfg3 = _c4_cfg(3)
+    alg3 = ckf.combinatorial_kalman_filter_algorithm(cfg3)
+    _, f3, _, n3 = alg3.run_device_logged(ddet, dev, capacity=600000)
+    f3 = f3.cpu().numpy()
+    assert np.all(f3[:, 9] >= 0) and np.mean(f3[:, 9] > 0) > 0.02
     assert np.all((f[:, 3] >= 0) & (f[:, 3] < 1))                      # layer fraction
 
 

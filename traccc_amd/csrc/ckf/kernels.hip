@@ -280,6 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t team = lane() / kTeam;
     const uint32_t tl = lane() % kTeam;
     const uint32_t Lc = P.list_len;
+    const uint32_t epoch = ctl->epoch;
     for (uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * kTeamsPerWave; ;
          i0 += gridDim.x * (blockDim.x / kWave) * kTeamsPerWave) {
         if (i0 >= n_in) {
@@ -337,9 +338,14 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                     P.list_m[r] = sm[j];
                     P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
+                    float* f = P.list_feat + static_cast<size_t>(row0 + q) * kFeatures;
                     candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
-                                       P.list_feat + static_cast<size_t>(row0 + q) * kFeatures);
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass, f);
+                    P.list_seed[row0 + q] = in.seed;
+                    if (q == 0u) {
+                        standing_offer(P.seed_run_min, in.seed, epoch,
+                                       branch_run(in.chi2_prev, in.ndf_prev, sc[j], md.dims));
+                    }
                 }
             }
         }
@@ -357,6 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t cap = n_in * P.list_len;
     const uint32_t rows = ctl->n_rows < cap ? ctl->n_rows : cap;
+    const uint32_t epoch = ctl->epoch;
     const uint32_t wave = threadIdx.x / kWave;
     short* lds_x = lds + wave * kMlpLdsPerWave;
     short* lds_h = lds_x + 64 * kMlpInPad;
@@ -368,6 +375,9 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
         float f[kMlpIn];
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
+        if (valid) {
+            f[kFeatStanding] = standing_of(P.seed_run_min, P.list_seed[r], epoch, f[kFeatStanding]);
+        }
         const float logit = mlp_tile_logit(P.scorer, f, valid, lds_x, lds_h);
         if (valid) {
             P.cand_keep[r] = logit >= P.score_threshold ? 1u : 0u;
@@ -398,9 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     }
     const uint32_t step = ctl->step;
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
-    const uint32_t link_base = ctl->link_base;
     const uint32_t prev_base = ctl->prev_link_base;
-    const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n_in + kPerWave - 1) / kPerWave;
     const uint32_t N = P.n_branch_surface;
     const uint32_t team = lane() / kT;
@@ -473,6 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                 cand_stream cs;
                 stream_init(P, in, tl, cs);
                 n_pass = cs.n_pass;
+                bool rank0 = true;
                 while (count < N) {
                     float best_c = 0.f;
                     uint32_t best_m = kInvalid;
@@ -481,6 +490,13 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                         break;
                     }
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                    if constexpr (logging) {
+                        if (rank0 && leader) {
+                            standing_offer(P.seed_run_min, in.seed, ctl->epoch,
+                                           branch_run(in.chi2_prev, in.ndf_prev, best_c, md.dims));
+                        }
+                        rank0 = false;
+                    }
                     float fx[6], fc[21];
 #pragma unroll
                     for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
@@ -516,7 +532,39 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                 }
             }
         }
-        const uint32_t n_out = (valid && leader) ? (count > 0u ? count : 1u) : 0u;
+        if (valid && leader) {
+            P.find_count[i] = count;
+        }
+    }
+}
+
+// Links of the step, after k_find: one lane per input, the inputs' link
+// counts (kept branches, or one hole link) laid out by a decoupled look-back
+// over 64-input partitions, then the links written.  Kept apart from k_find so
+// a partition with heavy inputs never holds the look-back chain of every later
+// partition while it works.
+template <bool kLogging>
+__global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t step = ctl->step;
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t link_base = ctl->link_base;
+    const uint32_t prev_base = ctl->prev_link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t n_parts = (n_in + kWave - 1) / kWave;
+    const uint32_t N = P.n_branch_surface;
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[4]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i = p * kWave + lane();
+        const bool valid = i < n_in;
+        const uint32_t count = valid ? P.find_count[i] : 0u;
+        const uint32_t n_out = valid ? (count > 0u ? count : 1u) : 0u;
         const uint32_t incl = wave_inclusive_scan(n_out);
         const uint32_t wtotal = wave_broadcast(incl, kWave - 1);
         const uint64_t base = wave_lookback(P.lb_flag[0], P.lb_agg[0], P.lb_inc[0], p, epoch,
@@ -525,62 +573,84 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         if (p == n_parts - 1u && lane() == 0u) {
             ctl->n_links = lo32(base) + wtotal;
         }
+        // the input's bookkeeping from its previous link
+        uint32_t seed = kInvalid, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
+        uint32_t in_slot = 0u;
+        float chi2_prev = 0.f;
+        if (valid) {
+            if (step == 0u) {
+                seed = i;
+            } else {
+                in_slot = P.live[i];
+                prev_g = prev_base + in_slot;
+                const dlink L = load_link(P, prev_g);
+                seed = L.seed;
+                skip = link_skipped(L);
+                consec = link_consec(L);
+                chi2_prev = L.chi2_sum;
+                ndf_prev = L.ndf_sum;
+            }
+        }
         // head of the seed's contiguous run of inputs
-        uint32_t prev_seed = __shfl(seed, (lane() + kWave - kT) % kWave, kWave);
-        if (valid && team == 0u && i > 0u) {
+        uint32_t prev_seed = __shfl(seed, (lane() + kWave - 1u) % kWave, kWave);
+        if (valid && lane() == 0u && i > 0u) {
             prev_seed = (step == 0u) ? i - 1u : load_link(P, prev_base + P.live[i - 1u]).seed;
         }
-        if (valid && leader) {
-            if (i == 0u || prev_seed != seed) {
-                P.seed_first[seed] = first;
+        if (!valid) {
+            continue;
+        }
+        if (i == 0u || prev_seed != seed) {
+            P.seed_first[seed] = first;
+        }
+        if (count == 0u) {
+            dlink L;
+            L.prev = prev_g;
+            L.meas = kInvalid;
+            L.seed = seed;
+            L.packed = pack_link(step, skip + 1u, consec + 1u);
+            L.chi2 = FLT_MAX;
+            L.chi2_sum = chi2_prev;
+            L.ndf_sum = ndf_prev;
+            L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
+                                       : ((kRefInput << kRefKindShift) | in_slot);
+            const uint32_t g = link_base + first;
+            if (g < P.link_cap && first < P.state_cap) {
+                P.links[g] = L;
+            } else {
+                set_overflow(ctl, 1u);
             }
-            if (count == 0u) {
+        } else {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t slot = i * N + k;
+                if (slot >= P.flt_cap) {
+                    break;
+                }
+                const uint32_t m = P.cand_meas[slot];
+                const float chi2 = P.cand_chi2[slot];
                 dlink L;
                 L.prev = prev_g;
-                L.meas = kInvalid;
+                L.meas = m;
                 L.seed = seed;
-                L.packed = pack_link(step, skip + 1u, consec + 1u);
-                L.chi2 = FLT_MAX;
-                L.chi2_sum = chi2_prev;
-                L.ndf_sum = ndf_prev;
-                L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
-                                           : ((kRefInput << kRefKindShift) | in_slot);
-                const uint32_t g = link_base + first;
-                if (g < P.link_cap && first < P.state_cap) {
+                L.packed = pack_link(step, skip, 0u);
+                L.chi2 = chi2;
+                L.chi2_sum = chi2_prev + chi2;
+                L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
+                L.state_ref = (kRefFiltered << kRefKindShift) | slot;
+                const uint32_t g = link_base + first + k;
+                if (g < P.link_cap && first + k < P.state_cap) {
                     P.links[g] = L;
+                    if constexpr (kLogging) {
+#pragma unroll
+                        for (int q = 0; q < kFeatures; ++q) {
+                            float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
+                            if (q == kFeatStanding) {
+                                v = standing_of(P.seed_run_min, seed, epoch, v);
+                            }
+                            P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
+                        }
+                    }
                 } else {
                     set_overflow(ctl, 1u);
-                }
-            } else {
-                for (uint32_t k = 0; k < count; ++k) {
-                    const uint32_t slot = i * N + k;
-                    if (slot >= P.flt_cap) {
-                        break;
-                    }
-                    const uint32_t m = P.cand_meas[slot];
-                    const float chi2 = P.cand_chi2[slot];
-                    dlink L;
-                    L.prev = prev_g;
-                    L.meas = m;
-                    L.seed = seed;
-                    L.packed = pack_link(step, skip, 0u);
-                    L.chi2 = chi2;
-                    L.chi2_sum = chi2_prev + chi2;
-                    L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
-                    L.state_ref = (kRefFiltered << kRefKindShift) | slot;
-                    const uint32_t g = link_base + first + k;
-                    if (g < P.link_cap && first + k < P.state_cap) {
-                        P.links[g] = L;
-                        if constexpr (logging) {
-#pragma unroll
-                            for (int q = 0; q < kFeatures; ++q) {
-                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] =
-                                    P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
-                            }
-                        }
-                    } else {
-                        set_overflow(ctl, 1u);
-                    }
                 }
             }
         }
@@ -867,7 +937,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->step += 1u;
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
-    for (int k = 0; k < 4; ++k) ctl->ticket[k] = 0u;
+    for (int k = 0; k < 5; ++k) ctl->ticket[k] = 0u;
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }
@@ -1238,6 +1308,11 @@ hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStr
         hipLaunchKernelGGL(k_find<kFindLogged>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     } else {
         hipLaunchKernelGGL(k_find<kFindPlain>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+    }
+    if (P.feat_slot != nullptr) {
+        hipLaunchKernelGGL(k_find_emit<true>, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
+    } else {
+        hipLaunchKernelGGL(k_find_emit<false>, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
     }
     return hipGetLastError();
 }

@@ -101,7 +101,7 @@ struct ckf_algorithm {
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
     dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row, list_feat,
-        cand_keep;
+        cand_keep, find_count, seed_run_min, list_seed;
     bool logging = false;
     const ckf_scorer* scorer = nullptr;
     float score_threshold = 0.f;
@@ -133,7 +133,8 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat,
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat, &find_count,
+                       &seed_run_min, &list_seed,
                        &cand_keep};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
@@ -156,9 +157,9 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     const uint32_t N = A.cfg.max_num_branches_per_surface;
     A.flt_cap = static_cast<uint32_t>(std::min<uint64_t>(uint64_t(A.in_cap) * N, 0x3fffffffu));
     A.state_cap = A.flt_cap;
-    // look-back partitions: k_find takes 16 inputs per wave (4-lane teams),
+    // look-back partitions: k_find_emit takes 64 inputs per wave,
     // k_propagate 64 / team links per wave (teams of up to 16 lanes), the
-    // other kernels 64 items per wave
+    // other kernels 64 items per wave (the 16-input bound is kept as slack)
     A.lb_cap = std::max({(A.in_cap + 15u) / 16u, (A.state_cap + 3u) / 4u, parts(A.tip_cap)}) + 1u;
     const size_t st = size_t(tamd::kStateCols) * 4u;
     A.state[0].ensure(st * A.state_cap);
@@ -168,6 +169,7 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.cand_chi2.ensure(4u * size_t(A.flt_cap));
     A.links.ensure(sizeof(tamd::dlink) * size_t(A.link_cap));
     A.live.ensure(4u * size_t(A.in_cap));
+    A.find_count.ensure(4u * size_t(A.in_cap));
     A.tips.ensure(4u * size_t(A.tip_cap));
     A.seed_first.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
     A.seed_base.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
@@ -210,9 +212,15 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         A.log_label.ensure(size_t(A.link_cap));
         A.seed_best.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
     }
+    if (A.logging || A.scorer != nullptr) {
+        A.seed_run_min.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
+        // every key of a real step compares below all-ones
+        hip_check(hipMemsetAsync(A.seed_run_min.p, 0xff, A.seed_run_min.bytes, A.stream), "memset");
+    }
     if (A.scorer != nullptr) {
         A.list_len = std::min(A.cfg.max_num_branches_per_surface + 4u, 16u);
         const size_t rows = size_t(A.in_cap) * A.list_len;
+        A.list_seed.ensure(4u * rows);
         A.list_m.ensure(4u * rows);
         A.list_c.ensure(4u * rows);
         A.list_feat.ensure(4u * 12u * rows);
@@ -268,6 +276,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.links = A.links.as<tamd::dlink>();
     P.link_cap = A.link_cap;
     P.live = A.live.as<uint32_t>();
+    P.find_count = A.find_count.as<uint32_t>();
     P.tips = A.tips.as<uint32_t>();
     P.tip_cap = A.tip_cap;
     P.seed_first = A.seed_first.as<uint32_t>();
@@ -300,8 +309,12 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
         P.log_label = A.log_label.as<uint8_t>();
         P.seed_best = A.seed_best.as<uint64_t>();
     }
+    if (A.logging || A.scorer != nullptr) {
+        P.seed_run_min = A.seed_run_min.as<uint64_t>();
+    }
     if (A.scorer != nullptr) {
         P.scored = 1u;
+        P.list_seed = A.list_seed.as<uint32_t>();
         P.list_len = A.list_len;
         P.list_m = A.list_m.as<uint32_t>();
         P.list_c = A.list_c.as<float>();
@@ -333,6 +346,7 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     const uint64_t rows = 2u * uint64_t(A.in_cap);
     s.score_blocks = static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>((rows / 64u + 4u) / 4u, max_blocks)));
     s.build_blocks = blocks(A.tip_cap);
+    s.emit_blocks = blocks(A.in_cap);
     return s;
 }
 
