@@ -57,7 +57,8 @@ struct step_ctl {
     uint32_t n_live;      // inputs of the next step (written by propagate)
     uint32_t n_tips;      // tips recorded before this step
     uint32_t n_tips_step; // tips recorded this step (written by propagate)
-    uint32_t ticket[5];   // work-queue tickets: find, dedup, propagate, build, find emit
+    uint32_t ticket[8];   // work-queue tickets: find, dedup, propagate, build, find emit,
+                          // propagate emit, dedup emit, scorer list
     uint32_t overflow;    // bit 0 links, bit 1 per-step states, bit 2 filtered, bit 3 tips, bit 4 inputs
     uint32_t done;
     uint32_t n_steps_run;
@@ -161,6 +162,7 @@ struct ckf_params {
     uint32_t* list_m;             // [in_cap][list_len] measurement
     float* list_c;                // [in_cap][list_len] chi2
     uint32_t* find_count;         // [in_cap] branches each input keeps this step
+    uint8_t* prop_flag;           // [state_cap] propagation outcome: bit 0 next input, bits 1-2 tips
     uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
     uint32_t* list_seed;          // [rows] seed of each scorer row
     uint32_t* list_n;             // [in_cap] entries

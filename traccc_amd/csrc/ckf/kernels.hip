@@ -125,9 +125,9 @@ struct find_input {
 // team's lists with one team minimum per candidate.  When some lane of the
 // team saw more passing candidates than it keeps, the stream falls back to
 // one scan_next pass per candidate (the order is the same either way).
-constexpr int kLaneKeep = 4;
-
+template <int kLaneKeep = 4>
 struct cand_stream {
+    static constexpr int kKeep = kLaneKeep;
     float lc[kLaneKeep];
     uint32_t lm[kLaneKeep];
     uint32_t head;      // next own list entry
@@ -138,8 +138,10 @@ struct cand_stream {
     uint32_t last_m;
 };
 
+template <typename Stream>
 __device__ inline void stream_init(const ckf_params& P, const find_input& in, uint32_t tl,
-                                   cand_stream& S) {
+                                   Stream& S) {
+    constexpr int kLaneKeep = Stream::kKeep;
     const float inf = __int_as_float(0x7f800000);
 #pragma unroll
     for (int j = 0; j < kLaneKeep; ++j) {
@@ -193,8 +195,10 @@ __device__ inline void stream_init(const ckf_params& P, const find_input& in, ui
     S.last_m = 0u;
 }
 
+template <typename Stream>
 __device__ inline void stream_next(const ckf_params& P, const find_input& in, uint32_t tl,
-                                   cand_stream& S, float& best_c, uint32_t& best_m) {
+                                   Stream& S, float& best_c, uint32_t& best_m) {
+    constexpr int kLaneKeep = Stream::kKeep;
     if (S.fallback) {
         uint32_t unused = 0u;
         scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, S.have_last, S.last_c, S.last_m, tl,
@@ -281,11 +285,14 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t tl = lane() % kTeam;
     const uint32_t Lc = P.list_len;
     const uint32_t epoch = ctl->epoch;
-    for (uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * kTeamsPerWave; ;
-         i0 += gridDim.x * (blockDim.x / kWave) * kTeamsPerWave) {
-        if (i0 >= n_in) {
-            break;  // uniform over the wave
+    const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
+    for (;;) {
+        // dynamic partitions: dense surfaces make some inputs far heavier
+        const uint32_t p = wave_ticket(&ctl->ticket[7]);
+        if (p >= n_parts) {
+            break;
         }
+        const uint32_t i0 = p * kTeamsPerWave;
         const uint32_t i = i0 + team;
         const bool active = i < n_in;
         find_input in;
@@ -295,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         float sc[kListSlots];
         if (active) {
             load_input(P, step, i, prev_base, in);
-            cand_stream cs;
+            cand_stream<8> cs;  // a list runs deeper than the kept branches
             stream_init(P, in, tl, cs);
             n_pass = cs.n_pass;
             for (; k < Lc; ++k) {
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             } else {
                 // candidates in stream order; the gain update runs on every
                 // lane of the team (uniform), the leader writes
-                cand_stream cs;
+                cand_stream<4> cs;
                 stream_init(P, in, tl, cs);
                 n_pass = cs.n_pass;
                 bool rank0 = true;
@@ -538,11 +545,23 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     }
 }
 
-// Links of the step, after k_find: one lane per input, the inputs' link
-// counts (kept branches, or one hole link) laid out by a decoupled look-back
-// over 64-input partitions, then the links written.  Kept apart from k_find so
-// a partition with heavy inputs never holds the look-back chain of every later
-// partition while it works.
+// Compaction partitions of the emit kernels: a wave takes kEmitChunks
+// consecutive 64-item chunks per ticket, counts them, runs one decoupled
+// look-back for the whole partition, then writes.  Large partitions keep the
+// look-back chain short (its round trips, not the bytes, bound these kernels).
+constexpr uint32_t kEmitChunks = 16;
+
+// Chunks per partition for n items: as many as keep >= 2048 partitions
+// (every SIMD busy), at most kEmitChunks.
+__device__ inline uint32_t emit_chunks(uint32_t n) {
+    const uint32_t c = n / (kWave * 2048u);
+    return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
+}
+
+// Links of the step, after k_find: the inputs' link counts (kept branches, or
+// one hole link) laid out in input order, then the links written.  Kept apart
+// from k_find so a partition with heavy inputs never holds the look-back
+// chain of every later partition while it works.
 template <bool kLogging>
 __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     step_ctl* ctl = P.ctl;
@@ -554,103 +573,133 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     const uint32_t link_base = ctl->link_base;
     const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n_in + kWave - 1) / kWave;
+    const uint32_t kc = emit_chunks(n_in);
+    const uint32_t part = kWave * kc;
+    const uint32_t n_parts = (n_in + part - 1u) / part;
     const uint32_t N = P.n_branch_surface;
+    auto input_seed = [&](uint32_t j) -> uint32_t {
+        return step == 0u ? j : load_link(P, prev_base + P.live[j]).seed;
+    };
     for (;;) {
         const uint32_t p = wave_ticket(&ctl->ticket[4]);
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i = p * kWave + lane();
-        const bool valid = i < n_in;
-        const uint32_t count = valid ? P.find_count[i] : 0u;
-        const uint32_t n_out = valid ? (count > 0u ? count : 1u) : 0u;
-        const uint32_t incl = wave_inclusive_scan(n_out);
-        const uint32_t wtotal = wave_broadcast(incl, kWave - 1);
+        const uint32_t i0 = p * part;
+        uint32_t n_out[kEmitChunks];
+        uint32_t total = 0u;
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            const uint32_t count = (static_cast<uint32_t>(ch) < kc && i < n_in) ? P.find_count[i] : 0u;
+            n_out[ch] = (static_cast<uint32_t>(ch) < kc && i < n_in) ? (count > 0u ? count : 1u) : 0u;
+            total += wave_broadcast(wave_inclusive_scan(n_out[ch]), kWave - 1);
+        }
         const uint64_t base = wave_lookback(P.lb_flag[0], P.lb_agg[0], P.lb_inc[0], p, epoch,
-                                            pack2(wtotal, 0u));
-        const uint32_t first = lo32(base) + incl - n_out;
+                                            pack2(total, 0u));
         if (p == n_parts - 1u && lane() == 0u) {
-            ctl->n_links = lo32(base) + wtotal;
+            ctl->n_links = lo32(base) + total;
         }
-        // the input's bookkeeping from its previous link
-        uint32_t seed = kInvalid, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
-        uint32_t in_slot = 0u;
-        float chi2_prev = 0.f;
-        if (valid) {
-            if (step == 0u) {
-                seed = i;
-            } else {
-                in_slot = P.live[i];
-                prev_g = prev_base + in_slot;
-                const dlink L = load_link(P, prev_g);
-                seed = L.seed;
-                skip = link_skipped(L);
-                consec = link_consec(L);
-                chi2_prev = L.chi2_sum;
-                ndf_prev = L.ndf_sum;
+        uint32_t run = lo32(base);
+        uint32_t last_seed = (i0 > 0u && lane() == 0u) ? input_seed(i0 - 1u) : kInvalid;
+        last_seed = wave_broadcast(last_seed, 0);
+#pragma unroll 1
+        for (uint32_t ch = 0; ch < kc; ++ch) {
+            if (i0 + ch * kWave >= n_in) {
+                break;  // uniform
             }
-        }
-        // head of the seed's contiguous run of inputs
-        uint32_t prev_seed = __shfl(seed, (lane() + kWave - 1u) % kWave, kWave);
-        if (valid && lane() == 0u && i > 0u) {
-            prev_seed = (step == 0u) ? i - 1u : load_link(P, prev_base + P.live[i - 1u]).seed;
-        }
-        if (!valid) {
-            continue;
-        }
-        if (i == 0u || prev_seed != seed) {
-            P.seed_first[seed] = first;
-        }
-        if (count == 0u) {
-            dlink L;
-            L.prev = prev_g;
-            L.meas = kInvalid;
-            L.seed = seed;
-            L.packed = pack_link(step, skip + 1u, consec + 1u);
-            L.chi2 = FLT_MAX;
-            L.chi2_sum = chi2_prev;
-            L.ndf_sum = ndf_prev;
-            L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
-                                       : ((kRefInput << kRefKindShift) | in_slot);
-            const uint32_t g = link_base + first;
-            if (g < P.link_cap && first < P.state_cap) {
-                P.links[g] = L;
-            } else {
-                set_overflow(ctl, 1u);
+            const uint32_t i = i0 + ch * kWave + lane();
+            const bool valid = i < n_in;
+            // select n_out[ch] without a runtime index into the array
+            uint32_t no = 0u;
+#pragma unroll
+            for (int q = 0; q < static_cast<int>(kEmitChunks); ++q) {
+                if (static_cast<uint32_t>(q) == ch) no = n_out[q];
             }
-        } else {
-            for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t slot = i * N + k;
-                if (slot >= P.flt_cap) {
-                    break;
+            const uint32_t incl = wave_inclusive_scan(no);
+            const uint32_t first = run + incl - no;
+            run += wave_broadcast(incl, kWave - 1);
+            const uint32_t count = valid ? P.find_count[i] : 0u;
+            // the input's bookkeeping from its previous link
+            uint32_t seed = kInvalid, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
+            uint32_t in_slot = 0u;
+            float chi2_prev = 0.f;
+            if (valid) {
+                if (step == 0u) {
+                    seed = i;
+                } else {
+                    in_slot = P.live[i];
+                    prev_g = prev_base + in_slot;
+                    const dlink L = load_link(P, prev_g);
+                    seed = L.seed;
+                    skip = link_skipped(L);
+                    consec = link_consec(L);
+                    chi2_prev = L.chi2_sum;
+                    ndf_prev = L.ndf_sum;
                 }
-                const uint32_t m = P.cand_meas[slot];
-                const float chi2 = P.cand_chi2[slot];
+            }
+            // head of the seed's contiguous run of inputs
+            uint32_t prev_seed = __shfl(seed, (lane() + kWave - 1u) % kWave, kWave);
+            if (lane() == 0u) {
+                prev_seed = last_seed;
+            }
+            last_seed = wave_broadcast(seed, kWave - 1);
+            if (!valid) {
+                continue;
+            }
+            if (i == 0u || prev_seed != seed) {
+                P.seed_first[seed] = first;
+            }
+            if (count == 0u) {
                 dlink L;
                 L.prev = prev_g;
-                L.meas = m;
+                L.meas = kInvalid;
                 L.seed = seed;
-                L.packed = pack_link(step, skip, 0u);
-                L.chi2 = chi2;
-                L.chi2_sum = chi2_prev + chi2;
-                L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
-                L.state_ref = (kRefFiltered << kRefKindShift) | slot;
-                const uint32_t g = link_base + first + k;
-                if (g < P.link_cap && first + k < P.state_cap) {
+                L.packed = pack_link(step, skip + 1u, consec + 1u);
+                L.chi2 = FLT_MAX;
+                L.chi2_sum = chi2_prev;
+                L.ndf_sum = ndf_prev;
+                L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
+                                           : ((kRefInput << kRefKindShift) | in_slot);
+                const uint32_t g = link_base + first;
+                if (g < P.link_cap && first < P.state_cap) {
                     P.links[g] = L;
-                    if constexpr (kLogging) {
-#pragma unroll
-                        for (int q = 0; q < kFeatures; ++q) {
-                            float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
-                            if (q == kFeatStanding) {
-                                v = standing_of(P.seed_run_min, seed, epoch, v);
-                            }
-                            P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
-                        }
-                    }
                 } else {
                     set_overflow(ctl, 1u);
+                }
+            } else {
+                for (uint32_t k = 0; k < count; ++k) {
+                    const uint32_t slot = i * N + k;
+                    if (slot >= P.flt_cap) {
+                        break;
+                    }
+                    const uint32_t m = P.cand_meas[slot];
+                    const float chi2 = P.cand_chi2[slot];
+                    dlink L;
+                    L.prev = prev_g;
+                    L.meas = m;
+                    L.seed = seed;
+                    L.packed = pack_link(step, skip, 0u);
+                    L.chi2 = chi2;
+                    L.chi2_sum = chi2_prev + chi2;
+                    L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
+                    L.state_ref = (kRefFiltered << kRefKindShift) | slot;
+                    const uint32_t g = link_base + first + k;
+                    if (g < P.link_cap && first + k < P.state_cap) {
+                        P.links[g] = L;
+                        if constexpr (kLogging) {
+#pragma unroll
+                            for (int q = 0; q < kFeatures; ++q) {
+                                float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
+                                if (q == kFeatStanding) {
+                                    v = standing_of(P.seed_run_min, seed, epoch, v);
+                                }
+                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
+                            }
+                        }
+                    } else {
+                        set_overflow(ctl, 1u);
+                    }
                 }
             }
         }
@@ -763,16 +812,57 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
                 }
             }
         }
-        const uint64_t m = wave_ballot(valid && alive);
-        const uint32_t wtotal = static_cast<uint32_t>(__popcll(m));
-        const uint64_t b = wave_lookback(P.lb_flag[1], P.lb_agg[1], P.lb_inc[1], p, epoch,
-                                         pack2(wtotal, 0u));
-        const uint32_t before = lo32(b) + prefix_count(m);
         if (valid) {
-            P.alive_prefix[i] = (alive ? 0x80000000u : 0u) | before;
-            if (P.seed_first[seed] == i) {
-                P.seed_base[seed] = before;
+            P.alive_prefix[i] = alive ? 0x80000000u : 0u;  // ranks: k_dedup_emit
+        }
+        (void)seed;
+    }
+}
+
+// After k_dedup_mark: each link's rank among the step's surviving links
+// (alive_prefix = alive bit | rank) and, per seed, the rank of its first link.
+__global__ __launch_bounds__(kBlock) void k_dedup_emit(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done || ctl->step < P.dup_min_len || (ctl->overflow & kFindOverflow) != 0u) {
+        return;
+    }
+    const uint32_t n = links_this_step(P, ctl);
+    const uint32_t base = ctl->link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t kc = emit_chunks(n);
+    const uint32_t part = kWave * kc;
+    const uint32_t n_parts = (n + part - 1u) / part;
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[6]);
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i0 = p * part;
+        uint64_t am[kEmitChunks];
+        uint32_t total = 0u;
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            const bool a = static_cast<uint32_t>(ch) < kc && i < n && (P.alive_prefix[i] >> 31) != 0u;
+            am[ch] = wave_ballot(a);
+            total += static_cast<uint32_t>(__popcll(am[ch]));
+        }
+        const uint64_t b = wave_lookback(P.lb_flag[1], P.lb_agg[1], P.lb_inc[1], p, epoch,
+                                         pack2(total, 0u));
+        uint32_t run = lo32(b);
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            if (static_cast<uint32_t>(ch) < kc && i < n) {
+                const bool a = ((am[ch] >> lane()) & 1ull) != 0ull;
+                const uint32_t before = run + prefix_count(am[ch]);
+                P.alive_prefix[i] = (a ? 0x80000000u : 0u) | before;
+                const uint32_t seed = load_link(P, base + i).seed;
+                if (P.seed_first[seed] == i) {
+                    P.seed_base[seed] = before;
+                }
             }
+            run += static_cast<uint32_t>(__popcll(am[ch]));
         }
     }
 }
@@ -785,7 +875,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl);
 // kT lanes propagate one link together (team navigator); 64 / kT links per
 // wave partition.  The team leader writes the results.
 template <int kT>
-__global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_advance) {
+__global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
         return;
@@ -799,8 +889,6 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
     const uint32_t step = ctl->step;
     const uint32_t n = links_this_step(P, ctl);
     const uint32_t base = ctl->link_base;
-    const uint32_t epoch = ctl->epoch;
-    const uint32_t tip_base = ctl->n_tips;
     constexpr uint32_t kLinksPerWave = kWave / kT;
     const uint32_t n_parts = (n + kLinksPerWave - 1) / kLinksPerWave;
     const uint32_t in_par = (step + 1u) & 1u;
@@ -864,36 +952,82 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P, int fused_ad
                 }
             }
         }
-        // one vote per link: the team leaders'
-        const uint64_t ms = wave_ballot(leader && succ);
-        const uint64_t m1 = wave_ballot(leader && ntip >= 1u);
-        const uint64_t m2 = wave_ballot(leader && ntip >= 2u);
+        // the link's outcome for k_prop_emit; the propagation count here
+        if (valid && leader) {
+            P.prop_flag[i] = static_cast<uint8_t>((succ ? 1u : 0u) | (ntip << 1));
+        }
         const uint64_t mp = wave_ballot(leader && propagated);
-        const uint32_t s_tot = static_cast<uint32_t>(__popcll(ms));
-        const uint32_t t_tot = static_cast<uint32_t>(__popcll(m1) + __popcll(m2));
-        const uint64_t b = wave_lookback(P.lb_flag[2], P.lb_agg[2], P.lb_inc[2], p, epoch,
-                                         pack2(s_tot, t_tot));
         if (lane() == 0u && mp != 0ull) {
             atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props),
                       static_cast<unsigned long long>(__popcll(mp)));
         }
-        if (leader && succ) {
-            const uint32_t pos = lo32(b) + prefix_count(ms);
-            if (pos < P.in_cap) {
-                P.live[pos] = i;
-            } else {
-                set_overflow(ctl, 16u);
-            }
+    }
+}
+
+// After k_propagate: the next step's inputs (links that reached a surface)
+// and the tips, compacted in link order over 1024-link partitions; the last
+// wave to leave runs the step advance (fused_advance) so the step loop never
+// waits on the host.
+__global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_advance) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t n = links_this_step(P, ctl);
+    const uint32_t base = ctl->link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t tip_base = ctl->n_tips;
+    const uint32_t kc = emit_chunks(n);
+    const uint32_t part = kWave * kc;
+    const uint32_t n_parts = (n + part - 1u) / part;
+    for (;;) {
+        const uint32_t p = wave_ticket(&ctl->ticket[5]);
+        if (p >= n_parts) {
+            break;
         }
-        if (leader && ntip > 0u) {
-            uint32_t pos = tip_base + hi32(b) + prefix_count(m1) + prefix_count(m2);
-            for (uint32_t k = 0; k < ntip; ++k, ++pos) {
-                if (pos < P.tip_cap) {
-                    P.tips[pos] = base + i;
+        const uint32_t i0 = p * part;
+        uint32_t fl[kEmitChunks];
+        uint32_t s_tot = 0u, t_tot = 0u;
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            fl[ch] = (static_cast<uint32_t>(ch) < kc && i < n) ? P.prop_flag[i] : 0u;
+            s_tot += static_cast<uint32_t>(__popcll(wave_ballot((fl[ch] & 1u) != 0u)));
+            t_tot += static_cast<uint32_t>(__popcll(wave_ballot((fl[ch] >> 1) >= 1u)) +
+                                           __popcll(wave_ballot((fl[ch] >> 1) >= 2u)));
+        }
+        const uint64_t b = wave_lookback(P.lb_flag[2], P.lb_agg[2], P.lb_inc[2], p, epoch,
+                                         pack2(s_tot, t_tot));
+        uint32_t s_run = lo32(b), t_run = hi32(b);
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            const uint32_t f = fl[ch];
+            const bool succ = (f & 1u) != 0u;
+            const uint32_t ntip = f >> 1;
+            const uint64_t ms = wave_ballot(succ);
+            const uint64_t m1 = wave_ballot(ntip >= 1u);
+            const uint64_t m2 = wave_ballot(ntip >= 2u);
+            if (succ) {
+                const uint32_t pos = s_run + prefix_count(ms);
+                if (pos < P.in_cap) {
+                    P.live[pos] = i;
                 } else {
-                    set_overflow(ctl, 8u);
+                    set_overflow(ctl, 16u);
                 }
             }
+            if (ntip > 0u) {
+                uint32_t pos = tip_base + t_run + prefix_count(m1) + prefix_count(m2);
+                for (uint32_t k = 0; k < ntip; ++k, ++pos) {
+                    if (pos < P.tip_cap) {
+                        P.tips[pos] = base + i;
+                    } else {
+                        set_overflow(ctl, 8u);
+                    }
+                }
+            }
+            s_run += static_cast<uint32_t>(__popcll(ms));
+            t_run += static_cast<uint32_t>(__popcll(m1) + __popcll(m2));
         }
         if (p == n_parts - 1u && lane() == 0u) {
             ctl->n_live = lo32(b) + s_tot;
@@ -937,7 +1071,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->step += 1u;
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
-    for (int k = 0; k < 5; ++k) ctl->ticket[k] = 0u;
+    for (int k = 0; k < 8; ++k) ctl->ticket[k] = 0u;
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }
@@ -1379,6 +1513,7 @@ hipError_t launch_log_write(const ckf_params& P, uint32_t n, const uint32_t* blo
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     hipLaunchKernelGGL(k_dedup_insert, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
     hipLaunchKernelGGL(k_dedup_mark, dim3(s.dedup_blocks), dim3(kBlock), 0, stream, P);
+    hipLaunchKernelGGL(k_dedup_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
     return hipGetLastError();
 }
 
@@ -1387,18 +1522,19 @@ hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStrea
     const int fa = fused_advance ? 1 : 0;
     switch (s.prop_team) {
         case 16:
-            hipLaunchKernelGGL(k_propagate<16>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            hipLaunchKernelGGL(k_propagate<16>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
         case 8:
-            hipLaunchKernelGGL(k_propagate<8>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            hipLaunchKernelGGL(k_propagate<8>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
         case 4:
-            hipLaunchKernelGGL(k_propagate<4>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            hipLaunchKernelGGL(k_propagate<4>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
         default:
-            hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P, fa);
+            hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
     }
+    hipLaunchKernelGGL(k_prop_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P, fa);
     return hipGetLastError();
 }
 

@@ -101,7 +101,7 @@ struct ckf_algorithm {
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
     dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row, list_feat,
-        cand_keep, find_count, seed_run_min, list_seed;
+        cand_keep, find_count, seed_run_min, list_seed, prop_flag;
     bool logging = false;
     const ckf_scorer* scorer = nullptr;
     float score_threshold = 0.f;
@@ -134,7 +134,7 @@ struct ckf_algorithm {
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc, &feat_slot, &log_feat,
                        &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat, &find_count,
-                       &seed_run_min, &list_seed,
+                       &seed_run_min, &list_seed, &prop_flag,
                        &cand_keep};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
@@ -170,6 +170,7 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.links.ensure(sizeof(tamd::dlink) * size_t(A.link_cap));
     A.live.ensure(4u * size_t(A.in_cap));
     A.find_count.ensure(4u * size_t(A.in_cap));
+    A.prop_flag.ensure(size_t(A.state_cap));
     A.tips.ensure(4u * size_t(A.tip_cap));
     A.seed_first.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
     A.seed_base.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
@@ -277,6 +278,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.link_cap = A.link_cap;
     P.live = A.live.as<uint32_t>();
     P.find_count = A.find_count.as<uint32_t>();
+    P.prop_flag = A.prop_flag.as<uint8_t>();
     P.tips = A.tips.as<uint32_t>();
     P.tip_cap = A.tip_cap;
     P.seed_first = A.seed_first.as<uint32_t>();
