@@ -135,6 +135,7 @@ struct ckf_params {
     uint32_t* event_track_off;    // [n_events + 1]
     uint32_t* event_const_off;    // [n_events + 1]
     const uint32_t* event_meas_off;  // [n_events]
+    const float* event_win_var;      // [n_events] largest loc0 variance, -1 = no window search
     // output arrays
     uint16_t* out_outcome;
     uint32_t* out_surface;        // surface index of the seed

@@ -122,9 +122,10 @@ struct ckf_device_event {
     float* d_seed_cov = nullptr;
     uint32_t* d_meas_off = nullptr;
     uint32_t* d_meas_sf = nullptr;  // surface index of every measurement
+    float* d_win_var = nullptr;     // per event: largest loc0 variance, or -1 (no window search)
     ~ckf_device_event() {
         void* ps[] = {d_pos, d_var, d_info, d_ranges, d_seed_event, d_seed_surface,
-                      d_seed_vec, d_seed_cov, d_meas_off, d_meas_sf};
+                      d_seed_vec, d_seed_cov, d_meas_off, d_meas_sf, d_win_var};
         for (void* p : ps) {
             if (p) (void)hipFree(p);
         }

@@ -116,8 +116,45 @@ struct find_input {
     float x[6], c[21];
     uint32_t sf, seed, prev_g, skip, consec, ndf_prev, in_slot, lo, up;
     float chi2_prev;
+    float win_var;  // the event's largest loc0 variance, < 0: no window search
     bool rule0;
 };
+
+// A superset [lo, up) of the surface's measurements whose loc0 can lie in
+// [a, b], for measurements sorted by loc0: cooperative 16-ary narrowing by
+// the team (kTeam lanes x kBatch probes per round, one memory latency each)
+// of the first index >= a and of the first index > b.
+__device__ inline void window_range(const ckf_params& P, float a, float b, uint32_t tl,
+                                    uint32_t& lo, uint32_t& up) {
+    constexpr uint32_t kProbes = kTeam * kBatch;  // 16
+    for (int bound = 0; bound < 2; ++bound) {
+        const float key = bound == 0 ? a : b;
+        uint32_t L = lo, H = up;  // the wanted index lies in [L, H]
+        while (H - L > kProbes) {
+            const uint32_t span = H - L;
+            uint32_t below = 0u;
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t k = static_cast<uint32_t>(u * kTeam) + tl;
+                const uint32_t pos = L + ((k + 1u) * span) / (kProbes + 1u);
+                const float v = P.meas_pos[pos].x;
+                below += (bound == 0 ? (v < key) : !(v > key)) ? 1u : 0u;
+            }
+#pragma unroll
+            for (int d = kTeam / 2; d >= 1; d >>= 1) below += __shfl_xor(below, d, kTeam);
+            // probes are sorted: `below` of them precede the wanted index
+            const uint32_t nL = below == 0u ? L : L + (below * span) / (kProbes + 1u) + 1u;
+            const uint32_t nH = below == kProbes ? H : L + ((below + 1u) * span) / (kProbes + 1u);
+            L = nL;
+            H = nH;
+        }
+        if (bound == 0) {
+            lo = L;
+        } else {
+            up = H;
+        }
+    }
+}
 
 // Candidates of one input in (chi2, index) order from a single pass over the
 // surface's measurements: every lane keeps the kLaneKeep best passing
@@ -149,18 +186,28 @@ __device__ inline void stream_init(const ckf_params& P, const find_input& in, ui
         S.lm[j] = kInvalid;
     }
     uint32_t np = 0u;
-    for (uint32_t m0 = in.lo + tl; m0 < in.up; m0 += kTeam * kBatch) {
+    // a measurement passing the cut has r0^2 <= chi2_max (P00 + V00)
+    // (chi2 >= r0^2 / R00 for any 2x2 SPD R): outside that loc0 window none
+    // can pass, so on a dense surface sorted by loc0 only the window is read
+    uint32_t lo = in.lo, up = in.up;
+    if (in.win_var >= 0.f && up - lo > static_cast<uint32_t>(kTeam * kBatch)) {
+        const float w = sqrtf(P.chi2_max * (in.c[sym_idx(0, 0)] + in.win_var)) * 1.01f + 1e-3f;
+        if (w == w) {
+            window_range(P, in.x[e_loc0] - w, in.x[e_loc0] + w, tl, lo, up);
+        }
+    }
+    for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
         meas_data mb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
-            mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < in.up ? m : in.lo);
+            mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
             const float chi2 = predicted_chi2(mb[u], in.x, in.c);
-            bool take = m < in.up && chi2 <= P.chi2_max && chi2 >= 0.f;
+            bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
             take = take && !(in.rule0 && chi2 != 0.f);
             if (take) {
                 ++np;
@@ -266,6 +313,7 @@ __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i
     const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
     in.lo = rg[in.sf];
     in.up = rg[in.sf + 1u];
+    in.win_var = P.event_win_var[ev];
     in.rule0 = (step == 0u) && !(P.sfs[in.sf].thickness_x0 > 0.f);
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <map>
@@ -305,6 +306,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.event_track_off = A.ev_toff.as<uint32_t>();
     P.event_const_off = A.ev_coff.as<uint32_t>();
     P.event_meas_off = ev.d_meas_off;
+    P.event_win_var = ev.d_win_var;
     if (A.logging) {
         P.feat_slot = A.feat_slot.as<float>();
         P.log_feat = A.log_feat.as<float>();
@@ -881,6 +883,7 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
         E->n_seeds = ns;
         std::vector<float2> pos(nm), var(nm);
         std::vector<uint32_t> info(nm), ranges(size_t(n_events) * (S + 1u)), meas_sf(nm, 0u);
+    std::vector<float> win_var(n_events, -1.f);
         std::vector<uint32_t> seed_event(ns), seed_surface(ns);
         std::vector<float> seed_vec(size_t(ns) * 6u), seed_cov(size_t(ns) * 36u);
         for (uint32_t e = 0; e < n_events; ++e) {
@@ -918,6 +921,23 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
                 row[s + 1u] = off + ub;
                 for (uint32_t m = row[s]; m < row[s + 1u]; ++m) meas_sf[m] = s;
             }
+            // the candidate search may bisect a surface's measurements on
+            // loc0 when they are sorted by it (traccc's measurement ordering)
+            // and every one of them measures loc0
+            {
+                bool ok = true;
+                float vmax = 0.f;
+                for (uint32_t i = 0; i < M.size && ok; ++i) {
+                    const float x = M.local_position[2u * i];
+                    const float v = M.local_variance[2u * i];
+                    ok = M.subspace[2u * i] == 0u && std::isfinite(x) && std::isfinite(v) && v >= 0.f;
+                    if (ok && i > 0u && M.surface_link[i] == M.surface_link[i - 1u]) {
+                        ok = !(x < M.local_position[2u * (i - 1u)]);
+                    }
+                    vmax = std::max(vmax, v);
+                }
+                win_var[e] = ok ? vmax : -1.f;
+            }
             const ckf_bound_params& B = seeds[e];
             if (B.size > 0u && (B.surface_link == nullptr || B.vector == nullptr ||
                                 B.covariance == nullptr)) {
@@ -945,6 +965,7 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
         E->d_seed_cov = dev_copy(seed_cov.data(), seed_cov.size());
         E->d_meas_off = dev_copy(E->meas_off.data(), n_events);
         E->d_meas_sf = dev_copy(meas_sf.data(), nm);
+        E->d_win_var = dev_copy(win_var.data(), n_events);
     } catch (...) {
         delete E;
         throw;
