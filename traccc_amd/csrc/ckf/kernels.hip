@@ -410,34 +410,61 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
 // Scorer-pruned finding, second kernel: the MLP over every listed candidate
 // (64-row tiles on the matrix cores), keep = logit >= threshold.
 __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
-    __shared__ short lds[(kBlock / kWave) * kMlpLdsPerWave];
+    __shared__ mlp_vecs V;
     const step_ctl* ctl = P.ctl;
     if (ctl->done) {
-        return;
+        return;  // the whole grid returns: no barrier is left waiting
     }
+    mlp_fill_vecs(P.scorer, V, threadIdx.x, blockDim.x);
+    __syncthreads();
+    mlp_frags F;
+    mlp_load_frags(P.scorer, F);
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t cap = n_in * P.list_len;
     const uint32_t rows = ctl->n_rows < cap ? ctl->n_rows : cap;
     const uint32_t epoch = ctl->epoch;
     const uint32_t wave = threadIdx.x / kWave;
-    short* lds_x = lds + wave * kMlpLdsPerWave;
-    short* lds_h = lds_x + 64 * kMlpInPad;
+    const uint32_t l = lane();
+    const uint32_t r16 = l % 16u;
+    const uint32_t grp = l / 16u;  // features 4 grp .. 4 grp + 3
     const uint32_t n_tiles = (rows + 63u) / 64u;
     for (uint32_t t = blockIdx.x * (blockDim.x / kWave) + wave; t < n_tiles;
          t += gridDim.x * (blockDim.x / kWave)) {
-        const uint32_t r = t * 64u + lane();
-        const bool valid = r < rows;
-        float f[kMlpIn];
+        float x[4][4];
+        bool valid[4];
 #pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
-        if (valid) {
-            f[kFeatStanding] = standing_of(P.seed_run_min, P.list_seed[r], epoch, f[kFeatStanding]);
+        for (int mt = 0; mt < 4; ++mt) {
+            const uint32_t r = t * 64u + 16u * mt + r16;
+            valid[mt] = r < rows;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t k = 4u * grp + static_cast<uint32_t>(e);
+                x[mt][e] = (valid[mt] && k < static_cast<uint32_t>(kMlpIn))
+                               ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
+            }
+            if (valid[mt] && grp == static_cast<uint32_t>(kFeatStanding / 4)) {
+                float& f = x[mt][kFeatStanding % 4];
+                f = standing_of(P.seed_run_min, P.list_seed[r], epoch, f);
+            }
         }
-        const float logit = mlp_tile_logit(P.scorer, f, valid, lds_x, lds_h);
-        if (valid) {
-            P.cand_keep[r] = logit >= P.score_threshold ? 1u : 0u;
+        // one sub-tile at a time (the next ones shift down) keeps a single
+        // MLP evaluation live, and the wave at 3 waves / SIMD
+        float mine = 0.f;
+#pragma unroll 1
+        for (int mt = 0; mt < 4; ++mt) {
+            const float s = mlp_logit16(F, V, x[0], valid[0]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                valid[q] = valid[q + 1];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[q][e] = x[q + 1][e];
+            }
+            if (grp == static_cast<uint32_t>(mt)) mine = s;  // row 64 t + l
         }
-        wave_lds_sync();
+        const uint32_t r = t * 64u + l;
+        if (r < rows) {
+            P.cand_keep[r] = mine >= P.score_threshold ? 1u : 0u;
+        }
     }
 }
 
@@ -1136,61 +1163,95 @@ __global__ void k_advance(ckf_params P) {
 // ---------------------------------------------------------------------------
 // build_tracks
 // ---------------------------------------------------------------------------
+// Tips in (step, event) segments: each tip's kept rank and constituent
+// prefix, and every segment's bounds.  A partition is emit_chunks(n) chunks
+// of 64 tips with one look-back, so the look-back chain stays short when
+// there are millions of tips; the (event, step) key of a tip is
+// ev * max_cands + step (the host keeps n_events * max_cands below 2^31).
 __global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
     step_ctl* ctl = P.ctl;
     const uint32_t n = ctl->n_tips < P.tip_cap ? ctl->n_tips : P.tip_cap;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n + kWave - 1) / kWave;
+    const uint32_t kc = emit_chunks(n);
+    const uint32_t part = kWave * kc;
+    const uint32_t n_parts = (n + part - 1u) / part;
     const uint32_t S = P.max_cands;
+    auto seg_key = [&](uint32_t g) -> uint32_t {
+        const dlink L = load_link(P, g);
+        return P.seed_event[L.seed] * S + link_step(L);
+    };
     for (;;) {
         const uint32_t p = wave_ticket(&ctl->ticket[3]);
         if (p >= n_parts) {
             break;
         }
-        const uint32_t t = p * kWave + lane();
-        const bool valid = t < n;
-        uint32_t keep = 0u, nc = 0u, ev = 0u, st = 0u;
-        if (valid) {
-            const dlink L = load_link(P, P.tips[t]);
-            st = link_step(L);
-            nc = st + 1u - link_skipped(L);
-            keep = (nc >= P.min_cands && nc <= P.max_cands) ? 1u : 0u;
-            ev = P.seed_event[L.seed];
+        const uint32_t t0 = p * part;
+        uint32_t kn[kEmitChunks];   // keep << 31 | n_cands
+        uint32_t key[kEmitChunks];  // segment key
+        uint32_t k_tot = 0u, c_tot = 0u;
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t t = t0 + static_cast<uint32_t>(ch) * kWave + lane();
+            kn[ch] = 0u;
+            key[ch] = 0u;
+            if (static_cast<uint32_t>(ch) < kc && t < n) {
+                const dlink L = load_link(P, P.tips[t]);
+                const uint32_t st = link_step(L);
+                const uint32_t nc = st + 1u - link_skipped(L);
+                const bool keep = nc >= P.min_cands && nc <= P.max_cands;
+                kn[ch] = (keep ? 0x80000000u : 0u) | nc;
+                key[ch] = P.seed_event[L.seed] * S + st;
+            }
+            const bool keep = (kn[ch] >> 31) != 0u;
+            k_tot += static_cast<uint32_t>(__popcll(wave_ballot(keep)));
+            c_tot += wave_broadcast(wave_inclusive_scan(keep ? (kn[ch] & 0x7fffffffu) : 0u), kWave - 1);
         }
-        const uint32_t w = keep ? nc : 0u;
-        const uint32_t cincl = wave_inclusive_scan(w);
-        const uint64_t mk = wave_ballot(keep != 0u);
-        const uint32_t k_tot = static_cast<uint32_t>(__popcll(mk));
-        const uint32_t c_tot = wave_broadcast(cincl, kWave - 1);
         const uint64_t b = wave_lookback(P.lb_flag[3], P.lb_agg[3], P.lb_inc[3], p, epoch,
                                          pack2(k_tot, c_tot));
-        if (valid) {
-            const uint32_t kp = lo32(b) + prefix_count(mk);
-            const uint32_t cp = hi32(b) + cincl - w;
-            P.tip_keep[t] = (keep ? 0x80000000u : 0u) | nc;
-            P.tip_rank[t] = kp;
-            P.tip_cprefix[t] = cp;
-            bool head = (t == 0u);
-            if (!head) {
-                const dlink Lp = load_link(P, P.tips[t - 1u]);
-                head = (link_step(Lp) != st) || (P.seed_event[Lp.seed] != ev);
+        uint32_t k_run = lo32(b), c_run = hi32(b);
+        // segment key of the tip before this partition's first
+        uint32_t key_before = 0u;
+        if (t0 > 0u && t0 < n) {
+            key_before = seg_key(P.tips[t0 - 1u]);
+        }
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            if (static_cast<uint32_t>(ch) >= kc) {
+                continue;  // kc is wave-uniform
             }
-            const size_t seg = static_cast<size_t>(ev) * S + st;
-            if (head) {
-                P.seg_base[seg] = kp;
-                P.seg_cbase[seg] = cp;
-                // the previous segment ends where this one starts
-                if (t > 0u) {
-                    const dlink Lp = load_link(P, P.tips[t - 1u]);
-                    const size_t prev = static_cast<size_t>(P.seed_event[Lp.seed]) * S + link_step(Lp);
-                    P.seg_count[prev] = kp;
-                    P.seg_ccount[prev] = cp;
+            const uint32_t t = t0 + static_cast<uint32_t>(ch) * kWave + lane();
+            const bool keep = (kn[ch] >> 31) != 0u;
+            const uint32_t nc = kn[ch] & 0x7fffffffu;
+            const uint32_t w = keep ? nc : 0u;
+            const uint32_t cincl = wave_inclusive_scan(w);
+            const uint64_t mk = wave_ballot(keep);
+            // the previous tip's key: the lane below, across chunk boundaries
+            const uint32_t below = __shfl(key[ch], static_cast<int>((lane() + kWave - 1u) % kWave), kWave);
+            const uint32_t prev_key = lane() == 0u ? key_before : below;
+            if (t < n) {
+                const uint32_t kp = k_run + prefix_count(mk);
+                const uint32_t cp = c_run + cincl - w;
+                P.tip_keep[t] = kn[ch];
+                P.tip_rank[t] = kp;
+                P.tip_cprefix[t] = cp;
+                const uint32_t seg = key[ch];
+                if (t == 0u || prev_key != seg) {
+                    P.seg_base[seg] = kp;
+                    P.seg_cbase[seg] = cp;
+                    // the previous segment ends where this one starts
+                    if (t > 0u) {
+                        P.seg_count[prev_key] = kp;
+                        P.seg_ccount[prev_key] = cp;
+                    }
+                }
+                if (t + 1u == n) {
+                    P.seg_count[seg] = kp + (keep ? 1u : 0u);
+                    P.seg_ccount[seg] = cp + w;
                 }
             }
-            if (t + 1u == n) {
-                P.seg_count[seg] = kp + (keep ? 1u : 0u);
-                P.seg_ccount[seg] = cp + (keep ? nc : 0u);
-            }
+            key_before = wave_broadcast(key[ch], kWave - 1);
+            k_run += static_cast<uint32_t>(__popcll(mk));
+            c_run += wave_broadcast(cincl, kWave - 1);
         }
     }
 }
@@ -1605,8 +1666,14 @@ hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    hipLaunchKernelGGL(k_build_write, dim3(s.build_blocks), dim3(kBlock), 0, stream, P);
+hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, uint32_t n_tips,
+                              hipStream_t stream) {
+    // pointer chasing back from every tip: as many lanes in flight as there
+    // are tips, so the walks overlap instead of queueing in a strided loop
+    (void)s;
+    const uint32_t want = (n_tips + kBlock - 1u) / kBlock;
+    const uint32_t blocks = want < 1u ? 1u : (want > (1u << 20) ? (1u << 20) : want);
+    hipLaunchKernelGGL(k_build_write, dim3(blocks), dim3(kBlock), 0, stream, P);
     return hipGetLastError();
 }
 

@@ -41,7 +41,9 @@ hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
 
 // build_tracks: tip scan, (event, step) segment table, output fill.
 hipError_t launch_build_count(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+// one lane per tip (n_tips from the control block after launch_build_count)
+hipError_t launch_build_write(const ckf_params& P, const launch_shape& s, uint32_t n_tips,
+                              hipStream_t stream);
 
 // Branch log (scorer training data): measurement links [0, n) in order into
 // caller arrays; block_n / block_off hold one count / offset per 1024 links.

@@ -200,6 +200,10 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.tip_rank.ensure(4u * size_t(A.tip_cap));
     A.tip_cprefix.ensure(4u * size_t(A.tip_cap));
     const size_t seg = size_t(ev.n_events) * A.cfg.max_track_candidates_per_track;
+    if (seg >= (size_t(1) << 31)) {
+        // build_tracks keys (event, step) segments by a 32-bit index
+        throw ckf_error(CKF_ERR_UNSUPPORTED, "n_events * max_track_candidates_per_track >= 2^31");
+    }
     A.seg_count.ensure(4u * seg);
     A.seg_ccount.ensure(4u * seg);
     A.seg_base.ensure(4u * seg);
@@ -832,7 +836,7 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
         if (A.logging) {
             hip_check(tamd::launch_label_best(P, shape, A.stream), "label best");
         }
-        hip_check(tamd::launch_build_write(P, shape, A.stream), "build write");
+        hip_check(tamd::launch_build_write(P, shape, A.h_ctl->n_tips, A.stream), "build write");
         const uint32_t* track_off = P.event_track_off;
         const uint32_t* const_off = P.event_const_off;
         ckf_device_tracks built;  // blocks the vote kernels read, released after the sync

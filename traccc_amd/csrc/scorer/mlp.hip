@@ -39,21 +39,48 @@ struct perm_map {
 
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(mlp_dev W, const float* feats, uint32_t n,
                                                        float* out) {
-    __shared__ short lds[kWavesPerBlock * kMlpLdsPerWave];
+    __shared__ mlp_vecs V;
+    mlp_fill_vecs(W, V, threadIdx.x, blockDim.x);
+    __syncthreads();
+    mlp_frags F;
+    mlp_load_frags(W, F);
     const uint32_t wave = threadIdx.x / 64u;
-    short* lds_x = lds + wave * kMlpLdsPerWave;
-    short* lds_h = lds_x + 64 * kMlpInPad;
+    const uint32_t l = __lane_id();
+    const uint32_t r16 = l % 16u;
+    const uint32_t g4 = 4u * (l / 16u);
     const uint32_t n_tiles = (n + 63u) / 64u;
     for (uint32_t t = blockIdx.x * kWavesPerBlock + wave; t < n_tiles;
          t += gridDim.x * kWavesPerBlock) {
-        const uint32_t i = t * 64u + __lane_id();
-        const bool valid = i < n;
-        float f[kMlpIn];
+        // sub-tile mt: lane l holds features g4 .. g4 + 3 of row 64 t + 16 mt + l % 16
+        float x[4][4];
+        bool valid[4];
 #pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? feats[static_cast<size_t>(i) * kMlpIn + k] : 0.f;
-        const float s = mlp_tile_logit(W, f, valid, lds_x, lds_h);
-        if (valid) out[i] = s;
-        wave_lds_sync();  // the next tile reuses the LDS area
+        for (int mt = 0; mt < 4; ++mt) {
+            const uint32_t r = t * 64u + 16u * mt + r16;
+            valid[mt] = r < n;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t k = g4 + static_cast<uint32_t>(e);
+                x[mt][e] = (valid[mt] && k < static_cast<uint32_t>(kMlpIn))
+                               ? feats[static_cast<size_t>(r) * kMlpIn + k] : 0.f;
+            }
+        }
+        // one sub-tile at a time (the next ones shift down) keeps a single
+        // MLP evaluation live, and the wave at 3 waves / SIMD
+        float mine = 0.f;
+#pragma unroll 1
+        for (int mt = 0; mt < 4; ++mt) {
+            const float s = mlp_logit16(F, V, x[0], valid[0]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                valid[q] = valid[q + 1];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[q][e] = x[q + 1][e];
+            }
+            if (l / 16u == static_cast<uint32_t>(mt)) mine = s;  // row 64 t + l
+        }
+        const uint32_t i = t * 64u + l;
+        if (i < n) out[i] = mine;
     }
 }
 

@@ -153,6 +153,100 @@ __device__ inline void mlp_tile_output(const mlp_dev& W, const float (&z2)[4][4]
     }
 }
 
+// Inference without LDS transposes.  The layers run transposed,
+// Z^T = W^T X^T, with the weights as the A operand: the D fragment of one
+// layer (rows = hidden units = the next K, columns = samples) is then the
+// B operand of the next as it stands.  A wave scores 16 samples per call;
+// lane l holds features 4 (l / 16) .. 4 (l / 16) + 3 of sample l % 16, and the
+// A fragments of W1^T / W2^T are the same registers as the B fragments of
+// W1 / W2 in mlp_tile_layers, kept for the wave's lifetime.
+struct mlp_frags {
+    bf16x4 w1[4];     // W1^T rows 16 nt .., the 16 (padded) inputs
+    bf16x4 w2[4][4];  // W2^T block (out tile nt, in tile kt)
+};
+
+// Per-block copy of the vectors a lane reads four at a time (hidden units
+// 16 nt + 4 (l / 16) .. + 3): biases, layer-3 weights, normalisation.
+struct alignas(16) mlp_vecs {
+    float b1[kMlpHidden];
+    float b2[kMlpHidden];
+    float w3[kMlpHidden];
+    float mean[kMlpInPad];
+    float inv_std[kMlpInPad];  // 0 for the padded inputs
+    float b3;
+};
+
+__device__ inline void mlp_load_frags(const mlp_dev& W, mlp_frags& F) {
+    const uint32_t l = __lane_id();
+    const uint32_t r16 = l % 16u;
+    const uint32_t g4 = 4u * (l / 16u);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        F.w1[nt] = load4(W.w1t + (16 * nt + r16) * kMlpInPad + g4);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            F.w2[nt][kt] = load4(W.w2t + (16 * nt + r16) * kMlpHidden + 16 * kt + g4);
+        }
+    }
+}
+
+// Fill V from the block's threads (tid of n); a block barrier must follow.
+__device__ inline void mlp_fill_vecs(const mlp_dev& W, mlp_vecs& V, uint32_t tid, uint32_t n) {
+    for (uint32_t k = tid; k < kMlpHidden; k += n) {
+        V.b1[k] = W.b1[k];
+        V.b2[k] = W.b2[k];
+        V.w3[k] = W.w3[k];
+    }
+    for (uint32_t k = tid; k < kMlpInPad; k += n) {
+        V.mean[k] = k < kMlpIn ? W.mean[k] : 0.f;
+        V.inv_std[k] = k < kMlpIn ? W.inv_std[k] : 0.f;
+    }
+    if (tid == 0u) V.b3 = W.b3[0];
+}
+
+// Logit of sample l % 16 (returned in all four lanes of that column).  x:
+// the lane's four raw features; valid = false gives a zero row.
+__device__ inline float mlp_logit16(const mlp_frags& F, const mlp_vecs& V, const float (&x)[4],
+                                    bool valid) {
+    const uint32_t g4 = 4u * (__lane_id() / 16u);
+    // read the vectors from LDS at each use instead of holding 56 registers
+    // across the caller's loop
+    __asm__ volatile("" ::: "memory");
+    const f32x4 mu = *reinterpret_cast<const f32x4*>(&V.mean[g4]);
+    const f32x4 is = *reinterpret_cast<const f32x4*>(&V.inv_std[g4]);
+    bf16x4 xb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool real = valid && g4 + static_cast<uint32_t>(e) < static_cast<uint32_t>(kMlpIn);
+        xb[e] = f2bf(real ? (x[e] - mu[e]) * is[e] : 0.f);
+    }
+    // layer 1: H1^T tile nt = GELU(W1^T X^T + b1), straight into B fragments
+    bf16x4 h[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma16(F.w1[nt], xb, acc);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(&V.b1[16 * nt + g4]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[nt][e] = f2bf(gelu(acc[e] + b[e]));
+    }
+    // layer 2 and the layer-3 dot product over the lane's hidden units
+    float part = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(F.w2[nt][kt], h[kt], acc);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(&V.b2[16 * nt + g4]);
+        const f32x4 w = *reinterpret_cast<const f32x4*>(&V.w3[16 * nt + g4]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) part += gelu(acc[e] + b[e]) * w[e];
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + V.b3;
+}
+
 // Scores of a 64-row tile: the logit of lane l's row returned in lane l.
 __device__ inline float mlp_tile_logit(const mlp_dev& W, const float* feat, bool valid,
                                        short* lds_x, short* lds_h) {
