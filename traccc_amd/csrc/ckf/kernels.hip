@@ -334,9 +334,10 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t Lc = P.list_len;
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
         // dynamic partitions: dense surfaces make some inputs far heavier
-        const uint32_t p = wave_ticket(&ctl->ticket[7]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[7]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -497,8 +498,9 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[0]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[0]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -655,8 +657,9 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     auto input_seed = [&](uint32_t j) -> uint32_t {
         return step == 0u ? j : load_link(P, prev_base + P.live[j]).seed;
     };
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[4]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[4]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -851,8 +854,9 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
     const uint32_t base = ctl->link_base;
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n + kWave - 1) / kWave;
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[1]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[1]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -907,8 +911,9 @@ __global__ __launch_bounds__(kBlock) void k_dedup_emit(ckf_params P) {
     const uint32_t kc = emit_chunks(n);
     const uint32_t part = kWave * kc;
     const uint32_t n_parts = (n + part - 1u) / part;
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[6]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[6]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -972,8 +977,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[2]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[2]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -1055,8 +1061,9 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
     const uint32_t kc = emit_chunks(n);
     const uint32_t part = kWave * kc;
     const uint32_t n_parts = (n + part - 1u) / part;
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[5]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[5]) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -1110,8 +1117,9 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
         }
     }
     if (fused_advance) {
-        // the last wave to leave runs the step advance; every wave reaches
-        // this point once the ticket queue is empty
+        // the last wave to leave runs the step advance; every wave of the grid
+        // (ticket holder or not) counts itself once it has read the control
+        // block, so no wave can start on the advanced one
         __threadfence();
         uint32_t before = 0u;
         if (lane() == 0u) {
@@ -1180,8 +1188,9 @@ __global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
         const dlink L = load_link(P, g);
         return P.seed_event[L.seed] * S + link_step(L);
     };
+    const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[3]);
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[3]) : n_parts;
         if (p >= n_parts) {
             break;
         }

@@ -44,6 +44,21 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
 
 __device__ inline uint32_t wave_broadcast(uint32_t v, int src) { return __shfl(v, src, kWave); }
 
+// Waves of a ticketed grid that join the queue.  A grid sized for the
+// largest step holds many more waves than a small step has partitions; waves
+// past the first n_parts take no ticket (each would only fail one more atomic
+// on the shared counter).  Wave 0 always joins, so work after the queue
+// still has a wave to run it.  Partition p is only waited on by tickets after
+// it, which running waves hold, so the queue drains with any set of joiners.
+__device__ inline uint32_t ticket_waves(uint32_t n_parts) {
+    const uint32_t all = gridDim.x * (blockDim.x / kWave);
+    const uint32_t want = n_parts > 0u ? n_parts : 1u;
+    return want < all ? want : all;
+}
+__device__ inline bool wave_takes_tickets(uint32_t n_parts) {
+    return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave < ticket_waves(n_parts);
+}
+
 // Next work partition for the calling wave.
 __device__ inline uint32_t wave_ticket(uint32_t* counter) {
     uint32_t t = 0;
