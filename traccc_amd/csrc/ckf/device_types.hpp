@@ -38,6 +38,25 @@ struct dlink {
 };
 static_assert(sizeof(dlink) == 32, "link record is 32 B");
 
+// What k_find hands k_find_emit so the links are built without re-reading
+// the inputs: per input its previous link, seed, hole counts, running sums
+// and own state reference (written before the updates, so none of it stays
+// in registers through them); per filtered slot the candidate.
+struct stage_head {
+    uint32_t prev;      // global index of the input's link, kInvalid at step 0
+    uint32_t seed;
+    uint32_t hist;      // n_skipped | n_consecutive << 8
+    uint32_t hole_ref;  // state_ref of a hole link (the input's own state)
+    float chi2_sum;     // the input's running sums
+    uint32_t ndf_sum;
+    uint32_t pad[2];
+};
+struct stage_cand {
+    uint32_t meas;      // measurement | (dims == 2) << 31; kInvalid: the input's hole
+    float chi2;
+};
+static_assert(sizeof(stage_head) == 32 && sizeof(stage_cand) == 8, "stage record sizes");
+
 TAMD_HD uint32_t link_step(const dlink& l) { return l.packed & 0xffffu; }
 TAMD_HD uint32_t link_skipped(const dlink& l) { return (l.packed >> 16) & 0xffu; }
 TAMD_HD uint32_t link_consec(const dlink& l) { return (l.packed >> 24) & 0xffu; }
@@ -103,8 +122,9 @@ struct ckf_params {
     float* state[2];              // ping-pong propagated states, SoA [28][state_cap]
     uint32_t state_cap;           // links per step
     float* flt;                   // filtered states SoA [28][flt_cap], slot = input * N + k
-    uint32_t* cand_meas;          // [flt_cap] measurement of the filtered slot
-    float* cand_chi2;             // [flt_cap] its predicted chi2
+    stage_head* stage_in;         // [in_cap] the input's link bookkeeping, from k_find
+    stage_cand* stage_slot;       // [flt_cap] the filtered slot's candidate (slot input * N
+                                  // holds the hole when nothing was kept)
     uint32_t flt_cap;
     uint32_t in_cap;              // inputs per step
     dlink* links;

@@ -317,6 +317,55 @@ __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i
     in.rule0 = (step == 0u) && !(P.sfs[in.sf].thickness_x0 > 0.f);
 }
 
+// Link bookkeeping handed from k_find to k_find_emit (see stage_head).
+__device__ inline stage_head make_stage_head(const find_input& in, uint32_t step, uint32_t i) {
+    stage_head h;
+    h.prev = in.prev_g;
+    h.seed = in.seed;
+    h.hist = in.skip | (in.consec << 8);
+    h.hole_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
+                              : ((kRefInput << kRefKindShift) | in.in_slot);
+    h.chi2_sum = in.chi2_prev;
+    h.ndf_sum = in.ndf_prev;
+    h.pad[0] = h.pad[1] = 0u;
+    return h;
+}
+
+// Measurement indices stay below 2^31 (the batch's measurement count is a
+// 32-bit size and far smaller), so the top bit carries the dimension.
+__device__ inline stage_cand make_stage_cand(uint32_t m, float chi2, uint32_t dims) {
+    stage_cand c;
+    c.meas = m | (dims == 2u ? 0x80000000u : 0u);
+    c.chi2 = chi2;
+    return c;
+}
+
+// The link a staged slot becomes (a hole link carries the input's own state).
+__device__ inline dlink staged_link(const stage_head& h, const stage_cand& c, uint32_t step,
+                                    uint32_t slot) {
+    const bool hole = c.meas == kInvalid;
+    const uint32_t skip = h.hist & 0xffu, consec = h.hist >> 8;
+    dlink L;
+    L.prev = h.prev;
+    L.seed = h.seed;
+    if (hole) {
+        L.meas = kInvalid;
+        L.packed = pack_link(step, skip + 1u, consec + 1u);
+        L.chi2 = FLT_MAX;
+        L.chi2_sum = h.chi2_sum;
+        L.ndf_sum = h.ndf_sum;
+        L.state_ref = h.hole_ref;
+    } else {
+        L.meas = c.meas & 0x7fffffffu;
+        L.packed = pack_link(step, skip, 0u);
+        L.chi2 = c.chi2;
+        L.chi2_sum = h.chi2_sum + c.chi2;
+        L.ndf_sum = h.ndf_sum + ((c.meas >> 31) != 0u ? 2u : 1u);
+        L.state_ref = (kRefFiltered << kRefKindShift) | slot;
+    }
+    return L;
+}
+
 // Scorer-pruned finding, first kernel: per input the list_len best candidates
 // in (chi2, index) order (fixed stride list_len), and their features in rows
 // packed densely over the step (a wave's inputs take one block of rows), so
@@ -506,19 +555,13 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         }
         const uint32_t i = p * kPerWave + team;
         const bool valid = i < n_in;
-        uint32_t count = 0u, seed = 0u, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
-        uint32_t in_slot = 0u;
-        float chi2_prev = 0.f;
+        uint32_t count = 0u;
         if (valid) {
             find_input in;
             load_input(P, step, i, prev_base, in);
-            seed = in.seed;
-            prev_g = in.prev_g;
-            skip = in.skip;
-            consec = in.consec;
-            ndf_prev = in.ndf_prev;
-            in_slot = in.in_slot;
-            chi2_prev = in.chi2_prev;
+            if (leader) {
+                P.stage_in[i] = make_stage_head(in, step, i);
+            }
             uint32_t n_pass = 0u;
             uint32_t li = 0u;
             const uint32_t ln = kScored ? P.list_n[i] : 0u;
@@ -551,8 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                         const uint32_t slot = i * N + count;
                         if (slot < P.flt_cap) {
                             store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
-                            P.cand_meas[slot] = best_m;
-                            P.cand_chi2[slot] = best_c;
+                            P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
                         } else {
                             set_overflow(ctl, 4u);
                         }
@@ -600,11 +642,10 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                         if (leader) {
                             if (slot < P.flt_cap) {
                                 store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
-                                P.cand_meas[slot] = best_m;
-                                P.cand_chi2[slot] = best_c;
+                                P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
                                 if constexpr (logging) {
-                                    candidate_features(P, md, in.x, in.c, in.sf, step, skip, consec,
-                                                       chi2_prev, ndf_prev, best_c, count, n_pass,
+                                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
+                                                       in.chi2_prev, in.ndf_prev, best_c, count, n_pass,
                                                        P.feat_slot + static_cast<size_t>(slot) * kFeatures);
                                 }
                             } else {
@@ -613,6 +654,15 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                         }
                         ++count;
                     }
+                }
+            }
+            if (count == 0u && leader) {
+                // no candidate kept: the input continues as one hole link
+                const uint32_t slot = i * N;
+                if (slot < P.flt_cap) {
+                    P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
+                } else {
+                    set_overflow(ctl, 4u);
                 }
             }
         }
@@ -648,15 +698,12 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     const uint32_t step = ctl->step;
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t link_base = ctl->link_base;
-    const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
     const uint32_t kc = emit_chunks(n_in);
     const uint32_t part = kWave * kc;
     const uint32_t n_parts = (n_in + part - 1u) / part;
     const uint32_t N = P.n_branch_surface;
-    auto input_seed = [&](uint32_t j) -> uint32_t {
-        return step == 0u ? j : load_link(P, prev_base + P.live[j]).seed;
-    };
+    auto input_seed = [&](uint32_t j) -> uint32_t { return P.stage_in[j].seed; };
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
         const uint32_t p = takes ? wave_ticket(&ctl->ticket[4]) : n_parts;
@@ -697,25 +744,8 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
             const uint32_t incl = wave_inclusive_scan(no);
             const uint32_t first = run + incl - no;
             run += wave_broadcast(incl, kWave - 1);
-            const uint32_t count = valid ? P.find_count[i] : 0u;
-            // the input's bookkeeping from its previous link
-            uint32_t seed = kInvalid, prev_g = kInvalid, skip = 0u, consec = 0u, ndf_prev = 0u;
-            uint32_t in_slot = 0u;
-            float chi2_prev = 0.f;
-            if (valid) {
-                if (step == 0u) {
-                    seed = i;
-                } else {
-                    in_slot = P.live[i];
-                    prev_g = prev_base + in_slot;
-                    const dlink L = load_link(P, prev_g);
-                    seed = L.seed;
-                    skip = link_skipped(L);
-                    consec = link_consec(L);
-                    chi2_prev = L.chi2_sum;
-                    ndf_prev = L.ndf_sum;
-                }
-            }
+            const stage_head h = valid ? P.stage_in[i] : stage_head{};
+            const uint32_t seed = valid ? h.seed : kInvalid;
             // head of the seed's contiguous run of inputs
             uint32_t prev_seed = __shfl(seed, (lane() + kWave - 1u) % kWave, kWave);
             if (lane() == 0u) {
@@ -728,44 +758,17 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
             if (i == 0u || prev_seed != seed) {
                 P.seed_first[seed] = first;
             }
-            if (count == 0u) {
-                dlink L;
-                L.prev = prev_g;
-                L.meas = kInvalid;
-                L.seed = seed;
-                L.packed = pack_link(step, skip + 1u, consec + 1u);
-                L.chi2 = FLT_MAX;
-                L.chi2_sum = chi2_prev;
-                L.ndf_sum = ndf_prev;
-                L.state_ref = (step == 0u) ? ((kRefSeed << kRefKindShift) | i)
-                                           : ((kRefInput << kRefKindShift) | in_slot);
-                const uint32_t g = link_base + first;
-                if (g < P.link_cap && first < P.state_cap) {
-                    P.links[g] = L;
-                } else {
-                    set_overflow(ctl, 1u);
+            for (uint32_t k = 0; k < no; ++k) {
+                const uint32_t slot = i * N + k;
+                if (slot >= P.flt_cap) {
+                    break;  // k_find raised the overflow that discards this step
                 }
-            } else {
-                for (uint32_t k = 0; k < count; ++k) {
-                    const uint32_t slot = i * N + k;
-                    if (slot >= P.flt_cap) {
-                        break;
-                    }
-                    const uint32_t m = P.cand_meas[slot];
-                    const float chi2 = P.cand_chi2[slot];
-                    dlink L;
-                    L.prev = prev_g;
-                    L.meas = m;
-                    L.seed = seed;
-                    L.packed = pack_link(step, skip, 0u);
-                    L.chi2 = chi2;
-                    L.chi2_sum = chi2_prev + chi2;
-                    L.ndf_sum = ndf_prev + (P.meas_info[m] & 0xffu);
-                    L.state_ref = (kRefFiltered << kRefKindShift) | slot;
-                    const uint32_t g = link_base + first + k;
-                    if (g < P.link_cap && first + k < P.state_cap) {
-                        P.links[g] = L;
-                        if constexpr (kLogging) {
+                const dlink L = staged_link(h, P.stage_slot[slot], step, slot);
+                const uint32_t g = link_base + first + k;
+                if (g < P.link_cap && first + k < P.state_cap) {
+                    P.links[g] = L;
+                    if constexpr (kLogging) {
+                        if (L.meas != kInvalid) {
 #pragma unroll
                             for (int q = 0; q < kFeatures; ++q) {
                                 float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
@@ -775,9 +778,9 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
                                 P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
                             }
                         }
-                    } else {
-                        set_overflow(ctl, 1u);
                     }
+                } else {
+                    set_overflow(ctl, 1u);
                 }
             }
         }
@@ -1050,10 +1053,15 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
 // wave to leave runs the step advance (fused_advance) so the step loop never
 // waits on the host.
 __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_advance) {
+    __shared__ uint32_t s_exit;  // waves of this block past the queue
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
-        return;
+        return;  // the whole grid returns
     }
+    if (threadIdx.x == 0u) {
+        s_exit = 0u;
+    }
+    __syncthreads();
     const uint32_t n = links_this_step(P, ctl);
     const uint32_t base = ctl->link_base;
     const uint32_t epoch = ctl->epoch;
@@ -1117,21 +1125,30 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
         }
     }
     if (fused_advance) {
-        // the last wave to leave runs the step advance; every wave of the grid
-        // (ticket holder or not) counts itself once it has read the control
-        // block, so no wave can start on the advanced one
-        __threadfence();
-        uint32_t before = 0u;
+        // the last block to leave runs the step advance.  Every wave of the
+        // grid (ticket holder or not) counts itself in its block once it has
+        // read the control block, and the block's last wave counts the block
+        // (one device atomic per block), so no wave can start on the
+        // advanced control block.
+        __threadfence();  // release this wave's writes
+        uint32_t in_block = 0u;
         if (lane() == 0u) {
-            before = atomicAdd(&ctl->exit_count, 1u);
+            in_block = atomicAdd(&s_exit, 1u) + 1u;
         }
-        before = wave_broadcast(before, 0);
-        const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-        if (before + 1u == waves && lane() == 0u) {
-            __threadfence();  // acquire: every other wave's control-block writes
-            advance_step(P, ctl);
-            ctl->exit_count = 0u;
-            __threadfence();
+        in_block = wave_broadcast(in_block, 0);
+        if (in_block == blockDim.x / kWave) {
+            __threadfence();  // acquire the block's waves, release them on
+            uint32_t before = 0u;
+            if (lane() == 0u) {
+                before = atomicAdd(&ctl->exit_count, 1u);
+            }
+            before = wave_broadcast(before, 0);
+            if (before + 1u == gridDim.x && lane() == 0u) {
+                __threadfence();  // acquire: every other block's writes
+                advance_step(P, ctl);
+                ctl->exit_count = 0u;
+                __threadfence();
+            }
         }
     }
 }

@@ -94,7 +94,7 @@ struct ckf_algorithm {
     uint32_t in_cap = 0, state_cap = 0, flt_cap = 0, link_cap = 0, tip_cap = 0, lb_cap = 0;
     uint32_t seed_cap = 0, meas_cap = 0, seg_cap = 0, ev_cap = 0;
     // work buffers
-    dbuf state[2], flt, cand_meas, cand_chi2, links, live, tips, seed_first, seed_base;
+    dbuf state[2], flt, stage_in, stage_slot, links, live, tips, seed_first, seed_base;
     dbuf alive_prefix, dedup_head, dedup_next, lb_flag[4], lb_agg[4], lb_inc[4], ctl;
     dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
         seg_coff, ev_toff, ev_coff;
@@ -127,7 +127,7 @@ struct ckf_algorithm {
         for (auto& g : graph) {
             if (g) (void)hipGraphExecDestroy(g);
         }
-        dbuf* all[] = {&state[0], &state[1], &flt, &cand_meas, &cand_chi2, &links, &live, &tips,
+        dbuf* all[] = {&state[0], &state[1], &flt, &stage_in, &stage_slot, &links, &live, &tips,
                        &seed_first, &seed_base, &alive_prefix, &dedup_head, &dedup_next, &ctl,
                        &tip_keep, &tip_rank, &tip_cprefix, &seg_count, &seg_ccount, &seg_base,
                        &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff, &vote_slots,
@@ -166,8 +166,8 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.state[0].ensure(st * A.state_cap);
     A.state[1].ensure(st * A.state_cap);
     A.flt.ensure(st * A.flt_cap);
-    A.cand_meas.ensure(4u * size_t(A.flt_cap));
-    A.cand_chi2.ensure(4u * size_t(A.flt_cap));
+    A.stage_in.ensure(sizeof(tamd::stage_head) * size_t(A.in_cap));
+    A.stage_slot.ensure(sizeof(tamd::stage_cand) * size_t(A.flt_cap));
     A.links.ensure(sizeof(tamd::dlink) * size_t(A.link_cap));
     A.live.ensure(4u * size_t(A.in_cap));
     A.find_count.ensure(4u * size_t(A.in_cap));
@@ -275,8 +275,8 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.state[1] = A.state[1].as<float>();
     P.state_cap = A.state_cap;
     P.flt = A.flt.as<float>();
-    P.cand_meas = A.cand_meas.as<uint32_t>();
-    P.cand_chi2 = A.cand_chi2.as<float>();
+    P.stage_in = A.stage_in.as<tamd::stage_head>();
+    P.stage_slot = A.stage_slot.as<tamd::stage_cand>();
     P.flt_cap = A.flt_cap;
     P.in_cap = A.in_cap;
     P.links = A.links.as<tamd::dlink>();
@@ -354,7 +354,8 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     const uint64_t rows = 2u * uint64_t(A.in_cap);
     s.score_blocks = static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>((rows / 64u + 4u) / 4u, max_blocks)));
     s.build_blocks = blocks(A.tip_cap);
-    s.emit_blocks = blocks(A.in_cap);
+    // compaction waves do little per partition: four blocks per CU
+    s.emit_blocks = std::min(blocks(A.in_cap), static_cast<uint32_t>(A.n_cu) * 4u);
     return s;
 }
 
