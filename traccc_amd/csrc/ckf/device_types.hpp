@@ -137,9 +137,8 @@ struct ckf_params {
     uint32_t* alive_prefix;       // [state_cap] bit 31 = alive, low bits = exclusive alive prefix
     uint64_t* dedup_head;         // [n_meas + 1] epoch << 32 | link (epoch-tagged list heads)
     uint32_t* dedup_next;         // [state_cap]
-    uint32_t* lb_flag[4];         // look-back descriptors per kernel
-    uint64_t* lb_agg[4];
-    uint64_t* lb_inc[4];
+    uint64_t* lb_lo[4];           // look-back words per kernel: low counter channel
+    uint64_t* lb_hi[4];           // and high counter channel
     uint32_t lb_cap;
     step_ctl* ctl;
     // build outputs

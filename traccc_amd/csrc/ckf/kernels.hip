@@ -720,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
             n_out[ch] = (static_cast<uint32_t>(ch) < kc && i < n_in) ? (count > 0u ? count : 1u) : 0u;
             total += wave_broadcast(wave_inclusive_scan(n_out[ch]), kWave - 1);
         }
-        const uint64_t base = wave_lookback(P.lb_flag[0], P.lb_agg[0], P.lb_inc[0], p, epoch,
+        const uint64_t base = wave_lookback(P.lb_lo[0], nullptr, p, epoch,
                                             pack2(total, 0u));
         if (p == n_parts - 1u && lane() == 0u) {
             ctl->n_links = lo32(base) + total;
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_emit(ckf_params P) {
             am[ch] = wave_ballot(a);
             total += static_cast<uint32_t>(__popcll(am[ch]));
         }
-        const uint64_t b = wave_lookback(P.lb_flag[1], P.lb_agg[1], P.lb_inc[1], p, epoch,
+        const uint64_t b = wave_lookback(P.lb_lo[1], nullptr, p, epoch,
                                          pack2(total, 0u));
         uint32_t run = lo32(b);
 #pragma unroll
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
             t_tot += static_cast<uint32_t>(__popcll(wave_ballot((fl[ch] >> 1) >= 1u)) +
                                            __popcll(wave_ballot((fl[ch] >> 1) >= 2u)));
         }
-        const uint64_t b = wave_lookback(P.lb_flag[2], P.lb_agg[2], P.lb_inc[2], p, epoch,
+        const uint64_t b = wave_lookback(P.lb_lo[2], P.lb_hi[2], p, epoch,
                                          pack2(s_tot, t_tot));
         uint32_t s_run = lo32(b), t_run = hi32(b);
 #pragma unroll
@@ -1119,35 +1119,43 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
             s_run += static_cast<uint32_t>(__popcll(ms));
             t_run += static_cast<uint32_t>(__popcll(m1) + __popcll(m2));
         }
-        if (p == n_parts - 1u && lane() == 0u) {
+        if (p == n_parts - 1u && lane() == 0u && !fused_advance) {
             ctl->n_live = lo32(b) + s_tot;
             ctl->n_tips_step = hi32(b) + t_tot;
         }
     }
     if (fused_advance) {
-        // the last block to leave runs the step advance.  Every wave of the
-        // grid (ticket holder or not) counts itself in its block once it has
-        // read the control block, and the block's last wave counts the block
-        // (one device atomic per block), so no wave can start on the
-        // advanced control block.
-        __threadfence();  // release this wave's writes
+        // The last block to leave runs the step advance.  Every wave of the
+        // grid (ticket holder or not) counts itself once it has read the
+        // control block -- in LDS, then the block's last wave with one device
+        // atomic -- so no wave can start on the advanced control block.  The
+        // step's totals come from the last partition's look-back words, which
+        // carry their own visibility; everything else the advance reads was
+        // written by earlier kernels, and its own stores reach the next
+        // kernel at the launch boundary.  No fence is needed.
         uint32_t in_block = 0u;
         if (lane() == 0u) {
             in_block = atomicAdd(&s_exit, 1u) + 1u;
         }
         in_block = wave_broadcast(in_block, 0);
         if (in_block == blockDim.x / kWave) {
-            __threadfence();  // acquire the block's waves, release them on
             uint32_t before = 0u;
             if (lane() == 0u) {
                 before = atomicAdd(&ctl->exit_count, 1u);
             }
             before = wave_broadcast(before, 0);
-            if (before + 1u == gridDim.x && lane() == 0u) {
-                __threadfence();  // acquire: every other block's writes
-                advance_step(P, ctl);
-                ctl->exit_count = 0u;
-                __threadfence();
+            if (before + 1u == gridDim.x) {
+                uint32_t live = 0u, tips = 0u;
+                if (n_parts > 0u) {
+                    live = lookback_inclusive(P.lb_lo[2], n_parts - 1u, epoch);
+                    tips = lookback_inclusive(P.lb_hi[2], n_parts - 1u, epoch);
+                }
+                if (lane() == 0u) {
+                    ctl->n_live = live;
+                    ctl->n_tips_step = tips;
+                    advance_step(P, ctl);
+                    ctl->exit_count = 0u;
+                }
             }
         }
     }
@@ -1232,7 +1240,7 @@ __global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
             k_tot += static_cast<uint32_t>(__popcll(wave_ballot(keep)));
             c_tot += wave_broadcast(wave_inclusive_scan(keep ? (kn[ch] & 0x7fffffffu) : 0u), kWave - 1);
         }
-        const uint64_t b = wave_lookback(P.lb_flag[3], P.lb_agg[3], P.lb_inc[3], p, epoch,
+        const uint64_t b = wave_lookback(P.lb_lo[3], P.lb_hi[3], p, epoch,
                                          pack2(k_tot, c_tot));
         uint32_t k_run = lo32(b), c_run = hi32(b);
         // segment key of the tip before this partition's first

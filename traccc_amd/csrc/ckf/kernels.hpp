@@ -69,9 +69,8 @@ struct vote_params {
     uint32_t* totals;               // [2] kept tracks, kept constituents
     uint32_t* new_track_off;        // [n_events + 1]
     uint32_t* new_const_off;        // [n_events + 1]
-    uint32_t* lb_flag;
-    uint64_t* lb_agg;
-    uint64_t* lb_inc;
+    uint64_t* lb_lo;  // look-back words (low / high counter channel)
+    uint64_t* lb_hi;
     // built track columns
     const uint16_t* outcome;
     const uint32_t* surface;
@@ -105,9 +104,8 @@ struct pkf_params {
     uint32_t* holes;     // [n_seeds]
     uint32_t* cpos;      // [n_seeds] constituent offset
     uint32_t max_states;
-    uint32_t* lb_flag;
-    uint64_t* lb_agg;
-    uint64_t* lb_inc;
+    uint64_t* lb_lo;  // look-back words (low / high counter channel)
+    uint64_t* lb_hi;
 };
 hipError_t launch_pkf(const ckf_params& P, const pkf_params& Q, uint32_t n_cu, hipStream_t stream);
 hipError_t launch_pkf_write(const ckf_params& P, const pkf_params& Q, uint32_t n_cu,

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_pkf_scan(ckf_params P, pkf_params Q)
         const uint32_t w = s < n ? Q.n_states[s] : 0u;
         const uint32_t incl = wave_inclusive_scan(w);
         const uint32_t tot = wave_broadcast(incl, kWave - 1);
-        const uint64_t b = wave_lookback(Q.lb_flag, Q.lb_agg, Q.lb_inc, p, epoch, pack2(tot, 0u));
+        const uint64_t b = wave_lookback(Q.lb_lo, nullptr, p, epoch, pack2(tot, 0u));
         if (s < n) {
             Q.cpos[s] = lo32(b) + incl - w;
         }

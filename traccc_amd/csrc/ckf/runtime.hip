@@ -95,7 +95,7 @@ struct ckf_algorithm {
     uint32_t seed_cap = 0, meas_cap = 0, seg_cap = 0, ev_cap = 0;
     // work buffers
     dbuf state[2], flt, stage_in, stage_slot, links, live, tips, seed_first, seed_base;
-    dbuf alive_prefix, dedup_head, dedup_next, lb_flag[4], lb_agg[4], lb_inc[4], ctl;
+    dbuf alive_prefix, dedup_head, dedup_next, lb_lo[4], lb_hi[4], ctl;
     dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
         seg_coff, ev_toff, ev_coff;
     // measurement votes (max_num_tracks_per_measurement > 0)
@@ -108,8 +108,7 @@ struct ckf_algorithm {
     float score_threshold = 0.f;
     uint32_t list_len = 0;
     // progressive filter (run_pkf)
-    dbuf pkf_states, pkf_n, pkf_chi2, pkf_ndf, pkf_holes, pkf_cpos, pkf_lb_flag, pkf_lb_agg,
-        pkf_lb_inc;
+    dbuf pkf_states, pkf_n, pkf_chi2, pkf_ndf, pkf_holes, pkf_cpos, pkf_lb_lo;
     tamd::step_ctl* h_ctl = nullptr;  // pinned mirror
     // captured step chunks: graph[1] with duplicate removal, graph[0] without
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -133,15 +132,14 @@ struct ckf_algorithm {
                        &seg_cbase, &seg_off, &seg_coff, &ev_toff, &ev_coff, &vote_slots,
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
-                       &pkf_lb_flag, &pkf_lb_agg, &pkf_lb_inc, &feat_slot, &log_feat,
+                       &pkf_lb_lo, &feat_slot, &log_feat,
                        &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat, &find_count,
                        &seed_run_min, &list_seed, &prop_flag,
                        &cand_keep};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < 4; ++k) {
-            lb_flag[k].release();
-            lb_agg[k].release();
-            lb_inc[k].release();
+            lb_lo[k].release();
+            lb_hi[k].release();
         }
         if (h_ctl) (void)hipHostFree(h_ctl);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -186,14 +184,14 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     }
     A.dedup_next.ensure(4u * size_t(A.state_cap));
     for (int k = 0; k < 4; ++k) {
-        const size_t old = A.lb_flag[k].bytes;
-        A.lb_flag[k].ensure(4u * size_t(A.lb_cap));
-        if (A.lb_flag[k].bytes != old) {
-            // fresh descriptors carry epoch 0, which no step uses
-            hip_check(hipMemsetAsync(A.lb_flag[k].p, 0, A.lb_flag[k].bytes, A.stream), "memset");
+        for (dbuf* w : {&A.lb_lo[k], &A.lb_hi[k]}) {
+            const size_t old = w->bytes;
+            w->ensure(8u * size_t(A.lb_cap));
+            if (w->bytes != old) {
+                // fresh descriptors carry epoch 0, which no step uses
+                hip_check(hipMemsetAsync(w->p, 0, w->bytes, A.stream), "memset");
+            }
         }
-        A.lb_agg[k].ensure(8u * size_t(A.lb_cap));
-        A.lb_inc[k].ensure(8u * size_t(A.lb_cap));
     }
     A.ctl.ensure(sizeof(tamd::step_ctl));
     A.tip_keep.ensure(4u * size_t(A.tip_cap));
@@ -292,9 +290,8 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.dedup_head = A.dedup_head.as<uint64_t>();
     P.dedup_next = A.dedup_next.as<uint32_t>();
     for (int k = 0; k < 4; ++k) {
-        P.lb_flag[k] = A.lb_flag[k].as<uint32_t>();
-        P.lb_agg[k] = A.lb_agg[k].as<uint64_t>();
-        P.lb_inc[k] = A.lb_inc[k].as<uint64_t>();
+        P.lb_lo[k] = A.lb_lo[k].as<uint64_t>();
+        P.lb_hi[k] = A.lb_hi[k].as<uint64_t>();
     }
     P.lb_cap = A.lb_cap;
     P.ctl = A.ctl.as<tamd::step_ctl>();
@@ -534,9 +531,8 @@ void run_votes(ckf_algorithm& A, const tamd::ckf_params& P, const ckf_device_eve
     V.totals = A.vote_misc.as<uint32_t>();
     V.new_track_off = V.totals + 2u;
     V.new_const_off = V.new_track_off + (ev.n_events + 1u);
-    V.lb_flag = A.lb_flag[1].as<uint32_t>();
-    V.lb_agg = A.lb_agg[1].as<uint64_t>();
-    V.lb_inc = A.lb_inc[1].as<uint64_t>();
+    V.lb_lo = A.lb_lo[1].as<uint64_t>();
+    V.lb_hi = A.lb_hi[1].as<uint64_t>();
     V.outcome = T.outcome;
     V.surface = T.surface;
     V.vec = T.vec;
@@ -626,15 +622,13 @@ ckf_device_tracks* find_tracks_pkf(ckf_algorithm& A, const ckf_device_detector& 
     A.pkf_cpos.ensure(4u * n1);
     const uint32_t lb = parts(ns) + 1u;
     {
-        const size_t old = A.pkf_lb_flag.bytes;
-        A.pkf_lb_flag.ensure(4u * size_t(lb));
-        if (A.pkf_lb_flag.bytes != old) {
+        const size_t old = A.pkf_lb_lo.bytes;
+        A.pkf_lb_lo.ensure(8u * size_t(lb));
+        if (A.pkf_lb_lo.bytes != old) {
             // fresh descriptors carry epoch 0, which no launch uses
-            hip_check(hipMemsetAsync(A.pkf_lb_flag.p, 0, A.pkf_lb_flag.bytes, A.stream), "memset");
+            hip_check(hipMemsetAsync(A.pkf_lb_lo.p, 0, A.pkf_lb_lo.bytes, A.stream), "memset");
         }
     }
-    A.pkf_lb_agg.ensure(8u * size_t(lb));
-    A.pkf_lb_inc.ensure(8u * size_t(lb));
     A.ctl.ensure(sizeof(tamd::step_ctl));
     A.ev_toff.ensure(4u * (size_t(ev.n_events) + 1u));
     A.ev_coff.ensure(4u * (size_t(ev.n_events) + 1u));
@@ -653,9 +647,8 @@ ckf_device_tracks* find_tracks_pkf(ckf_algorithm& A, const ckf_device_detector& 
     Q.holes = A.pkf_holes.as<uint32_t>();
     Q.cpos = A.pkf_cpos.as<uint32_t>();
     Q.max_states = max_states;
-    Q.lb_flag = A.pkf_lb_flag.as<uint32_t>();
-    Q.lb_agg = A.pkf_lb_agg.as<uint64_t>();
-    Q.lb_inc = A.pkf_lb_inc.as<uint64_t>();
+    Q.lb_lo = A.pkf_lb_lo.as<uint64_t>();
+    Q.lb_hi = nullptr;
 
     tamd::step_ctl init{};
     init.epoch = A.epoch_next;

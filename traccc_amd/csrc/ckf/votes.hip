@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_vote_scan(vote_params V) {
         const uint64_t mk = wave_ballot(keep != 0u);
         const uint32_t k_tot = static_cast<uint32_t>(__popcll(mk));
         const uint32_t c_tot = wave_broadcast(cincl, kWave - 1);
-        const uint64_t b = wave_lookback(V.lb_flag, V.lb_agg, V.lb_inc, p, epoch, pack2(k_tot, c_tot));
+        const uint64_t b = wave_lookback(V.lb_lo, V.lb_hi, p, epoch, pack2(k_tot, c_tot));
         if (valid) {
             V.rank[t] = lo32(b) + prefix_count(mk);
             V.cpos[t] = hi32(b) + cincl - w;

@@ -68,12 +68,6 @@ __device__ inline uint32_t wave_ticket(uint32_t* counter) {
     return wave_broadcast(t, 0);
 }
 
-__device__ inline uint32_t ld_acquire(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_release(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline uint64_t ld_relaxed64(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -88,43 +82,39 @@ __device__ inline uint64_t pack2(uint32_t lo, uint32_t hi) {
 __device__ inline uint32_t lo32(uint64_t v) { return static_cast<uint32_t>(v); }
 __device__ inline uint32_t hi32(uint64_t v) { return static_cast<uint32_t>(v >> 32); }
 
-// Exclusive prefix of partition p's packed aggregate over partitions 0..p-1.
-// Called by all lanes of the wave.  Lane 0 publishes the aggregate; then the
-// whole wave inspects the 64 preceding descriptors per round trip (lane l
-// reads partition q - 1 - l), stops at the nearest inclusive prefix, and sums
-// the aggregates in between with a wave reduction.
-__device__ inline uint64_t wave_lookback(uint32_t* flags, uint64_t* agg_slot, uint64_t* inc_slot,
-                                         uint32_t p, uint32_t epoch, uint64_t agg) {
+// Decoupled look-back descriptors: one 64-bit word per partition and
+// channel holding the value with its status, so a single relaxed device-scope
+// atomic carries both and no release / acquire fence is needed (on this GPU
+// such fences write back / invalidate the whole L2 of the XCD, which made
+// look-back kernels spend most of their time in cache maintenance):
+//   value (32 b) << 32 | (epoch mod 2^30) << 2 | status (1 aggregate,
+//   2 inclusive prefix; any other epoch reads as not ready).
+__device__ inline uint64_t lb_word(uint32_t value, uint32_t epoch, uint32_t status) {
+    return (static_cast<uint64_t>(value) << 32) |
+           static_cast<uint64_t>(((epoch & 0x3fffffffu) << 2) | status);
+}
+
+// Exclusive prefix of one channel over partitions 0..p-1 (the aggregate of
+// p already published).  The whole wave inspects the 64 preceding words per
+// round trip (lane l reads partition q - 1 - l), stops at the nearest
+// inclusive prefix, and sums the aggregates in between.
+__device__ inline uint32_t lookback_channel(uint64_t* words, uint32_t p, uint32_t epoch,
+                                            uint32_t agg) {
     const uint32_t l = lane();
-    const uint32_t tag = epoch << 2;
-    if (p == 0u) {
-        if (l == 0u) {
-            st_relaxed64(&inc_slot[0], agg);
-            st_release(&flags[0], tag | 2u);
-        }
-        return 0ull;
-    }
-    if (l == 0u) {
-        st_relaxed64(&agg_slot[p], agg);
-        st_release(&flags[p], tag | 1u);
-    }
-    uint64_t excl = 0ull;
+    const uint32_t tag = epoch & 0x3fffffffu;
+    uint32_t excl = 0u;
     int64_t window_end = static_cast<int64_t>(p);  // partitions below this are unread
     for (;;) {
         const int64_t q = window_end - 1 - static_cast<int64_t>(l);
         uint32_t status = 2u;  // lanes beyond partition 0 act as a terminating prefix of 0
-        uint64_t v = 0ull;
+        uint32_t v = 0u;
         if (q >= 0) {
-            const uint32_t f = ld_acquire(&flags[q]);
-            status = ((f >> 2) == epoch) ? (f & 3u) : 0u;
-            if (status == 2u) {
-                v = ld_relaxed64(&inc_slot[q]);
-            } else if (status == 1u) {
-                v = ld_relaxed64(&agg_slot[q]);
-            }
+            const uint64_t w = ld_relaxed64(&words[q]);
+            status = ((lo32(w) >> 2) == tag) ? (lo32(w) & 3u) : 0u;
+            v = hi32(w);
         }
         const uint64_t m_pref = wave_ballot(status == 2u);
-        const uint64_t m_ready = wave_ballot(status != 0u);
+        const uint64_t m_ready = wave_ballot(status == 1u || status == 2u);
         // lanes up to and including the nearest prefix must all be ready
         const uint32_t first_pref = m_pref ? static_cast<uint32_t>(__ffsll(static_cast<long long>(m_pref)) - 1) : 64u;
         const uint64_t need = (first_pref >= 63u) ? ~0ull : ((2ull << first_pref) - 1ull);
@@ -132,24 +122,54 @@ __device__ inline uint64_t wave_lookback(uint32_t* flags, uint64_t* agg_slot, ui
             __builtin_amdgcn_s_sleep(1);
             continue;  // re-read the same window
         }
-        uint64_t take = (l <= first_pref) ? v : 0ull;
-        uint32_t lo = lo32(take), hi = hi32(take);
+        uint32_t take = (l <= first_pref) ? v : 0u;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            lo += __shfl_xor(lo, d, kWave);
-            hi += __shfl_xor(hi, d, kWave);
-        }
-        excl += pack2(lo, hi);
+        for (int d = 32; d >= 1; d >>= 1) take += __shfl_xor(take, d, kWave);
+        excl += take;
         if (first_pref < 64u) {
             break;
         }
         window_end -= kWave;
     }
     if (l == 0u) {
-        st_relaxed64(&inc_slot[p], excl + agg);
-        st_release(&flags[p], tag | 2u);
+        st_relaxed64(&words[p], lb_word(excl + agg, epoch, 2u));
     }
     return excl;
+}
+
+// Inclusive prefix of partition q once published (status 2 of this epoch);
+// waits for it.  Used after every partition's wave has left the queue.
+__device__ inline uint32_t lookback_inclusive(const uint64_t* words, uint32_t q, uint32_t epoch) {
+    const uint32_t want = ((epoch & 0x3fffffffu) << 2) | 2u;
+    for (;;) {
+        const uint64_t w = ld_relaxed64(&words[q]);
+        if (lo32(w) == want) {
+            return hi32(w);
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Exclusive prefix of partition p's packed pair of counters (low, high) over
+// partitions 0..p-1.  Called by all lanes of the wave.  hi_words may be null
+// when the high counter is always 0.  Both aggregates are published before
+// either channel is looked back on, so no partition waits on a channel its
+// predecessor has not published.
+__device__ inline uint64_t wave_lookback(uint64_t* lo_words, uint64_t* hi_words, uint32_t p,
+                                         uint32_t epoch, uint64_t agg) {
+    const uint32_t status = p == 0u ? 2u : 1u;
+    if (lane() == 0u) {
+        st_relaxed64(&lo_words[p], lb_word(lo32(agg), epoch, status));
+        if (hi_words != nullptr) {
+            st_relaxed64(&hi_words[p], lb_word(hi32(agg), epoch, status));
+        }
+    }
+    if (p == 0u) {
+        return 0ull;
+    }
+    const uint32_t lo = lookback_channel(lo_words, p, epoch, lo32(agg));
+    const uint32_t hi = hi_words != nullptr ? lookback_channel(hi_words, p, epoch, hi32(agg)) : 0u;
+    return pack2(lo, hi);
 }
 
 }  // namespace tamd
