@@ -1422,10 +1422,8 @@ __global__ __launch_bounds__(kBlock) void k_build_write(ckf_params P) {
         const uint32_t s = L.seed;
         P.out_outcome[pos] = 0u;
         P.out_surface[pos] = P.seed_surface[s];
-        #pragma unroll
-        for (int k = 0; k < 6; ++k) P.out_vec[6u * pos + k] = P.seed_vec[6u * s + k];
-        #pragma unroll
-        for (int k = 0; k < 36; ++k) P.out_cov[36u * pos + k] = P.seed_cov[36u * s + k];
+        copy_track_params(P.seed_vec + 6u * static_cast<size_t>(s), P.seed_cov + 36u * static_cast<size_t>(s),
+                          P.out_vec + 6u * static_cast<size_t>(pos), P.out_cov + 36u * static_cast<size_t>(pos));
         P.out_ndf[pos] = ndf;
         P.out_chi2[pos] = chi2;
         P.out_pval[pos] = prob_chi2(chi2, ndf);

@@ -188,10 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_pkf_write(ckf_params P, pkf_params Q
         if (k_n > 0u) {
             // track parameters are the seed's, as in traccc without a smoother
             P.out_surface[s] = P.seed_surface[s];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) P.out_vec[6u * s + k] = P.seed_vec[6u * s + k];
-#pragma unroll
-            for (int k = 0; k < 36; ++k) P.out_cov[36u * s + k] = P.seed_cov[36u * s + k];
+            copy_track_params(P.seed_vec + 6u * static_cast<size_t>(s), P.seed_cov + 36u * static_cast<size_t>(s),
+                              P.out_vec + 6u * static_cast<size_t>(s), P.out_cov + 36u * static_cast<size_t>(s));
             const float ndf = static_cast<float>(static_cast<int>(Q.ndf[s]) - 5);
             P.out_ndf[s] = ndf;
             P.out_chi2[s] = Q.chi2[s];
@@ -200,10 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_pkf_write(ckf_params P, pkf_params Q
         } else {
             // an empty entry: default track
             P.out_surface[s] = kInvalid;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) P.out_vec[6u * s + k] = 0.f;
-#pragma unroll
-            for (int k = 0; k < 36; ++k) P.out_cov[36u * s + k] = 0.f;
+            copy_track_params(nullptr, nullptr, P.out_vec + 6u * static_cast<size_t>(s),
+                              P.out_cov + 36u * static_cast<size_t>(s));
             P.out_ndf[s] = 0.f;
             P.out_chi2[s] = 0.f;
             P.out_pval[s] = 0.f;

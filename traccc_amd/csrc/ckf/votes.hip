@@ -161,10 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_vote_write(vote_params V) {
         V.o_offsets[r] = cp;
         V.o_outcome[r] = V.outcome[t];
         V.o_surface[r] = V.surface[t];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) V.o_vec[6u * r + k] = V.vec[6u * t + k];
-#pragma unroll
-        for (int k = 0; k < 36; ++k) V.o_cov[36u * r + k] = V.cov[36u * t + k];
+        copy_track_params(V.vec + 6u * static_cast<size_t>(t), V.cov + 36u * static_cast<size_t>(t),
+                          V.o_vec + 6u * static_cast<size_t>(r), V.o_cov + 36u * static_cast<size_t>(r));
         V.o_ndf[r] = V.ndf[t];
         V.o_chi2[r] = V.chi2[t];
         V.o_pval[r] = V.pval[t];

@@ -59,6 +59,21 @@ __device__ inline bool wave_takes_tickets(uint32_t n_parts) {
     return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave < ticket_waves(n_parts);
 }
 
+// One track's 6 parameters and 6 x 6 covariance (24 B and 144 B rows, so
+// 8 / 16 B pieces stay aligned): 3 + 9 memory instructions instead of 42
+// 4-byte accesses strided across the wave.  src_cov == nullptr writes zeros.
+__device__ inline void copy_track_params(const float* src_vec, const float* src_cov,
+                                         float* dst_vec, float* dst_cov) {
+    const float2* sv = reinterpret_cast<const float2*>(src_vec);
+    const float4* sc = reinterpret_cast<const float4*>(src_cov);
+    float2* dv = reinterpret_cast<float2*>(dst_vec);
+    float4* dc = reinterpret_cast<float4*>(dst_cov);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dv[k] = src_vec != nullptr ? sv[k] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dc[k] = src_cov != nullptr ? sc[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // Next work partition for the calling wave.
 __device__ inline uint32_t wave_ticket(uint32_t* counter) {
     uint32_t t = 0;
