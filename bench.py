@@ -50,8 +50,11 @@ def kernel_times(alg, ddet, devents):
     n = C.c_uint32()
     lib.ckf_debug_kernel_times(alg._h, *[C.byref(v) for v in vals], C.byref(n))
     lib.ckf_debug_set_kernel_timing(alg._h, 0)
+    lib.ckf_debug_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    lanes = C.c_uint32()
+    lib.ckf_debug_get_schedule(alg._h, C.byref(lanes))
     return {"find": vals[0].value, "dedup": vals[1].value, "propagate": vals[2].value,
-            "advance": vals[3].value, "steps": n.value}
+            "advance": vals[3].value, "steps": n.value, "lanes": lanes.value}
 
 
 def batched_line(a, ckf, S, torch):
@@ -362,15 +365,20 @@ def main() -> int:
     alg_bytes = {"propagate": prop_bytes, "find": find_bytes}.get(dom, prop_bytes)
     avg_ms = groups[dom] / launches
     achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    # measured HBM bytes per launch of that kernel: FETCH_SIZE / WRITE_SIZE
+    # the kernels the timed group holds: the work kernel (propagation at the
+    # schedule the autotuner kept) and its compaction kernel
+    members = {"propagate": [f"k_propagate<{kt['lanes']}>", "k_prop_emit"],
+               "find": ["k_find<0>", "k_find_emit<false>"]}.get(dom, [f"k_{dom}"])
+    # measured HBM bytes per launch of those kernels: FETCH_SIZE / WRITE_SIZE
     # counter runs over this command, scaled by a 1 GiB copy (profiles/)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{a.workload}_10k_muons.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("kernels", {}).get(f"k_{dom}", {}).get(
-                "hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 2),
+            tk = json.load(fh).get("kernels", {})
+        if all(m in tk for m in members):
+            traffic = sum(tk[m]["hbm_bytes_per_launch"] for m in members)
+    roofline = {"bound": "hbm", "kernel": " + ".join(members), "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                 "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
                 "bytes_per_launch": int(alg_bytes / launches),

@@ -1,0 +1,127 @@
+"""Markdown / JSON tables from rocprofv3 result databases (rocpd SQLite).
+
+    python -m traccc_amd.rocprof_tables kernels <results.db>
+        per-kernel totals (the database's top_kernels view) with min / max
+    python -m traccc_amd.rocprof_tables calls <results.db>
+        kernel time per operator call of the last plain and the last
+        scorer-pruned CKF call in the trace (calls end at k_build_write)
+    python -m traccc_amd.rocprof_tables traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_mib>
+        HBM bytes per launch from one FETCH_SIZE and one WRITE_SIZE counter
+        run, each scaled by what the same counter reads for a device copy of
+        copy_mib MiB (read and written once), as JSON
+
+Host-side tooling for the committed profile summaries; nothing here touches
+a GPU.
+"""
+from __future__ import annotations
+
+import json
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def _short(name: str) -> str:
+    name = re.sub(r"\(anonymous namespace\)::", "", name)
+    name = re.sub(r"^void ", "", name)
+    name = re.sub(r"\(.*\)$", "", name)
+    return name.replace("tamd::", "")
+
+
+def kernels_table(db: str) -> str:
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    mm = {n: (lo, hi) for n, lo, hi in
+          cur.execute("select name, min(duration), max(duration) from kernels group by name")}
+    out = ["| kernel | calls | total us | avg us | min us | max us | % |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    for name, calls, tot, avg, pct in cur.execute(
+            "select name, total_calls, total_duration, average, percentage from top_kernels"):
+        lo, hi = mm.get(name, (0, 0))
+        out.append(f"| `{_short(name)}` | {calls} | {tot:.1f} | {avg:.1f} | {lo / 1e3:.1f} | "
+                   f"{hi / 1e3:.1f} | {pct:.2f} |")
+    return "\n".join(out)
+
+
+def calls_table(db: str) -> str:
+    con = sqlite3.connect(db)
+    rows = list(con.execute("select name, start, end from kernels order by start"))
+    ends = [i for i, r in enumerate(rows) if "k_build_write" in r[0]]
+    last = {}
+    prev = 0
+    for i in ends:
+        seg = rows[prev:i + 1]
+        prev = i + 1
+        names = {_short(r[0]) for r in seg}
+        kind = ("pruned" if "k_find_list" in names else
+                "logged" if "k_find<1>" in names else "plain")
+        last[kind] = seg
+    out = []
+    for kind in ("plain", "pruned"):
+        seg = last.get(kind)
+        if not seg:
+            continue
+        t0 = min(r[1] for r in seg if r[0].find("k_find") >= 0 or r[0].find("k_prop") >= 0)
+        agg = defaultdict(lambda: [0, 0.0])
+        for r in seg:
+            if r[1] >= t0:
+                a = agg[_short(r[0])]
+                a[0] += 1
+                a[1] += (r[2] - r[1]) / 1e6
+        total = sum(v[1] for v in agg.values())
+        out += [f"{kind} call: {(seg[-1][2] - t0) / 1e6:.2f} ms from its first step kernel to "
+                f"the end of k_build_write, {total:.2f} ms in kernels", "",
+                "| kernel | launches | ms per call |", "|---|---:|---:|"]
+        for k, v in sorted(agg.items(), key=lambda x: -x[1][1]):
+            if v[1] >= 0.01:
+                out.append(f"| `{k}` | {v[0]} | {v[1]:.3f} |")
+        out.append("")
+    return "\n".join(out)
+
+
+def _counter_means(db: str) -> dict[str, tuple[int, float]]:
+    """Per kernel (full name): dispatches and the mean counter value."""
+    con = sqlite3.connect(db)
+    acc = defaultdict(lambda: [set(), 0.0])
+    for kn, did, v in con.execute("select kernel_name, dispatch_id, value from counters_collection"):
+        a = acc[_short(kn)]
+        a[0].add(did)
+        a[1] += v
+    return {k: (len(d), s / max(len(d), 1)) for k, (d, s) in acc.items()}
+
+
+def traffic(fetch_db: str, write_db: str, copy_fetch_db: str, copy_write_db: str,
+            copy_mib: float) -> dict:
+    kib = copy_mib * 1024.0
+
+    def copy_scale(db: str) -> float:
+        m = _counter_means(db)
+        # the copy kernel is the dispatch with the largest mean
+        return kib / max(v for _, v in m.values())
+
+    fs, ws = copy_scale(copy_fetch_db), copy_scale(copy_write_db)
+    f, w = _counter_means(fetch_db), _counter_means(write_db)
+    kernels = {}
+    for k in sorted(set(f) & set(w), key=lambda k: -f[k][0]):
+        kernels[k] = {"dispatches": f[k][0], "fetch_kib_raw": round(f[k][1], 1),
+                      "write_kib_raw": round(w[k][1], 1),
+                      "hbm_bytes_per_launch": int(1024 * (fs * f[k][1] + ws * w[k][1]))}
+    return {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "kernels": kernels}
+
+
+def main(argv: list[str]) -> int:
+    if len(argv) >= 2 and argv[0] == "kernels":
+        print(kernels_table(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "calls":
+        print(calls_table(argv[1]))
+    elif len(argv) >= 6 and argv[0] == "traffic":
+        print(json.dumps(traffic(*argv[1:5], float(argv[5])), indent=1))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
