@@ -65,8 +65,22 @@ TAMD_HD uint32_t pack_link(uint32_t step, uint32_t skipped, uint32_t consec) {
            ((consec > 255u ? 255u : consec) << 24);
 }
 
+// A counter alone on its 128-byte cache line.  The work-queue counters take
+// an atomic per partition from every wave of a grid; counters sharing a line
+// (with each other, or with the fields every wave reads at kernel start) are
+// served one after another by that line's L2 channel.
+struct alignas(128) line_u32 {
+    uint32_t v;
+};
+struct alignas(128) line_u64 {
+    uint64_t v;
+};
+static_assert(sizeof(line_u32) == 128 && sizeof(line_u64) == 128, "one counter per line");
+
 // Step control block in device memory; the step loop reads and advances it
-// without host round trips.
+// without host round trips.  The fields every wave reads at kernel start
+// share the first line; each counter that waves hammer with atomics has its
+// own.
 struct step_ctl {
     uint32_t step;        // current CKF step
     uint32_t epoch;       // tag of this step's look-back descriptors
@@ -76,19 +90,19 @@ struct step_ctl {
     uint32_t n_live;      // inputs of the next step (written by propagate)
     uint32_t n_tips;      // tips recorded before this step
     uint32_t n_tips_step; // tips recorded this step (written by propagate)
-    uint32_t ticket[8];   // work-queue tickets: find, dedup, propagate, build, find emit,
-                          // propagate emit, dedup emit, scorer list
     uint32_t overflow;    // bit 0 links, bit 1 per-step states, bit 2 filtered, bit 3 tips, bit 4 inputs
     uint32_t done;
     uint32_t n_steps_run;
     uint32_t n_tracks;
     uint32_t n_constituents;
     uint32_t prev_link_base;  // global index of the previous step's first link
-    uint32_t exit_count;      // waves of k_propagate that have finished this step
-    uint32_t n_rows;          // scorer rows listed this step (written by k_find_list)
     uint64_t n_links_total;
-    uint64_t n_props;
     uint64_t n_scored;        // scorer rows of the finished steps
+    line_u32 ticket[8];   // work-queue tickets: find, dedup, propagate, build, find emit,
+                          // propagate emit, dedup emit, scorer list
+    line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
+    line_u32 n_rows;      // scorer rows listed this step (written by k_find_list)
+    line_u64 n_props;     // propagations so far
 };
 
 struct ckf_params {

@@ -383,10 +383,10 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t Lc = P.list_len;
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
-    const bool takes = wave_takes_tickets(n_parts);
+    // dynamic partitions (dense surfaces make some inputs far heavier)
+    ticket_run queue(&ctl->ticket[7].v, n_parts);
     for (;;) {
-        // dynamic partitions: dense surfaces make some inputs far heavier
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[7]) : n_parts;
+        const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         const uint32_t total = wave_broadcast(incl, kWave - 1);
         uint32_t base = 0u;
         if (lane() == 0u && total != 0u) {
-            base = atomicAdd(&ctl->n_rows, total);
+            base = atomicAdd(&ctl->n_rows.v, total);
         }
         base = wave_broadcast(base, 0);
         const uint32_t row0 = __shfl(base + incl - cnt, static_cast<int>(lane() & ~(kTeam - 1u)), kWave);
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
     mlp_load_frags(P.scorer, F);
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t cap = n_in * P.list_len;
-    const uint32_t rows = ctl->n_rows < cap ? ctl->n_rows : cap;
+    const uint32_t rows = ctl->n_rows.v < cap ? ctl->n_rows.v : cap;
     const uint32_t epoch = ctl->epoch;
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t l = lane();
@@ -547,9 +547,9 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
-    const bool takes = wave_takes_tickets(n_parts);
+    ticket_run queue(&ctl->ticket[0].v, n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[0]) : n_parts;
+        const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     auto input_seed = [&](uint32_t j) -> uint32_t { return P.stage_in[j].seed; };
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[4]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[4].v) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
     const uint32_t n_parts = (n + kWave - 1) / kWave;
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[1]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[1].v) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -916,7 +916,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_emit(ckf_params P) {
     const uint32_t n_parts = (n + part - 1u) / part;
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[6]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[6].v) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -982,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
 
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[2]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[2].v) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -1042,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
         }
         const uint64_t mp = wave_ballot(leader && propagated);
         if (lane() == 0u && mp != 0ull) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props),
+            atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props.v),
                       static_cast<unsigned long long>(__popcll(mp)));
         }
     }
@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
     const uint32_t n_parts = (n + part - 1u) / part;
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[5]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[5].v) : n_parts;
         if (p >= n_parts) {
             break;
         }
@@ -1141,7 +1141,7 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
         if (in_block == blockDim.x / kWave) {
             uint32_t before = 0u;
             if (lane() == 0u) {
-                before = atomicAdd(&ctl->exit_count, 1u);
+                before = atomicAdd(&ctl->exit_count.v, 1u);
             }
             before = wave_broadcast(before, 0);
             if (before + 1u == gridDim.x) {
@@ -1154,7 +1154,7 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
                     ctl->n_live = live;
                     ctl->n_tips_step = tips;
                     advance_step(P, ctl);
-                    ctl->exit_count = 0u;
+                    ctl->exit_count.v = 0u;
                 }
             }
         }
@@ -1174,12 +1174,12 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->n_in = ctl->n_live;
     ctl->n_live = 0u;
     ctl->n_links = 0u;
-    ctl->n_scored += ctl->n_rows;
-    ctl->n_rows = 0u;
+    ctl->n_scored += ctl->n_rows.v;
+    ctl->n_rows.v = 0u;
     ctl->step += 1u;
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
-    for (int k = 0; k < 8; ++k) ctl->ticket[k] = 0u;
+    for (int k = 0; k < 8; ++k) ctl->ticket[k].v = 0u;
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }
@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) void k_build_scan(ckf_params P) {
     };
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[3]) : n_parts;
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[3].v) : n_parts;
         if (p >= n_parts) {
             break;
         }

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_pkf(ckf_params P, pkf_params Q) {
     const uint32_t n = P.n_seeds;
     const uint32_t n_parts = (n + kWave - 1) / kWave;
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[0]);
+        const uint32_t p = wave_ticket(&ctl->ticket[0].v);
         if (p >= n_parts) {
             break;
         }
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_pkf_scan(ckf_params P, pkf_params Q)
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n + kWave - 1) / kWave;
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[1]);
+        const uint32_t p = wave_ticket(&ctl->ticket[1].v);
         if (p >= n_parts) {
             break;
         }

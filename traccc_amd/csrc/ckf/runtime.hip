@@ -497,7 +497,7 @@ void run_votes(ckf_algorithm& A, const tamd::ckf_params& P, const ckf_device_eve
     A.vote_cpos.ensure(4u * size_t(nt));
     A.vote_misc.ensure(4u * (2u + 2u * (size_t(ev.n_events) + 1u)));
     hip_check(hipMemsetAsync(A.vote_slots.p, 0, slot_bytes, A.stream), "memset");
-    hip_check(hipMemsetAsync(&P.ctl->ticket[1], 0, 4u, A.stream), "memset");
+    hip_check(hipMemsetAsync(&P.ctl->ticket[1].v, 0, 4u, A.stream), "memset");
 
     const size_t n1 = std::max<size_t>(nt, 1);
     const size_t sizes[10] = {2u * n1, 4u * n1, 24u * n1, 144u * n1, 4u * n1, 4u * n1,
@@ -769,7 +769,7 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
     const tamd::step_ctl done = *A.h_ctl;
     A.stats.n_steps = done.n_steps_run;
     A.stats.n_links = done.n_links_total;
-    A.stats.n_propagations = done.n_props;
+    A.stats.n_propagations = done.n_props.v;
     A.stats.n_tips = done.n_tips;
     A.stats.n_scored = done.n_scored;
 
@@ -781,7 +781,7 @@ ckf_device_tracks* find_tracks(ckf_algorithm& A, const ckf_device_detector& D,
     const size_t seg = size_t(ev.n_events) * c.max_track_candidates_per_track;
     hip_check(hipMemsetAsync(P.seg_count, 0, 4u * seg, A.stream), "memset");
     hip_check(hipMemsetAsync(P.seg_ccount, 0, 4u * seg, A.stream), "memset");
-    hip_check(hipMemsetAsync(&P.ctl->ticket[3], 0, 4u, A.stream), "memset");
+    hip_check(hipMemsetAsync(&P.ctl->ticket[3].v, 0, 4u, A.stream), "memset");
     hip_check(tamd::launch_build_count(P, shape, A.stream), "build count");
     hip_check(hipMemcpyAsync(A.h_ctl, P.ctl, sizeof(tamd::step_ctl), hipMemcpyDeviceToHost,
                              A.stream),

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_vote_scan(vote_params V) {
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n + kWave - 1) / kWave;
     for (;;) {
-        const uint32_t p = wave_ticket(&ctl->ticket[1]);
+        const uint32_t p = wave_ticket(&ctl->ticket[1].v);
         if (p >= n_parts) {
             break;
         }

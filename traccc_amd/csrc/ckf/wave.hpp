@@ -83,6 +83,40 @@ __device__ inline uint32_t wave_ticket(uint32_t* counter) {
     return wave_broadcast(t, 0);
 }
 
+// Partitions handed out several per ticket: consecutive partitions (the
+// branches of neighbouring seeds, which read the same surfaces) stay on one
+// wave, and a queue of millions of small partitions takes a quarter of the
+// atomics on its counter.  Small queues keep one partition per ticket so
+// every wave still gets work.  next() returns the wave's next partition
+// (n_parts when the queue is empty).
+struct ticket_run {
+    uint32_t* counter;
+    uint32_t n_parts;
+    uint32_t per;
+    bool takes;
+    uint32_t base = 0u;
+    uint32_t sub = 0u;
+    __device__ ticket_run(uint32_t* c, uint32_t n)
+        : counter(c), n_parts(n), per(per_ticket(n)), takes(wave_takes_tickets((n + per - 1u) / per)) {
+        sub = per;  // no ticket held
+    }
+    // one more partition per ticket for every 4,096 partitions, up to four
+    __device__ static uint32_t per_ticket(uint32_t n) {
+        const uint32_t q = n / 4096u;
+        return q < 1u ? 1u : (q > 4u ? 4u : q);
+    }
+    __device__ uint32_t next() {
+        if (sub == per) {
+            const uint32_t n_tickets = (n_parts + per - 1u) / per;
+            base = takes ? wave_ticket(counter) * per : n_tickets * per;
+            sub = 0u;
+        }
+        const uint32_t p = base + sub;
+        ++sub;
+        return p < n_parts ? p : n_parts;
+    }
+};
+
 __device__ inline uint64_t ld_relaxed64(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
