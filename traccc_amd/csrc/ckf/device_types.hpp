@@ -17,6 +17,9 @@
 namespace tamd {
 
 constexpr int kStateCols = 28;  // 6 + 21 + 1
+constexpr int kTickets = 9;     // work queues of the control block
+constexpr int kLookbacks = 5;   // compactions with their own look-back words:
+                                // find emit, dedup, propagate emit, build, list emit
 constexpr uint32_t kInvalid = 0xffffffffu;
 
 // state_ref encoding: where a link's (filtered) state lives
@@ -98,10 +101,10 @@ struct step_ctl {
     uint32_t prev_link_base;  // global index of the previous step's first link
     uint64_t n_links_total;
     uint64_t n_scored;        // scorer rows of the finished steps
-    line_u32 ticket[8];   // work-queue tickets: find, dedup, propagate, build, find emit,
-                          // propagate emit, dedup emit, scorer list
+    line_u32 ticket[kTickets];  // work-queue tickets: find, dedup, propagate, build, find
+                                // emit, propagate emit, dedup emit, scorer list, list emit
     line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
-    line_u32 n_rows;      // scorer rows listed this step (written by k_find_list)
+    line_u32 n_rows;      // scorer rows listed this step (written by k_list_emit)
     line_u64 n_props;     // propagations so far
 };
 
@@ -151,8 +154,8 @@ struct ckf_params {
     uint32_t* alive_prefix;       // [state_cap] bit 31 = alive, low bits = exclusive alive prefix
     uint64_t* dedup_head;         // [n_meas + 1] epoch << 32 | link (epoch-tagged list heads)
     uint32_t* dedup_next;         // [state_cap]
-    uint64_t* lb_lo[4];           // look-back words per kernel: low counter channel
-    uint64_t* lb_hi[4];           // and high counter channel
+    uint64_t* lb_lo[kLookbacks];  // look-back words per compaction: low counter channel
+    uint64_t* lb_hi[kLookbacks];  // and high counter channel
     uint32_t lb_cap;
     step_ctl* ctl;
     // build outputs
@@ -198,11 +201,12 @@ struct ckf_params {
     uint32_t* find_count;         // [in_cap] branches each input keeps this step
     uint8_t* prop_flag;           // [state_cap] propagation outcome: bit 0 next input, bits 1-2 tips
     uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
-    uint32_t* list_seed;          // [rows] seed of each scorer row
+    uint32_t* list_seed;          // [in_cap] seed of each listed input
     uint32_t* list_n;             // [in_cap] entries
-    uint32_t* list_row;           // [in_cap] first feature / verdict row of the input
-    float* list_feat;             // [rows][12], rows packed over the step
-    uint8_t* cand_keep;           // [rows] scorer verdict
+    uint32_t* row_slot;           // [in_cap * list_len] list slot (input * list_len + q) of each
+                                  // scorer row, rows packed over the step by k_list_emit
+    float* list_feat;             // [in_cap][list_len][12] features of each listed candidate
+    uint8_t* cand_keep;           // [in_cap][list_len] scorer verdict
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
 };

@@ -176,7 +176,7 @@ struct cand_stream {
 };
 
 template <typename Stream>
-__device__ inline void stream_init(const ckf_params& P, const find_input& in, uint32_t tl,
+__device__ __forceinline__ void stream_init(const ckf_params& P, const find_input& in, uint32_t tl,
                                    Stream& S) {
     constexpr int kLaneKeep = Stream::kKeep;
     const float inf = __int_as_float(0x7f800000);
@@ -243,7 +243,7 @@ __device__ inline void stream_init(const ckf_params& P, const find_input& in, ui
 }
 
 template <typename Stream>
-__device__ inline void stream_next(const ckf_params& P, const find_input& in, uint32_t tl,
+__device__ __forceinline__ void stream_next(const ckf_params& P, const find_input& in, uint32_t tl,
                                    Stream& S, float& best_c, uint32_t& best_m) {
     constexpr int kLaneKeep = Stream::kKeep;
     if (S.fallback) {
@@ -366,10 +366,22 @@ __device__ inline dlink staged_link(const stage_head& h, const stage_cand& c, ui
     return L;
 }
 
+// Compaction partitions of the emit kernels: a wave takes kEmitChunks
+// consecutive 64-item chunks per ticket, counts them, runs one decoupled
+// look-back for the whole partition, then writes.  Large partitions keep the
+// look-back chain short (its round trips, not the bytes, bound these kernels).
+constexpr uint32_t kEmitChunks = 16;
+
+// Chunks per partition for n items: as many as keep >= 2048 partitions
+// (every SIMD busy), at most kEmitChunks.
+__device__ inline uint32_t emit_chunks(uint32_t n) {
+    const uint32_t c = n / (kWave * 2048u);
+    return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
+}
+
 // Scorer-pruned finding, first kernel: per input the list_len best candidates
-// in (chi2, index) order (fixed stride list_len), and their features in rows
-// packed densely over the step (a wave's inputs take one block of rows), so
-// the MLP runs on full 64-row tiles.
+// in (chi2, index) order and their features, at the input's fixed slots
+// input * list_len + q; k_list_emit then packs the rows for the MLP.
 __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         float sc[kListSlots];
         if (active) {
             load_input(P, step, i, prev_base, in);
-            cand_stream<8> cs;  // a list runs deeper than the kept branches
+            cand_stream<4> cs;
             stream_init(P, in, tl, cs);
             n_pass = cs.n_pass;
             for (; k < Lc; ++k) {
@@ -419,22 +431,13 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                 }
             }
         }
-        // rows of this wave's inputs: one atomic per wave, leaders' prefix
-        const uint32_t cnt = (active && tl == 0u) ? k : 0u;
-        const uint32_t incl = wave_inclusive_scan(cnt);
-        const uint32_t total = wave_broadcast(incl, kWave - 1);
-        uint32_t base = 0u;
-        if (lane() == 0u && total != 0u) {
-            base = atomicAdd(&ctl->n_rows.v, total);
-        }
-        base = wave_broadcast(base, 0);
-        const uint32_t row0 = __shfl(base + incl - cnt, static_cast<int>(lane() & ~(kTeam - 1u)), kWave);
         if (active) {
             if (tl == 0u) {
                 P.list_n[i] = k;
-                P.list_row[i] = row0;
+                P.list_seed[i] = in.seed;
             }
-            // the team's lanes write the list and the features of their candidates
+            // the team's lanes write the list and the features of their
+            // candidates, at the input's fixed slots i * list_len + q
 #pragma unroll
             for (int j = 0; j < kListSlots; ++j) {
                 const uint32_t q = static_cast<uint32_t>(j * kTeam) + tl;
@@ -443,16 +446,66 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                     P.list_m[r] = sm[j];
                     P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
-                    float* f = P.list_feat + static_cast<size_t>(row0 + q) * kFeatures;
                     candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass, f);
-                    P.list_seed[row0 + q] = in.seed;
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
+                                       P.list_feat + r * kFeatures);
                     if (q == 0u) {
                         standing_offer(P.seed_run_min, in.seed, epoch,
                                        branch_run(in.chi2_prev, in.ndf_prev, sc[j], md.dims));
                     }
                 }
             }
+        }
+    }
+}
+
+// Scorer rows of the step: every input's listed candidates laid out densely
+// in input order (one look-back per partition of up to 1,024 inputs) as row
+// -> list slot, so k_score runs on full 64-row tiles.  Kept apart from
+// k_find_list so no wave of the list waits on a shared row counter.
+__global__ __launch_bounds__(kBlock) void k_list_emit(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t Lc = P.list_len;
+    const uint32_t kc = emit_chunks(n_in);
+    const uint32_t part = kWave * kc;
+    const uint32_t n_parts = (n_in + part - 1u) / part;
+    const bool takes = wave_takes_tickets(n_parts);
+    for (;;) {
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[8].v) : n_parts;
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i0 = p * part;
+        uint32_t cnt[kEmitChunks];
+        uint32_t total = 0u;
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            cnt[ch] = (static_cast<uint32_t>(ch) < kc && i < n_in) ? P.list_n[i] : 0u;
+            total += wave_broadcast(wave_inclusive_scan(cnt[ch]), kWave - 1);
+        }
+        const uint64_t b = wave_lookback(P.lb_lo[4], nullptr, p, epoch, pack2(total, 0u));
+        if (p == n_parts - 1u && lane() == 0u) {
+            ctl->n_rows.v = lo32(b) + total;
+        }
+        uint32_t run = lo32(b);
+#pragma unroll
+        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
+            if (static_cast<uint32_t>(ch) >= kc) {
+                continue;  // kc is wave-uniform
+            }
+            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
+            const uint32_t incl = wave_inclusive_scan(cnt[ch]);
+            const uint32_t first = run + incl - cnt[ch];
+            for (uint32_t q = 0; q < cnt[ch]; ++q) {
+                P.row_slot[first + q] = i * Lc + q;
+            }
+            run += wave_broadcast(incl, kWave - 1);
         }
     }
 }
@@ -473,6 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
     const uint32_t cap = n_in * P.list_len;
     const uint32_t rows = ctl->n_rows.v < cap ? ctl->n_rows.v : cap;
     const uint32_t epoch = ctl->epoch;
+    const uint32_t Lc = P.list_len;
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t l = lane();
     const uint32_t r16 = l % 16u;
@@ -486,15 +540,16 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
         for (int mt = 0; mt < 4; ++mt) {
             const uint32_t r = t * 64u + 16u * mt + r16;
             valid[mt] = r < rows;
+            const uint32_t slot = valid[mt] ? P.row_slot[r] : 0u;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const uint32_t k = 4u * grp + static_cast<uint32_t>(e);
                 x[mt][e] = (valid[mt] && k < static_cast<uint32_t>(kMlpIn))
-                               ? P.list_feat[static_cast<size_t>(r) * kFeatures + k] : 0.f;
+                               ? P.list_feat[static_cast<size_t>(slot) * kFeatures + k] : 0.f;
             }
             if (valid[mt] && grp == static_cast<uint32_t>(kFeatStanding / 4)) {
                 float& f = x[mt][kFeatStanding % 4];
-                f = standing_of(P.seed_run_min, P.list_seed[r], epoch, f);
+                f = standing_of(P.seed_run_min, P.list_seed[slot / Lc], epoch, f);
             }
         }
         // one sub-tile at a time (the next ones shift down) keeps a single
@@ -513,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
         }
         const uint32_t r = t * 64u + l;
         if (r < rows) {
-            P.cand_keep[r] = mine >= P.score_threshold ? 1u : 0u;
+            P.cand_keep[P.row_slot[r]] = mine >= P.score_threshold ? 1u : 0u;
         }
     }
 }
@@ -565,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             uint32_t n_pass = 0u;
             uint32_t li = 0u;
             const uint32_t ln = kScored ? P.list_n[i] : 0u;
-            const uint32_t row0 = kScored ? P.list_row[i] : 0u;
+            const uint32_t row0 = kScored ? i * P.list_len : 0u;  // the input's list slots
             if constexpr (kScored) {
                 // one lane per input: the kept list entries in order
                 while (count < N) {
@@ -670,19 +725,6 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             P.find_count[i] = count;
         }
     }
-}
-
-// Compaction partitions of the emit kernels: a wave takes kEmitChunks
-// consecutive 64-item chunks per ticket, counts them, runs one decoupled
-// look-back for the whole partition, then writes.  Large partitions keep the
-// look-back chain short (its round trips, not the bytes, bound these kernels).
-constexpr uint32_t kEmitChunks = 16;
-
-// Chunks per partition for n items: as many as keep >= 2048 partitions
-// (every SIMD busy), at most kEmitChunks.
-__device__ inline uint32_t emit_chunks(uint32_t n) {
-    const uint32_t c = n / (kWave * 2048u);
-    return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
 }
 
 // Links of the step, after k_find: the inputs' link counts (kept branches, or
@@ -1179,7 +1221,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->step += 1u;
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
-    for (int k = 0; k < 8; ++k) ctl->ticket[k].v = 0u;
+    for (int k = 0; k < kTickets; ++k) ctl->ticket[k].v = 0u;
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }
@@ -1557,6 +1599,7 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.scored) {
         hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_list_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
     }
     return hipGetLastError();
 }

@@ -95,13 +95,13 @@ struct ckf_algorithm {
     uint32_t seed_cap = 0, meas_cap = 0, seg_cap = 0, ev_cap = 0;
     // work buffers
     dbuf state[2], flt, stage_in, stage_slot, links, live, tips, seed_first, seed_base;
-    dbuf alive_prefix, dedup_head, dedup_next, lb_lo[4], lb_hi[4], ctl;
+    dbuf alive_prefix, dedup_head, dedup_next, lb_lo[tamd::kLookbacks], lb_hi[tamd::kLookbacks], ctl;
     dbuf tip_keep, tip_rank, tip_cprefix, seg_count, seg_ccount, seg_base, seg_cbase, seg_off,
         seg_coff, ev_toff, ev_coff;
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
-    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row, list_feat,
+    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, row_slot, list_feat,
         cand_keep, find_count, seed_run_min, list_seed, prop_flag;
     bool logging = false;
     const ckf_scorer* scorer = nullptr;
@@ -133,11 +133,11 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_lo, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &list_feat, &find_count,
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &row_slot, &list_feat, &find_count,
                        &seed_run_min, &list_seed, &prop_flag,
                        &cand_keep};
         for (dbuf* b : all) b->release();
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < tamd::kLookbacks; ++k) {
             lb_lo[k].release();
             lb_hi[k].release();
         }
@@ -183,7 +183,7 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         }
     }
     A.dedup_next.ensure(4u * size_t(A.state_cap));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < tamd::kLookbacks; ++k) {
         for (dbuf* w : {&A.lb_lo[k], &A.lb_hi[k]}) {
             const size_t old = w->bytes;
             w->ensure(8u * size_t(A.lb_cap));
@@ -224,13 +224,13 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     if (A.scorer != nullptr) {
         A.list_len = std::min(A.cfg.max_num_branches_per_surface + 4u, 16u);
         const size_t rows = size_t(A.in_cap) * A.list_len;
-        A.list_seed.ensure(4u * rows);
+        A.list_seed.ensure(4u * size_t(A.in_cap));
+        A.row_slot.ensure(4u * rows);
         A.list_m.ensure(4u * rows);
         A.list_c.ensure(4u * rows);
         A.list_feat.ensure(4u * 12u * rows);
         A.cand_keep.ensure(rows);
         A.list_n.ensure(4u * size_t(A.in_cap));
-        A.list_row.ensure(4u * size_t(A.in_cap));
     }
     if (A.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
@@ -289,7 +289,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.alive_prefix = A.alive_prefix.as<uint32_t>();
     P.dedup_head = A.dedup_head.as<uint64_t>();
     P.dedup_next = A.dedup_next.as<uint32_t>();
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < tamd::kLookbacks; ++k) {
         P.lb_lo[k] = A.lb_lo[k].as<uint64_t>();
         P.lb_hi[k] = A.lb_hi[k].as<uint64_t>();
     }
@@ -324,7 +324,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
         P.list_m = A.list_m.as<uint32_t>();
         P.list_c = A.list_c.as<float>();
         P.list_n = A.list_n.as<uint32_t>();
-        P.list_row = A.list_row.as<uint32_t>();
+        P.row_slot = A.row_slot.as<uint32_t>();
         P.list_feat = A.list_feat.as<float>();
         P.cand_keep = A.cand_keep.as<uint8_t>();
         P.scorer = tamd::scorer_view(A.scorer);
