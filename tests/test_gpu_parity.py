@@ -260,6 +260,44 @@ def test_edge_cases_empty_and_regrow():
     assert np.array_equal(tc.constituents, ref["constituents"])
 
 
+def _reverse_within_surfaces(ev):
+    """The event with each surface's measurements in reverse order (still
+    grouped by surface, no longer sorted by loc0) and the map new -> old."""
+    import copy
+    sl = np.asarray(ev.surface_link)
+    perm = np.arange(len(sl))
+    starts = np.flatnonzero(np.r_[True, sl[1:] != sl[:-1]])
+    ends = np.r_[starts[1:], len(sl)]
+    for a, b in zip(starts, ends):
+        perm[a:b] = perm[a:b][::-1]
+    out = copy.copy(ev)
+    for name in ("local_position", "local_variance", "dimensions", "time", "diameter",
+                 "identifier", "surface_link", "subspace", "cluster_index", "meas_particle"):
+        setattr(out, name, np.ascontiguousarray(np.asarray(getattr(ev, name))[perm]))
+    return out, perm
+
+
+def test_window_search_matches_full_scan():
+    """The loc0 window search (taken when every measurement measures loc0 and
+    each surface is sorted by it) finds the same tracks as the full scan
+    over each surface (taken when a surface is not sorted): the same event
+    with every surface's measurements reversed gives the same candidates,
+    mapped back, and still matches the oracle."""
+    det, fld, gun = S.workload("C4", n_particles=3000, seed=21)
+    ev = S.generate_event(det, fld, gun)
+    cfg = S.finding_config_for("C4")
+    cfg.max_num_branches_per_surface = 3
+    rev, perm = _reverse_within_surfaces(ev)
+    a = run_finder("gpu", cfg, det, fld, ev)
+    b = run_finder("gpu", cfg, det, fld, rev)
+    assert len(a["ndf"]) > 1000
+    assert np.array_equal(a["offsets"], b["offsets"])
+    assert np.array_equal(a["constituents"], perm[np.asarray(b["constituents"])])
+    assert np.array_equal(a["chi2"], b["chi2"])
+    match, _, _, _ = _chain_compare(cfg, det, fld, rev, "C4 reversed surfaces")
+    assert match >= 0.995
+
+
 def test_unsorted_measurements_rejected():
     det, fld, gun = S.workload("C1")
     ev = S.generate_event(det, fld, gun)

@@ -53,10 +53,26 @@ __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
+// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2),
+// t = 1 / (1 + p |x|), odd in x; absolute error <= 1.5e-7 in exact
+// arithmetic, 5.4e-7 evaluated in FP32 here -- far below the BF16 rounding of
+// the activations.  Branch-free (one reciprocal, one
+// exponential): the scorer's 128 GELUs per sample are most of its VALU work,
+// and a library erf's range branches diverge across the wave.
+__device__ inline float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.f + 0.3275911f * ax);
+    const float poly =
+        ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+         0.254829592f) * t;
+    return copysignf(1.f - poly * __expf(-ax * ax), x);
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.f + erf_as(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float z) {
-    return 0.5f * (1.f + erff(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * expf(-0.5f * z * z);
+    return 0.5f * (1.f + erf_as(z * 0.70710678118654752f)) +
+           z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
 // Wave-internal LDS hand-over: every lane's earlier LDS writes are visible to
