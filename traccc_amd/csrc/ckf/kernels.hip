@@ -372,10 +372,12 @@ __device__ inline dlink staged_link(const stage_head& h, const stage_cand& c, ui
 // look-back chain short (its round trips, not the bytes, bound these kernels).
 constexpr uint32_t kEmitChunks = 16;
 
-// Chunks per partition for n items: as many as keep >= 2048 partitions
-// (every SIMD busy), at most kEmitChunks.
+// Chunks per partition for n items: one per 64 chunks of work, at most
+// kEmitChunks -- a look-back round trip covers 64 partitions, so the chain
+// any partition waits on stays at a round or two, while small steps still
+// spread over many waves.
 __device__ inline uint32_t emit_chunks(uint32_t n) {
-    const uint32_t c = n / (kWave * 2048u);
+    const uint32_t c = n / (kWave * 64u);
     return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
 }
 
