@@ -289,6 +289,9 @@ __device__ __forceinline__ void stream_next(const ckf_params& P, const find_inpu
     }
 }
 
+// kRanges = false (the scored update, which scans nothing) skips the
+// event's measurement range and window lookups.
+template <bool kRanges = true>
 __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i,
                                   uint32_t prev_base, find_input& in) {
     const uint32_t in_par = (step + 1u) & 1u;
@@ -309,11 +312,15 @@ __device__ inline void load_input(const ckf_params& P, uint32_t step, uint32_t i
         in.ndf_prev = L.ndf_sum;
         load_soa(P.state[in_par], P.state_cap, in.in_slot, in.x, in.c, in.sf);
     }
-    const uint32_t ev = P.seed_event[in.seed];
-    const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
-    in.lo = rg[in.sf];
-    in.up = rg[in.sf + 1u];
-    in.win_var = P.event_win_var[ev];
+    in.lo = in.up = 0u;
+    in.win_var = -1.f;
+    if constexpr (kRanges) {
+        const uint32_t ev = P.seed_event[in.seed];
+        const uint32_t* rg = P.meas_ranges + static_cast<size_t>(ev) * (P.n_surfaces + 1u);
+        in.lo = rg[in.sf];
+        in.up = rg[in.sf + 1u];
+        in.win_var = P.event_win_var[ev];
+    }
     in.rule0 = (step == 0u) && !(P.sfs[in.sf].thickness_x0 > 0.f);
 }
 
@@ -496,15 +503,18 @@ __global__ __launch_bounds__(kBlock) void k_list_emit(ckf_params P) {
             ctl->n_rows.v = lo32(b) + total;
         }
         uint32_t run = lo32(b);
+#pragma unroll 1
+        for (uint32_t ch = 0; ch < kc; ++ch) {
+            // select cnt[ch] without a runtime index into the array
+            uint32_t c = 0u;
 #pragma unroll
-        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
-            if (static_cast<uint32_t>(ch) >= kc) {
-                continue;  // kc is wave-uniform
+            for (int q = 0; q < static_cast<int>(kEmitChunks); ++q) {
+                if (static_cast<uint32_t>(q) == ch) c = cnt[q];
             }
-            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
-            const uint32_t incl = wave_inclusive_scan(cnt[ch]);
-            const uint32_t first = run + incl - cnt[ch];
-            for (uint32_t q = 0; q < cnt[ch]; ++q) {
+            const uint32_t i = i0 + ch * kWave + lane();
+            const uint32_t incl = wave_inclusive_scan(c);
+            const uint32_t first = run + incl - c;
+            for (uint32_t q = 0; q < c; ++q) {
                 P.row_slot[first + q] = i * Lc + q;
             }
             run += wave_broadcast(incl, kWave - 1);
@@ -615,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         uint32_t count = 0u;
         if (valid) {
             find_input in;
-            load_input(P, step, i, prev_base, in);
+            load_input<!kScored>(P, step, i, prev_base, in);
             if (leader) {
                 P.stage_in[i] = make_stage_head(in, step, i);
             }
