@@ -45,15 +45,18 @@ __device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.li
 // ---------------------------------------------------------------------------
 // find_tracks
 // ---------------------------------------------------------------------------
-// A team of kTeam lanes handles one input branch: the lanes split the
+// A team of kT lanes handles one input branch: the lanes split the
 // surface's measurements, reduce the best (chi2, index) candidate with
 // shuffles, and run the gain update redundantly (so the team stays uniform);
-// the team leader writes.  A wave holds kWave / kTeam inputs.  Each lane
-// reads kBatch measurements per round with all loads issued before the chi2
-// arithmetic, so a round costs one memory latency.
-constexpr int kTeam = 4;
-constexpr int kTeamsPerWave = kWave / kTeam;
-// scorer lists: up to kTeam * kListSlots candidates per input
+// the team leader writes.  A wave holds kWave / kT inputs.  Each lane reads
+// kBatch measurements per round with all loads issued before the chi2
+// arithmetic, so a round costs one memory latency.  The plain find runs
+// two-lane teams (its gain updates are redundant over the team, and the loc0
+// window leaves few measurements to split); the scorer list, which updates
+// nothing, runs four-lane teams.
+constexpr int kFindTeam = 2;
+constexpr int kListTeam = 4;
+// scorer lists: up to kListTeam * kListSlots candidates per input
 constexpr int kListSlots = 4;
 constexpr int kBatch = 4;
 
@@ -62,6 +65,7 @@ constexpr int kBatch = 4;
 // !have_last) that passes the cut and the step-0 rule.  kInvalid when none.
 // With count_pass, n_pass receives the number of measurements passing the cut
 // (on every lane of the team).
+template <int kT>
 __device__ inline void scan_next(const ckf_params& P, const float* x, const float* c, uint32_t lo,
                                  uint32_t up, bool rule0, bool have_last, float last_c,
                                  uint32_t last_m, uint32_t tl, bool count_pass, float& best_c,
@@ -69,16 +73,16 @@ __device__ inline void scan_next(const ckf_params& P, const float* x, const floa
     best_c = 0.f;
     best_m = kInvalid;
     uint32_t np = 0u;
-    for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
+    for (uint32_t m0 = lo + tl; m0 < up; m0 += kT * kBatch) {
         meas_data mb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kT);
             mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kT);
             const float chi2 = predicted_chi2(mb[u], x, c);
             // outside the cut (a NaN never yields a link either)
             bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
@@ -93,9 +97,9 @@ __device__ inline void scan_next(const ckf_params& P, const float* x, const floa
     }
     // team minimum of (chi2, index); kInvalid sorts last
 #pragma unroll
-    for (int d = kTeam / 2; d >= 1; d >>= 1) {
-        const float oc = __shfl_xor(best_c, d, kTeam);
-        const uint32_t om = __shfl_xor(best_m, d, kTeam);
+    for (int d = kT / 2; d >= 1; d >>= 1) {
+        const float oc = __shfl_xor(best_c, d, kT);
+        const uint32_t om = __shfl_xor(best_m, d, kT);
         const bool take = (om != kInvalid) &&
                           (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
         if (take) {
@@ -105,7 +109,7 @@ __device__ inline void scan_next(const ckf_params& P, const float* x, const floa
     }
     if (count_pass) {
 #pragma unroll
-        for (int d = kTeam / 2; d >= 1; d >>= 1) np += __shfl_xor(np, d, kTeam);
+        for (int d = kT / 2; d >= 1; d >>= 1) np += __shfl_xor(np, d, kT);
         n_pass = np;
     }
 }
@@ -122,11 +126,12 @@ struct find_input {
 
 // A superset [lo, up) of the surface's measurements whose loc0 can lie in
 // [a, b], for measurements sorted by loc0: cooperative 16-ary narrowing by
-// the team (kTeam lanes x kBatch probes per round, one memory latency each)
+// the team (kT lanes x kBatch probes per round, one memory latency each)
 // of the first index >= a and of the first index > b.
+template <int kT>
 __device__ inline void window_range(const ckf_params& P, float a, float b, uint32_t tl,
                                     uint32_t& lo, uint32_t& up) {
-    constexpr uint32_t kProbes = kTeam * kBatch;  // 16
+    constexpr uint32_t kProbes = kT * kBatch;
     for (int bound = 0; bound < 2; ++bound) {
         const float key = bound == 0 ? a : b;
         uint32_t L = lo, H = up;  // the wanted index lies in [L, H]
@@ -135,13 +140,13 @@ __device__ inline void window_range(const ckf_params& P, float a, float b, uint3
             uint32_t below = 0u;
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t k = static_cast<uint32_t>(u * kTeam) + tl;
+                const uint32_t k = static_cast<uint32_t>(u * kT) + tl;
                 const uint32_t pos = L + ((k + 1u) * span) / (kProbes + 1u);
                 const float v = P.meas_pos[pos].x;
                 below += (bound == 0 ? (v < key) : !(v > key)) ? 1u : 0u;
             }
 #pragma unroll
-            for (int d = kTeam / 2; d >= 1; d >>= 1) below += __shfl_xor(below, d, kTeam);
+            for (int d = kT / 2; d >= 1; d >>= 1) below += __shfl_xor(below, d, kT);
             // probes are sorted: `below` of them precede the wanted index
             const uint32_t nL = below == 0u ? L : L + (below * span) / (kProbes + 1u) + 1u;
             const uint32_t nH = below == kProbes ? H : L + ((below + 1u) * span) / (kProbes + 1u);
@@ -157,16 +162,19 @@ __device__ inline void window_range(const ckf_params& P, float a, float b, uint3
 }
 
 // Candidates of one input in (chi2, index) order from a single pass over the
-// surface's measurements: every lane keeps the kLaneKeep best passing
-// candidates of its share in registers (sorted), and next() merges the
-// team's lists with one team minimum per candidate.  When some lane of the
-// team saw more passing candidates than it keeps, the stream falls back to
-// one scan_next pass per candidate (the order is the same either way).
-template <int kLaneKeep = 4>
+// surface's measurements (its loc0 window when the event allows one): every
+// lane keeps the kLaneKeep best passing candidates of its share in registers
+// (sorted), and next() merges the team's lists with one team minimum per
+// candidate.  When some lane of the team saw more passing candidates than it
+// keeps, the stream falls back to one scan_next pass over the same range per
+// candidate (the order is the same either way).
+template <int kLaneKeep, int kT>
 struct cand_stream {
     static constexpr int kKeep = kLaneKeep;
+    static constexpr int kTeamSize = kT;
     float lc[kLaneKeep];
     uint32_t lm[kLaneKeep];
+    uint32_t lo, up;    // the measurements searched (the loc0 window when taken)
     uint32_t head;      // next own list entry
     uint32_t n_pass;    // team: measurements passing the cut
     bool fallback;      // team-uniform
@@ -179,6 +187,7 @@ template <typename Stream>
 __device__ __forceinline__ void stream_init(const ckf_params& P, const find_input& in, uint32_t tl,
                                    Stream& S) {
     constexpr int kLaneKeep = Stream::kKeep;
+    constexpr int kT = Stream::kTeamSize;
     const float inf = __int_as_float(0x7f800000);
 #pragma unroll
     for (int j = 0; j < kLaneKeep; ++j) {
@@ -190,22 +199,22 @@ __device__ __forceinline__ void stream_init(const ckf_params& P, const find_inpu
     // (chi2 >= r0^2 / R00 for any 2x2 SPD R): outside that loc0 window none
     // can pass, so on a dense surface sorted by loc0 only the window is read
     uint32_t lo = in.lo, up = in.up;
-    if (in.win_var >= 0.f && up - lo > static_cast<uint32_t>(kTeam * kBatch)) {
+    if (in.win_var >= 0.f && up - lo > static_cast<uint32_t>(kT * kBatch)) {
         const float w = sqrtf(P.chi2_max * (in.c[sym_idx(0, 0)] + in.win_var)) * 1.01f + 1e-3f;
         if (w == w) {
-            window_range(P, in.x[e_loc0] - w, in.x[e_loc0] + w, tl, lo, up);
+            window_range<kT>(P, in.x[e_loc0] - w, in.x[e_loc0] + w, tl, lo, up);
         }
     }
-    for (uint32_t m0 = lo + tl; m0 < up; m0 += kTeam * kBatch) {
+    for (uint32_t m0 = lo + tl; m0 < up; m0 += kT * kBatch) {
         meas_data mb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kT);
             mb[u] = load_meas(P.meas_pos, P.meas_var, P.meas_info, m < up ? m : lo);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const uint32_t m = m0 + static_cast<uint32_t>(u * kTeam);
+            const uint32_t m = m0 + static_cast<uint32_t>(u * kT);
             const float chi2 = predicted_chi2(mb[u], in.x, in.c);
             bool take = m < up && chi2 <= P.chi2_max && chi2 >= 0.f;
             take = take && !(in.rule0 && chi2 != 0.f);
@@ -230,11 +239,13 @@ __device__ __forceinline__ void stream_init(const ckf_params& P, const find_inpu
     }
     uint32_t over = np > static_cast<uint32_t>(kLaneKeep) ? 1u : 0u;
 #pragma unroll
-    for (int d = kTeam / 2; d >= 1; d >>= 1) {
-        np += __shfl_xor(np, d, kTeam);
-        over |= __shfl_xor(over, d, kTeam);
+    for (int d = kT / 2; d >= 1; d >>= 1) {
+        np += __shfl_xor(np, d, kT);
+        over |= __shfl_xor(over, d, kT);
     }
     S.n_pass = np;
+    S.lo = lo;
+    S.up = up;
     S.fallback = over != 0u;
     S.head = 0u;
     S.have_last = false;
@@ -246,9 +257,10 @@ template <typename Stream>
 __device__ __forceinline__ void stream_next(const ckf_params& P, const find_input& in, uint32_t tl,
                                    Stream& S, float& best_c, uint32_t& best_m) {
     constexpr int kLaneKeep = Stream::kKeep;
+    constexpr int kT = Stream::kTeamSize;
     if (S.fallback) {
         uint32_t unused = 0u;
-        scan_next(P, in.x, in.c, in.lo, in.up, in.rule0, S.have_last, S.last_c, S.last_m, tl,
+        scan_next<kT>(P, in.x, in.c, S.lo, S.up, in.rule0, S.have_last, S.last_c, S.last_m, tl,
                   false, best_c, best_m, unused);
     } else {
         // own head: a select chain keeps the list in registers
@@ -268,9 +280,9 @@ __device__ __forceinline__ void stream_next(const ckf_params& P, const find_inpu
         best_m = m;
         // team minimum of (chi2, index); kInvalid sorts last
 #pragma unroll
-        for (int d = kTeam / 2; d >= 1; d >>= 1) {
-            const float oc = __shfl_xor(best_c, d, kTeam);
-            const uint32_t om = __shfl_xor(best_m, d, kTeam);
+        for (int d = kT / 2; d >= 1; d >>= 1) {
+            const float oc = __shfl_xor(best_c, d, kT);
+            const uint32_t om = __shfl_xor(best_m, d, kT);
             const bool take = (om != kInvalid) &&
                               (best_m == kInvalid || oc < best_c || (oc == best_c && om < best_m));
             if (take) {
@@ -399,11 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t step = ctl->step;
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t prev_base = ctl->prev_link_base;
-    const uint32_t team = lane() / kTeam;
-    const uint32_t tl = lane() % kTeam;
+    const uint32_t team = lane() / kListTeam;
+    const uint32_t tl = lane() % kListTeam;
     const uint32_t Lc = P.list_len;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n_in + kTeamsPerWave - 1) / kTeamsPerWave;
+    const uint32_t n_parts = (n_in + (kWave / kListTeam) - 1) / (kWave / kListTeam);
     // dynamic partitions (dense surfaces make some inputs far heavier)
     ticket_run queue(&ctl->ticket[7].v, n_parts);
     for (;;) {
@@ -411,17 +423,17 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i0 = p * kTeamsPerWave;
+        const uint32_t i0 = p * (kWave / kListTeam);
         const uint32_t i = i0 + team;
         const bool active = i < n_in;
         find_input in;
         uint32_t n_pass = 0u, k = 0u;
-        // candidate q of the list is held by team lane q % kTeam in slot q / kTeam
+        // candidate q of the list is held by team lane q % kListTeam in slot q / kListTeam
         uint32_t sm[kListSlots];
         float sc[kListSlots];
         if (active) {
             load_input(P, step, i, prev_base, in);
-            cand_stream<4> cs;
+            cand_stream<4, kListTeam> cs;
             stream_init(P, in, tl, cs);
             n_pass = cs.n_pass;
             for (; k < Lc; ++k) {
@@ -433,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                 }
 #pragma unroll
                 for (int j = 0; j < kListSlots; ++j) {
-                    if (k == static_cast<uint32_t>(j * kTeam) + tl) {
+                    if (k == static_cast<uint32_t>(j * kListTeam) + tl) {
                         sm[j] = best_m;
                         sc[j] = best_c;
                     }
@@ -449,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
             // candidates, at the input's fixed slots i * list_len + q
 #pragma unroll
             for (int j = 0; j < kListSlots; ++j) {
-                const uint32_t q = static_cast<uint32_t>(j * kTeam) + tl;
+                const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
                 if (q < k) {
                     const size_t r = static_cast<size_t>(i) * Lc + q;
                     P.list_m[r] = sm[j];
@@ -585,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
     }
 }
 
-// A team of kTeam lanes handles one input branch.  Plain: the lanes split
+// A team of kFindTeam lanes handles one input branch.  Plain: the lanes split
 // the surface's measurements (scan_next) and take candidates in (chi2, index)
 // order; logged: the same, with every filtered slot's features kept for the
 // branch log; scored: the candidates come from the scored list, pruned ones
@@ -599,7 +611,7 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     constexpr bool kScored = (kMode == kFindScored);
     constexpr bool logging = (kMode == kFindLogged);
-    constexpr int kT = kScored ? 1 : kTeam;
+    constexpr int kT = kScored ? 1 : kFindTeam;
     constexpr uint32_t kPerWave = kWave / kT;
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -671,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             } else {
                 // candidates in stream order; the gain update runs on every
                 // lane of the team (uniform), the leader writes
-                cand_stream<4> cs;
+                cand_stream<4, kFindTeam> cs;
                 stream_init(P, in, tl, cs);
                 n_pass = cs.n_pass;
                 bool rank0 = true;
