@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
         float sc[kListSlots];
         if (active) {
             load_input(P, step, i, prev_base, in);
-            cand_stream<4, kListTeam> cs;
+            cand_stream<6, kListTeam> cs;
             stream_init(P, in, tl, cs);
             n_pass = cs.n_pass;
             for (; k < Lc; ++k) {
@@ -683,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             } else {
                 // candidates in stream order; the gain update runs on every
                 // lane of the team (uniform), the leader writes
-                cand_stream<4, kFindTeam> cs;
+                cand_stream<6, kFindTeam> cs;
                 stream_init(P, in, tl, cs);
                 n_pass = cs.n_pass;
                 bool rank0 = true;
