@@ -340,6 +340,9 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
         return std::max(1u, std::min(b, max_blocks));
     };
     tamd::launch_shape s;
+    // grids are upper bounds for the largest step: a wave beyond a step's
+    // partitions leaves at once.  The find is sized for 16 inputs per wave
+    // (its two-lane teams hold 32, the scored update 64).
     s.find_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
     s.prop_team = A.prop_team;
