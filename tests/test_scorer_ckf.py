@@ -5,7 +5,8 @@
     labels count exactly the tracks' measurements; feature ranges.
   * Scored finding: a scorer that keeps everything gives the plain CKF's
     tracks bit for bit (C4, three branches per surface); one that prunes
-    everything gives no track.
+    everything gives no track; one that prunes about half gives a batch of
+    events the tracks each event gets alone.
   * Gain: trained on survive / die labels the plain CKF logs on dense C4
     events (three branches per surface; a link survives when it lies on its
     seed's best found track), the pruned CKF keeps the plain CKF's
@@ -96,6 +97,34 @@ def test_scored_extremes():
     alg.set_scorer(None)
     again = alg.run_device(ddet, dev).download()
     assert np.array_equal(plain.constituents, again.constituents)
+
+
+def test_scored_batched_equals_single_events():
+    """The scorer decides each candidate from its own features and its seed's
+    standing, so a batch of events gets, event by event, the tracks each
+    event gets alone -- with a threshold (the median logit) that prunes about
+    half of the candidates."""
+    from traccc_amd import scorer
+    det, fld, evs = _events(3, 1500, 701)
+    cfg = _c4_cfg(3)
+    ddet = ckf.DeviceDetector(det, fld)
+    alg = ckf.combinatorial_kalman_filter_algorithm(cfg)
+    plain = alg.run_device(ddet, ckf.DeviceEvents(ddet, evs)).download()
+    _, f, _, n = alg.run_device_logged(ddet, ckf.DeviceEvents(ddet, evs[0]), capacity=400000)
+    assert 0 < n <= 400000
+    s = scorer.Scorer(seed=5)
+    s.fit_normalisation(f)
+    threshold = float(np.median(s.forward(f).float().cpu().numpy()))
+    alg.set_scorer(s, threshold=threshold)
+    batch = alg.run_device(ddet, ckf.DeviceEvents(ddet, evs)).download()
+    assert alg.last_stats().n_scored > 0
+    assert not np.array_equal(batch.constituents, plain.constituents)   # it pruned
+    for e, ev in enumerate(evs):
+        single = alg.run_device(ddet, ckf.DeviceEvents(ddet, ev)).download()
+        part = batch.event(e)
+        assert np.array_equal(part.offsets, single.offsets)
+        assert np.array_equal(part.constituents, single.constituents)
+        assert np.array_equal(part.chi2, single.chi2)
 
 
 def _run(alg, ddet, events):
