@@ -45,7 +45,7 @@ ckf_status ckf_debug_propagate(const ckf_finding_config* cfg, const ckf_detector
 ckf_status ckf_debug_set_kernel_timing(ckf_algorithm* alg, int enable);
 
 /* Propagation schedule: lanes that navigate one link together in the
- * propagation kernel (1, 4, 8 or 16; 0 = autotuned per seed-count class, the
+ * propagation kernel (1, 2, 4, 8 or 16; 0 = autotuned per seed-count class, the
  * default).  *lanes receives the schedule of the last call. */
 ckf_status ckf_debug_set_schedule(ckf_algorithm* alg, uint32_t lanes_per_link);
 ckf_status ckf_debug_get_schedule(const ckf_algorithm* alg, uint32_t* lanes_per_link);

@@ -309,7 +309,7 @@ def test_unsorted_measurements_rejected():
 
 @pytest.mark.parametrize("workload,n", [("C2", 3000), ("C4", 2000)])
 def test_propagation_schedules_identical(workload, n):
-    """Every propagation schedule (lanes per link 1, 4, 8, 16) gives the same
+    """Every propagation schedule (lanes per link 1, 2, 4, 8, 16) gives the same
     tracks, bit for bit; the autotuned default is one of them."""
     det, fld, gun = S.workload(workload, n_particles=n, seed=17)
     ev = S.generate_event(det, fld, gun)
@@ -319,7 +319,7 @@ def test_propagation_schedules_identical(workload, n):
     lib.ckf_debug_set_schedule.argtypes = [C.c_void_p, C.c_uint32]
     lib.ckf_debug_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     ref = None
-    for lanes in (0, 1, 4, 8, 16):
+    for lanes in (0, 1, 2, 4, 8, 16):
         alg = ckf.combinatorial_kalman_filter_algorithm(cfg)
         assert lib.ckf_debug_set_schedule(alg._h, lanes) == 0
         tc = alg(det, fld, ev)

@@ -1738,6 +1738,9 @@ hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStrea
         case 4:
             hipLaunchKernelGGL(k_propagate<4>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
+        case 2:
+            hipLaunchKernelGGL(k_propagate<2>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
+            break;
         default:
             hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;

@@ -12,7 +12,7 @@ struct launch_shape {
     uint32_t dedup_blocks;
     uint32_t prop_blocks;
     uint32_t build_blocks;
-    uint32_t prop_team;  // lanes per link in k_propagate: 1, 4, 8 or 16
+    uint32_t prop_team;  // lanes per link in k_propagate: 1, 2, 4, 8 or 16
     uint32_t list_blocks;   // scorer-pruned finding: candidate lists
     uint32_t score_blocks;  // and the MLP over them
     uint32_t emit_blocks;   // links of the step (k_find_emit)
