@@ -1228,8 +1228,8 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
 }
 
 // Step advance: move the control block to the next step.  Run by one lane
-// once every wave of the step's propagation has finished (k_advance, or the
-// last wave to leave k_propagate).
+// once the step's propagation and its compaction have finished (k_advance,
+// or the last block to leave k_prop_emit).
 __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     const uint32_t n_links = ctl->n_links;
     ctl->n_links_total += n_links;

@@ -15,12 +15,12 @@ struct launch_shape {
     uint32_t prop_team;  // lanes per link in k_propagate: 1, 2, 4, 8 or 16
     uint32_t list_blocks;   // scorer-pruned finding: candidate lists
     uint32_t score_blocks;  // and the MLP over them
-    uint32_t emit_blocks;   // links of the step (k_find_emit)
+    uint32_t emit_blocks;   // the compaction kernels (k_*_emit)
 };
 
 // One CKF step: find_tracks, duplicate removal (active from
 // duplicate_removal_minimum_length on), propagation to the next surface with
-// the step advance folded into its last wave.  Every count is read from the
+// the step advance folded into the last block of its compaction kernel.  Every count is read from the
 // device step control block, so the sequence can be captured in a graph.
 // with_dedup = false drops the duplicate-removal kernels (steps before
 // duplicate_removal_minimum_length, where they would exit at once).
@@ -28,8 +28,8 @@ hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream
                            bool with_dedup);
 // the same step, kernel group by kernel group (timed runs)
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-// launch_find = the scorer's list + score kernels (scored runs only), then the
-// find / update kernel
+// launch_find = the scorer's list, row-packing and score kernels (scored runs
+// only), then the find / update kernel and its link compaction
 hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_score(const ckf_params& P, const launch_shape& s, hipStream_t stream);
