@@ -1115,9 +1115,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
 }
 
 // After k_propagate: the next step's inputs (links that reached a surface)
-// and the tips, compacted in link order over 1024-link partitions; the last
-// wave to leave runs the step advance (fused_advance) so the step loop never
-// waits on the host.
+// and the tips, compacted in link order (partitions of up to 1024 links); the
+// last block to leave runs the step advance (fused_advance) so the step loop
+// never waits on the host.
 __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_advance) {
     __shared__ uint32_t s_exit;  // waves of this block past the queue
     step_ctl* ctl = P.ctl;
