@@ -8,7 +8,7 @@
  * by the plain CKF itself (ckf_find_tracks_logged) with the project's own
  * BF16-MFMA forward / backward kernels, and prunes candidates before the
  * Kalman update (ckf_algorithm_set_scorer).  The plain CKF stays available
- * and is the judge of finding efficiency.
+ * and is the yardstick of finding efficiency.
  *
  * Features (in this order, FP32, normalised in the kernel with the scorer's
  * mean / std):

@@ -1,6 +1,6 @@
 """HIP CKF vs the FP64 CPU oracle on the same seeded inputs.
 
-Bars (BASELINE north star): track-candidate measurement-index sets match at
+Bars: track-candidate measurement-index sets match at
 >= 99.5%; Kalman-filtered parameters and chi2 agree within 1e-4 relative
 (FP32 device arithmetic vs FP64 oracle).  Unit level: the gain-matrix update
 and the propagation to the next surface; chain level: workloads C1-C4 of the
