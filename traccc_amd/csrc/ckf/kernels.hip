@@ -646,16 +646,24 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             const uint32_t ln = kScored ? P.list_n[i] : 0u;
             const uint32_t row0 = kScored ? i * P.list_len : 0u;  // the input's list slots
             if constexpr (kScored) {
-                // one lane per input: the kept list entries in order
-                while (count < N) {
-                    while (li < ln && P.cand_keep[row0 + li] == 0u) ++li;
-                    if (li >= ln) {
-                        break;
+                // one lane per input: the kept list entries in order.  The
+                // verdicts are read all at once (independent loads, one
+                // memory latency) into a mask of the kept entries.
+                constexpr int kListMax = kListTeam * kListSlots;
+                static_assert(kListMax <= 32, "kept-entry mask is 32 bits");
+                uint32_t kept = 0u;
+#pragma unroll
+                for (int q = 0; q < kListMax; ++q) {
+                    if (static_cast<uint32_t>(q) < ln && P.cand_keep[row0 + q] != 0u) {
+                        kept |= 1u << q;
                     }
+                }
+                while (count < N && kept != 0u) {
+                    li = static_cast<uint32_t>(__builtin_ctz(kept));
+                    kept &= kept - 1u;
                     const size_t r = static_cast<size_t>(i) * P.list_len + li;
                     const uint32_t best_m = P.list_m[r];
                     const float best_c = P.list_c[r];
-                    ++li;
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
                     float fx[6], fc[21];
 #pragma unroll
