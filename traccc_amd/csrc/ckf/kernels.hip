@@ -54,6 +54,9 @@ __device__ inline dlink load_link(const ckf_params& P, uint32_t g) { return P.li
 // two-lane teams (its gain updates are redundant over the team, and the loc0
 // window leaves few measurements to split); the scorer list, which updates
 // nothing, runs four-lane teams.
+static_assert(sizeof(line_u32) == kXcdCounterStride * 4u &&
+                  sizeof(step_ctl::xcd_ticket[0]) == kXcds * sizeof(line_u32),
+              "per-XCD counters: one per XCD, 128 B apart");
 constexpr int kFindTeam = 2;
 constexpr int kListTeam = 4;
 // scorer lists: up to kListTeam * kListSlots candidates per input
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n_in + (kWave / kListTeam) - 1) / (kWave / kListTeam);
     // dynamic partitions (dense surfaces make some inputs far heavier)
-    ticket_run queue(&ctl->ticket[7].v, n_parts);
+    xcd_ticket_run queue(&ctl->xcd_ticket[1][0].v, n_parts);
     for (;;) {
         const uint32_t p = queue.next();
         if (p >= n_parts) {
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
-    ticket_run queue(&ctl->ticket[0].v, n_parts);
+    xcd_ticket_run queue(&ctl->xcd_ticket[0][0].v, n_parts);
     for (;;) {
         const uint32_t p = queue.next();
         if (p >= n_parts) {
@@ -1254,6 +1257,9 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
     for (int k = 0; k < kTickets; ++k) ctl->ticket[k].v = 0u;
+    for (int q = 0; q < 2; ++q) {
+        for (uint32_t x = 0; x < kXcds; ++x) ctl->xcd_ticket[q][x].v = 0u;
+    }
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
         ctl->done = 1u;
     }

@@ -86,18 +86,36 @@ __device__ inline uint32_t wave_ticket(uint32_t* counter) {
 // Partitions handed out several per ticket: consecutive partitions (the
 // branches of neighbouring seeds, which read the same surfaces) stay on one
 // wave, and a queue of millions of small partitions takes a quarter of the
-// atomics on its counter.  Small queues keep one partition per ticket so
+// atomics on its counters.  Small queues keep one partition per ticket so
 // every wave still gets work.  next() returns the wave's next partition
 // (n_parts when the queue is empty).
-struct ticket_run {
-    uint32_t* counter;
+//
+// The tickets form kXcds contiguous ranges with a counter each (128 B
+// apart); a wave takes from its own XCD's range first -- neighbouring
+// partitions read the same measurement lines, which then stay in that XCD's
+// L2 -- and moves on to the next range when its own is spent, until every
+// range is.  Workgroups are dispatched to the XCDs round-robin, so block b
+// runs on XCD b % kXcds (a different placement only costs locality).  Only
+// for kernels without look-back: a partition would otherwise wait on every
+// partition of the lower ranges.
+constexpr uint32_t kXcds = 8u;
+constexpr uint32_t kXcdCounterStride = 32u;  // u32 words between counters
+
+struct xcd_ticket_run {
+    uint32_t* counters;
     uint32_t n_parts;
     uint32_t per;
+    uint32_t n_tickets;
     bool takes;
+    uint32_t home;        // this block's XCD
+    uint32_t tried = 0u;  // ranges found spent, in order from home
     uint32_t base = 0u;
     uint32_t sub = 0u;
-    __device__ ticket_run(uint32_t* c, uint32_t n)
-        : counter(c), n_parts(n), per(per_ticket(n)), takes(wave_takes_tickets((n + per - 1u) / per)) {
+    __device__ xcd_ticket_run(uint32_t* c, uint32_t n)
+        : counters(c), n_parts(n), per(per_ticket(n)) {
+        n_tickets = (n_parts + per - 1u) / per;
+        takes = wave_takes_tickets(n_tickets);
+        home = blockIdx.x % kXcds;
         sub = per;  // no ticket held
     }
     // one more partition per ticket for every 4,096 partitions, up to four
@@ -107,8 +125,21 @@ struct ticket_run {
     }
     __device__ uint32_t next() {
         if (sub == per) {
-            const uint32_t n_tickets = (n_parts + per - 1u) / per;
-            base = takes ? wave_ticket(counter) * per : n_tickets * per;
+            uint32_t t = n_tickets;
+            while (takes && tried < kXcds) {
+                const uint32_t x = (home + tried) % kXcds;
+                const uint32_t lo = static_cast<uint32_t>(uint64_t(n_tickets) * x / kXcds);
+                const uint32_t hi = static_cast<uint32_t>(uint64_t(n_tickets) * (x + 1u) / kXcds);
+                if (lo < hi) {
+                    const uint32_t got = lo + wave_ticket(counters + x * kXcdCounterStride);
+                    if (got < hi) {
+                        t = got;
+                        break;
+                    }
+                }
+                ++tried;
+            }
+            base = t * per;
             sub = 0u;
         }
         const uint32_t p = base + sub;
