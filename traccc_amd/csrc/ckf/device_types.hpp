@@ -101,12 +101,14 @@ struct step_ctl {
     uint32_t prev_link_base;  // global index of the previous step's first link
     uint64_t n_links_total;
     uint64_t n_scored;        // scorer rows of the finished steps
-    line_u32 ticket[kTickets];  // work-queue tickets: progressive filter, dedup, propagate, build, find
-                                // emit, propagate emit, dedup emit, (unused), list emit
+    line_u32 ticket[kTickets];  // work-queue tickets: [0] progressive filter, [1] votes /
+                                // progressive filter, [2] free, [3] build, [4] find emit,
+                                // [5] propagate emit, [6] dedup emit, [7] free, [8] list emit
     line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
     line_u32 n_rows;      // scorer rows listed this step (written by k_list_emit)
     line_u64 n_props;     // propagations so far
-    line_u32 xcd_ticket[2][8];  // per-XCD work queues (one counter per XCD): find, scorer list
+    line_u32 xcd_ticket[4][8];  // per-XCD work queues (one counter per XCD): find, scorer
+                                // list, propagate, dedup mark
 };
 
 struct ckf_params {

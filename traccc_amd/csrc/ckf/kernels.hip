@@ -934,9 +934,9 @@ __global__ __launch_bounds__(kBlock) void k_dedup_mark(ckf_params P) {
     const uint32_t base = ctl->link_base;
     const uint32_t epoch = ctl->epoch;
     const uint32_t n_parts = (n + kWave - 1) / kWave;
-    const bool takes = wave_takes_tickets(n_parts);
+    xcd_ticket_run queue(&ctl->xcd_ticket[3][0].v, n_parts, 1u);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[1].v) : n_parts;
+        const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
@@ -1057,9 +1057,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
 
-    const bool takes = wave_takes_tickets(n_parts);
+    xcd_ticket_run queue(&ctl->xcd_ticket[2][0].v, n_parts, 1u);
     for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[2].v) : n_parts;
+        const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
@@ -1257,7 +1257,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
     for (int k = 0; k < kTickets; ++k) ctl->ticket[k].v = 0u;
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
         for (uint32_t x = 0; x < kXcds; ++x) ctl->xcd_ticket[q][x].v = 0u;
     }
     if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {

@@ -111,8 +111,8 @@ struct xcd_ticket_run {
     uint32_t tried = 0u;  // ranges found spent, in order from home
     uint32_t base = 0u;
     uint32_t sub = 0u;
-    __device__ xcd_ticket_run(uint32_t* c, uint32_t n)
-        : counters(c), n_parts(n), per(per_ticket(n)) {
+    __device__ xcd_ticket_run(uint32_t* c, uint32_t n, uint32_t max_per = 4u)
+        : counters(c), n_parts(n), per(per_ticket(n) < max_per ? per_ticket(n) : max_per) {
         n_tickets = (n_parts + per - 1u) / per;
         takes = wave_takes_tickets(n_tickets);
         home = blockIdx.x % kXcds;
