@@ -95,11 +95,15 @@ __device__ inline uint32_t wave_ticket(uint32_t* counter) {
 // partitions read the same measurement lines, which then stay in that XCD's
 // L2 -- and moves on to the next range when its own is spent, until every
 // range is.  Workgroups are dispatched to the XCDs round-robin, so block b
-// runs on XCD b % kXcds (a different placement only costs locality).  Only
+// runs on XCD b % kXcds (a different placement only costs locality).  A
+// queue of fewer than kXcdMinTickets tickets stays one range: its waves
+// would hold a ticket or two each and then spend more time failing on the
+// other ranges' counters than the locality saves.  Only
 // for kernels without look-back: a partition would otherwise wait on every
 // partition of the lower ranges.
 constexpr uint32_t kXcds = 8u;
 constexpr uint32_t kXcdCounterStride = 32u;  // u32 words between counters
+constexpr uint32_t kXcdMinTickets = 1024u;
 
 struct xcd_ticket_run {
     uint32_t* counters;
@@ -107,7 +111,8 @@ struct xcd_ticket_run {
     uint32_t per;
     uint32_t n_tickets;
     bool takes;
-    uint32_t home;        // this block's XCD
+    uint32_t n_ranges;    // kXcds, or 1 for a small queue
+    uint32_t home;        // this block's range: its XCD
     uint32_t tried = 0u;  // ranges found spent, in order from home
     uint32_t base = 0u;
     uint32_t sub = 0u;
@@ -115,7 +120,8 @@ struct xcd_ticket_run {
         : counters(c), n_parts(n), per(per_ticket(n) < max_per ? per_ticket(n) : max_per) {
         n_tickets = (n_parts + per - 1u) / per;
         takes = wave_takes_tickets(n_tickets);
-        home = blockIdx.x % kXcds;
+        n_ranges = n_tickets >= kXcdMinTickets ? kXcds : 1u;
+        home = blockIdx.x % n_ranges;
         sub = per;  // no ticket held
     }
     // one more partition per ticket for every 4,096 partitions, up to four
@@ -126,10 +132,10 @@ struct xcd_ticket_run {
     __device__ uint32_t next() {
         if (sub == per) {
             uint32_t t = n_tickets;
-            while (takes && tried < kXcds) {
-                const uint32_t x = (home + tried) % kXcds;
-                const uint32_t lo = static_cast<uint32_t>(uint64_t(n_tickets) * x / kXcds);
-                const uint32_t hi = static_cast<uint32_t>(uint64_t(n_tickets) * (x + 1u) / kXcds);
+            while (takes && tried < n_ranges) {
+                const uint32_t x = (home + tried) % n_ranges;
+                const uint32_t lo = static_cast<uint32_t>(uint64_t(n_tickets) * x / n_ranges);
+                const uint32_t hi = static_cast<uint32_t>(uint64_t(n_tickets) * (x + 1u) / n_ranges);
                 if (lo < hi) {
                     const uint32_t got = lo + wave_ticket(counters + x * kXcdCounterStride);
                     if (got < hi) {
