@@ -292,6 +292,11 @@ ckf_status ckf_tracks_event_offsets(const ckf_device_tracks* trk, uint32_t* n_ev
 ckf_status ckf_tracks_download(const ckf_device_tracks* trk, ckf_track_candidates* out);
 ckf_status ckf_tracks_destroy(ckf_device_tracks* trk);
 
+/* Device blocks of destroyed results are kept (up to 4 GiB) for the next
+ * calls' results; this frees them.  No counterpart in the reference (traccc's
+ * results are owned by the caller's vecmem memory resource). */
+ckf_status ckf_release_cached_memory(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,21 @@ ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, dou
  * ms_find. */
 ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list, double* ms_score);
 
+/* Candidate-verdict log of scorer-pruned calls, for replay by a CPU
+ * checker: while the capacity is above 0 every ckf_find_tracks call with a
+ * scorer records one 64-bit key per scored candidate, in no particular
+ * order.  The key's low 63 bits are splitmix64 chained over the seed index,
+ * the step, the measurement index of every earlier link of the branch
+ * (latest first, holes as 0xffffffff) and the candidate's measurement index
+ * (h = mix(h ^ v), from h = 0); bit 63 is set when the candidate was kept.
+ * The log restarts with each call (and with each internal rerun after a
+ * buffer regrow).  ckf_debug_get_verdicts copies up to `capacity` keys of the
+ * last call and gives the number recorded (which may exceed the log's
+ * capacity: then enlarge it and call again). */
+ckf_status ckf_debug_set_verdict_log(ckf_algorithm* alg, uint32_t capacity);
+ckf_status ckf_debug_get_verdicts(const ckf_algorithm* alg, uint64_t* keys, uint32_t capacity,
+                                  uint32_t* n_records);
+
 #ifdef __cplusplus
 }
 #endif

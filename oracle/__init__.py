@@ -84,6 +84,11 @@ def lib() -> C.CDLL:
                                      C.POINTER(F.BoundParams), C.c_int, C.POINTER(OracleTracks)]
         L.oracle_ckf_run.restype = C.c_int
         L.oracle_tracks_free.argtypes = [C.POINTER(OracleTracks)]
+        L.oracle_ckf_run_masked.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
+                                            C.POINTER(F.BField), C.POINTER(F.Measurements),
+                                            C.POINTER(F.BoundParams), C.c_int, C.c_uint32,
+                                            F.c_u64_p, C.c_uint64, C.POINTER(OracleTracks)]
+        L.oracle_ckf_run_masked.restype = C.c_int
         L.oracle_ckf_run_many.argtypes = [C.POINTER(F.FindingConfig), C.POINTER(F.Detector),
                                           C.POINTER(F.BField), C.POINTER(F.Measurements),
                                           C.POINTER(F.BoundParams), C.c_uint32, C.c_int,
@@ -139,15 +144,8 @@ class Tracks:
         return self.constituents[self.offsets[i]:self.offsets[i + 1]]
 
 
-def run(cfg, det, fld, event, precision: int = 64) -> Tracks:
+def _tracks(out) -> Tracks:
     L = lib()
-    out = OracleTracks()
-    m = event.measurements_struct()
-    s = event.seeds_struct()
-    rc = L.oracle_ckf_run(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), C.byref(m),
-                          C.byref(s), precision, C.byref(out))
-    if rc != 0:
-        raise ValueError(f"oracle_ckf_run failed ({rc})")
     try:
         n, nc = out.n_tracks, out.n_constituents
         t = Tracks(
@@ -164,6 +162,43 @@ def run(cfg, det, fld, event, precision: int = 64) -> Tracks:
     finally:
         L.oracle_tracks_free(C.byref(out))
     return t
+
+
+def run(cfg, det, fld, event, precision: int = 64) -> Tracks:
+    L = lib()
+    out = OracleTracks()
+    m = event.measurements_struct()
+    s = event.seeds_struct()
+    rc = L.oracle_ckf_run(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct), C.byref(m),
+                          C.byref(s), precision, C.byref(out))
+    if rc != 0:
+        raise ValueError(f"oracle_ckf_run failed ({rc})")
+    return _tracks(out)
+
+
+def run_masked(cfg, det, fld, event, list_len: int, kept_keys, precision: int = 64) -> Tracks:
+    """The CKF pruned by given candidate verdicts: per input the list_len best
+    candidates by (chi2, index) passing the cut, of which only those whose key
+    is in kept_keys are updated (in order, until max_num_branches_per_surface
+    succeed).  A key (low 63 bits) is splitmix64 chained over the seed, the
+    step, the branch's earlier measurements (latest first, holes 0xffffffff)
+    and the candidate's measurement, as the device's verdict log writes it.
+    kept_keys None keeps every listed candidate."""
+    L = lib()
+    out = OracleTracks()
+    m = event.measurements_struct()
+    s = event.seeds_struct()
+    if kept_keys is None:   # keep every listed candidate
+        kp, nk = None, 0xffffffffffffffff
+    else:
+        keys = np.ascontiguousarray(np.unique(np.asarray(kept_keys, np.uint64)))
+        kp, nk = F.ptr(keys, C.c_uint64), len(keys)
+    rc = L.oracle_ckf_run_masked(C.byref(cfg), C.byref(det.struct), C.byref(fld.struct),
+                                 C.byref(m), C.byref(s), precision, list_len, kp, nk,
+                                 C.byref(out))
+    if rc != 0:
+        raise ValueError(f"oracle_ckf_run_masked failed ({rc})")
+    return _tracks(out)
 
 
 def run_many(cfg, det, fld, events, precision: int = 32, threads: int = 1):

@@ -897,6 +897,34 @@ struct run_stats {
     uint64_t links = 0, props = 0, tips = 0;
 };
 
+// Candidate verdicts of a pruned run (the learned branch scorer's decisions,
+// taken from the device): per input only the list_len best candidates by
+// (chi2, index) that pass the cut are listed, and of those only the ones whose
+// key is in `kept` (sorted) go to the gain update, in that order, until
+// max_num_branches_per_surface succeed.  A key names a candidate by its
+// branch's history: splitmix64 (Steele, Lea, Flood 2014) chained as
+// h = mix(h ^ v) over the seed, the step, the measurement of every earlier
+// link of the branch (latest first, holes as all-ones) and the candidate's
+// measurement, low 63 bits.
+inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
+    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+struct cand_mask {
+    const uint64_t* kept = nullptr;
+    size_t n_kept = 0;
+    uint32_t list_len = 0;
+    bool keep_all = false;
+    bool has(uint64_t branch, uint32_t m) const {
+        if (keep_all) return true;
+        const uint64_t k = verdict_mix(branch, m) & 0x7fffffffffffffffull;
+        return std::binary_search(kept, kept + n_kept, k);
+    }
+};
+
 // surface identifier -> index
 inline uint32_t surface_index(const ckf_detector& det, uint64_t id) {
     uint32_t lo = 0, hi = det.n_surfaces;
@@ -911,7 +939,8 @@ inline uint32_t surface_index(const ckf_detector& det, uint64_t id) {
 template <typename S>
 std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& det,
                            const ckf_bfield& field, const ckf_measurements& meas,
-                           const ckf_bound_params& seeds, run_stats& stats) {
+                           const ckf_bound_params& seeds, run_stats& stats,
+                           const cand_mask* mask = nullptr) {
     const uint32_t n_meas = meas.size;
     const S mass = static_cast<S>(cfg.ptc_mass);
     const S abs_q = static_cast<S>(std::fabs(cfg.ptc_charge));
@@ -975,7 +1004,52 @@ std::vector<track_out> ckf(const ckf_finding_config& cfg, const ckf_detector& de
                 S chi2;
             };
             std::vector<cand> best;
-            for (uint32_t mi = lo; mi < up; ++mi) {
+            if (mask) {
+                // pruned run: the listed candidates in (chi2, index) order,
+                // the kept ones updated until N succeed
+                const bool rule0 = step == 0 && !(sf.thickness_x0 > 0.f);
+                std::vector<std::pair<S, uint32_t>> listed;
+                for (uint32_t mi = lo; mi < up; ++mi) {
+                    const S chi2 = predicted_chi2<S>(load_meas<S>(meas, mi), in);
+                    if (chi2 > chi2_max || chi2 < S(0)) continue;
+                    if (rule0 && chi2 != S(0)) continue;
+                    listed.push_back({chi2, mi});
+                }
+                std::sort(listed.begin(), listed.end());
+                if (listed.size() > mask->list_len) listed.resize(mask->list_len);
+                // the branch part of the candidates' verdict keys
+                uint64_t branch = verdict_mix(verdict_mix(0u, orig), step);
+                if (prev) {
+                    link L = *prev;
+                    for (;;) {
+                        branch = verdict_mix(branch, L.meas_idx);
+                        if (L.step == 0u) break;
+                        L = back(L);
+                    }
+                }
+                for (const auto& lc : listed) {
+                    if (best.size() >= cfg.max_num_branches_per_surface) break;
+                    if (!mask->has(branch, lc.second)) continue;
+                    const meas_view<S> mv = load_meas<S>(meas, lc.second);
+                    bound_state<S> filt;
+                    int res = CKF_FIT_SUCCESS;
+                    if (rule0) {
+                        filt = in;
+                        filt.cov.a[0][0] = mv.var[0];
+                        filt.cov.a[1][1] = mv.var[1];
+                    } else {
+                        res = gain_update<S>(mv, in, filt);
+                    }
+                    if (res != CKF_FIT_SUCCESS) continue;
+                    link l{step, ip, lc.second, orig, skip, 0u, static_cast<double>(lc.first),
+                           chi2_prev + static_cast<double>(lc.first), ndf_prev + mv.dims};
+                    if (!std::is_same<S, double>::value) {
+                        l.chi2_sum = static_cast<double>(static_cast<S>(chi2_prev) + lc.first);
+                    }
+                    best.push_back({l, filt, lc.first});
+                }
+            }
+            for (uint32_t mi = mask ? up : lo; mi < up; ++mi) {
                 const meas_view<S> mv = load_meas<S>(meas, mi);
                 const S chi2 = predicted_chi2<S>(mv, in);
                 if (chi2 > chi2_max || chi2 < S(0)) continue;
@@ -1720,6 +1794,30 @@ void oracle_tracks_free(oracle_tracks* t) {
     std::free(t->constituent_offsets);
     std::free(t->constituent_index);
     std::memset(t, 0, sizeof(*t));
+}
+
+int oracle_ckf_run_masked(const ckf_finding_config* cfg, const ckf_detector* det,
+                          const ckf_bfield* field, const ckf_measurements* meas,
+                          const ckf_bound_params* seeds, int precision, uint32_t list_len,
+                          const uint64_t* kept, uint64_t n_kept, oracle_tracks* out) {
+    const bool keep_all = kept == nullptr && n_kept == ~uint64_t(0);
+    if (!cfg || !det || !field || !meas || !seeds || !out || (n_kept && !kept && !keep_all))
+        return CKF_ERR_INVALID_ARGUMENT;
+    if (cfg->min_track_candidates_per_track == 0u || cfg->run_pkf) return CKF_ERR_INVALID_ARGUMENT;
+    for (uint64_t k = 1; !keep_all && k < n_kept; ++k) {
+        if (kept[k - 1] > kept[k]) return CKF_ERR_INVALID_ARGUMENT;  // must be sorted
+    }
+    oracle::cand_mask mk;
+    mk.kept = kept;
+    mk.n_kept = keep_all ? 0u : static_cast<size_t>(n_kept);
+    mk.keep_all = keep_all;
+    mk.list_len = list_len;
+    oracle::run_stats st;
+    std::vector<oracle::track_out> tr =
+        precision == 32 ? oracle::ckf<float>(*cfg, *det, *field, *meas, *seeds, st, &mk)
+                        : oracle::ckf<double>(*cfg, *det, *field, *meas, *seeds, st, &mk);
+    oracle::fill_out(tr, *seeds, st, out);
+    return CKF_OK;
 }
 
 int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,

@@ -48,6 +48,20 @@ int oracle_ckf_run(const ckf_finding_config* cfg, const ckf_detector* det, const
                    oracle_tracks* out);
 void oracle_tracks_free(oracle_tracks* t);
 
+/* The CKF pruned by given candidate verdicts (the learned branch scorer's,
+ * read back from the device): per input only the list_len best candidates by
+ * (chi2, index) passing the cut are listed, and only the listed ones whose key
+ * is in `kept` (sorted ascending) are updated, in order, until
+ * max_num_branches_per_surface succeed.  A key is splitmix64 chained as
+ * h = mix(h ^ v) from h = 0 over the seed index, the step, the measurement
+ * index of every earlier link of the branch (latest first, holes as
+ * 0xffffffff) and the candidate's measurement index, low 63 bits.
+ * kept = NULL with n_kept = UINT64_MAX keeps every listed candidate. */
+int oracle_ckf_run_masked(const ckf_finding_config* cfg, const ckf_detector* det,
+                          const ckf_bfield* field, const ckf_measurements* meas,
+                          const ckf_bound_params* seeds, int precision, uint32_t list_len,
+                          const uint64_t* kept, uint64_t n_kept, oracle_tracks* out);
+
 /* Events spread over n_threads host threads, one CKF instance per thread
  * (traccc's throughput_mt pattern).  Returns wall seconds in *seconds. */
 int oracle_ckf_run_many(const ckf_finding_config* cfg, const ckf_detector* det,

@@ -394,3 +394,25 @@ def test_map_field_bilinear_and_axes():
     pn, Tn = rk4(free[:3], free[4:7], free[7], 25.0)
     np.testing.assert_allclose(g[:3], pn, atol=1e-9)
     np.testing.assert_allclose(g[4:7], Tn, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# replay of pruning verdicts (the oracle side of the scorer-pruned parity test)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("workload", ["C4", "C5"])
+def test_verdict_replay_reduces_to_the_plain_ckf(workload):
+    """Keeping every listed candidate with an unbounded list is the plain CKF
+    (same tracks, same order); keeping none leaves no track; a list as short
+    as N still gives the plain tracks when no update fails."""
+    det, fld, gun = S.workload(workload, n_particles=1500, seed=3)
+    ev = S.generate_event(det, fld, gun)
+    cfg = S.finding_config_for(workload)
+    cfg.max_num_branches_per_surface = 3
+    ref = oracle.run(cfg, det, fld, ev)
+    keep_all = oracle.run_masked(cfg, det, fld, ev, 1 << 30, None)
+    assert np.array_equal(ref.offsets, keep_all.offsets)
+    assert np.array_equal(ref.constituents, keep_all.constituents)
+    assert np.array_equal(ref.chi2, keep_all.chi2)
+    assert len(oracle.run_masked(cfg, det, fld, ev, 1 << 30, np.zeros(0, np.uint64))) == 0
+    short = oracle.run_masked(cfg, det, fld, ev, 3, None)
+    assert np.array_equal(ref.constituents, short.constituents)

@@ -212,6 +212,11 @@ struct ckf_params {
     uint8_t* cand_keep;           // [in_cap][list_len] scorer verdict
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
+    // candidate-verdict log (diagnostics): one key per scored candidate,
+    // step << 48 | input << 24 | measurement, bit 63 = kept
+    uint64_t* verdict_log;        // [verdict_cap] or null
+    uint32_t* verdict_n;          // records written (may exceed verdict_cap)
+    uint32_t verdict_cap;
 };
 
 }  // namespace tamd

@@ -48,10 +48,14 @@ struct dbuf {
 
 // Process-wide cache of device blocks for per-call results: freed blocks are
 // kept by power-of-two size class and handed out again, so a steady stream of
-// operator calls does not call hipMalloc / hipFree.
+// operator calls does not call hipMalloc / hipFree.  At most kMaxHeld bytes
+// are kept (a returned block beyond that is freed); release() frees them all
+// (ckf_release_cached_memory).
 struct block_cache {
+    static constexpr size_t kMaxHeld = size_t(4) << 30;
     std::mutex mu;
     std::map<size_t, std::vector<void*>> free_blocks;
+    size_t held = 0;
     static size_t size_class(size_t bytes) {
         size_t c = 256;
         while (c < bytes) c <<= 1;
@@ -65,6 +69,7 @@ struct block_cache {
             if (it != free_blocks.end() && !it->second.empty()) {
                 void* p = it->second.back();
                 it->second.pop_back();
+                held -= c;
                 return p;
             }
         }
@@ -74,8 +79,22 @@ struct block_cache {
     }
     void put(void* p, size_t bytes) {
         if (p == nullptr) return;
+        const size_t c = size_class(std::max<size_t>(bytes, 1));
         std::lock_guard<std::mutex> lk(mu);
-        free_blocks[size_class(std::max<size_t>(bytes, 1))].push_back(p);
+        if (held + c > kMaxHeld) {
+            (void)hipFree(p);
+            return;
+        }
+        free_blocks[c].push_back(p);
+        held += c;
+    }
+    void release() {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : free_blocks) {
+            for (void* p : kv.second) (void)hipFree(p);
+        }
+        free_blocks.clear();
+        held = 0;
     }
 };
 

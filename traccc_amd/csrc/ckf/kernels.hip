@@ -537,6 +537,42 @@ __global__ __launch_bounds__(kBlock) void k_list_emit(ckf_params P) {
     }
 }
 
+// Diagnostics: one scored candidate's verdict as a key a CPU checker replays.
+// The key names the candidate by its branch's history, so it does not depend
+// on where the branch sits in the step's input order: splitmix64 chained over
+// the seed, the step, the measurement of every earlier link of the branch
+// (latest first, holes as all-ones) and the candidate's measurement; low 63
+// bits, bit 63 = kept.
+__device__ inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
+    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t input,
+                                   uint32_t meas, bool keep) {
+    const uint32_t at = atomicAdd(P.verdict_n, 1u);
+    if (at >= P.verdict_cap) {
+        return;
+    }
+    uint64_t h = verdict_mix(0u, P.list_seed[input]);
+    h = verdict_mix(h, step);
+    if (step > 0u) {
+        uint32_t g = P.ctl->prev_link_base + P.live[input];
+        for (uint32_t k = 0; k < step && g < P.link_cap; ++k) {
+            const dlink L = P.links[g];
+            h = verdict_mix(h, L.meas);
+            if (link_step(L) == 0u) {
+                break;
+            }
+            g = L.prev;
+        }
+    }
+    h = verdict_mix(h, meas);
+    P.verdict_log[at] = (h & 0x7fffffffffffffffull) | (keep ? (1ull << 63) : 0ull);
+}
+
 // Scorer-pruned finding, second kernel: the MLP over every listed candidate
 // (64-row tiles on the matrix cores), keep = logit >= threshold.
 __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
@@ -595,7 +631,12 @@ __global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
         }
         const uint32_t r = t * 64u + l;
         if (r < rows) {
-            P.cand_keep[P.row_slot[r]] = mine >= P.score_threshold ? 1u : 0u;
+            const uint32_t slot = P.row_slot[r];
+            const bool keep = mine >= P.score_threshold;
+            P.cand_keep[slot] = keep ? 1u : 0u;
+            if (P.verdict_log != nullptr) {
+                log_verdict(P, ctl->step, slot / Lc, P.list_m[slot], keep);
+            }
         }
     }
 }
@@ -1207,9 +1248,11 @@ __global__ __launch_bounds__(kBlock) void k_prop_emit(ckf_params P, int fused_ad
         // control block -- in LDS, then the block's last wave with one device
         // atomic -- so no wave can start on the advanced control block.  The
         // step's totals come from the last partition's look-back words, which
-        // carry their own visibility; everything else the advance reads was
-        // written by earlier kernels, and its own stores reach the next
-        // kernel at the launch boundary.  No fence is needed.
+        // carry their own visibility; the overflow bits that other blocks of
+        // this kernel raise are read with a device-scope atomic load (the
+        // bits they set with atomicOr live at the device's coherence point);
+        // everything else the advance reads was written by earlier kernels,
+        // and its own stores reach the next kernel at the launch boundary.
         uint32_t in_block = 0u;
         if (lane() == 0u) {
             in_block = atomicAdd(&s_exit, 1u) + 1u;
@@ -1260,7 +1303,9 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     for (int q = 0; q < 4; ++q) {
         for (uint32_t x = 0; x < kXcds; ++x) ctl->xcd_ticket[q][x].v = 0u;
     }
-    if (ctl->n_in == 0u || ctl->step >= P.max_cands || ctl->overflow != 0u) {
+    const uint32_t overflow =
+        __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ctl->n_in == 0u || ctl->step >= P.max_cands || overflow != 0u) {
         ctl->done = 1u;
     }
 }

@@ -104,6 +104,9 @@ struct ckf_algorithm {
     dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, row_slot, list_feat,
         cand_keep, find_count, seed_run_min, list_seed, prop_flag;
     bool logging = false;
+    // candidate-verdict log of scored calls (diagnostics)
+    dbuf verdict_log, verdict_n;
+    uint32_t verdict_cap = 0;
     const ckf_scorer* scorer = nullptr;
     float score_threshold = 0.f;
     uint32_t list_len = 0;
@@ -135,7 +138,7 @@ struct ckf_algorithm {
                        &pkf_lb_lo, &feat_slot, &log_feat,
                        &log_label, &seed_best, &list_m, &list_c, &list_n, &row_slot, &list_feat, &find_count,
                        &seed_run_min, &list_seed, &prop_flag,
-                       &cand_keep};
+                       &cand_keep, &verdict_log, &verdict_n};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < tamd::kLookbacks; ++k) {
             lb_lo[k].release();
@@ -329,6 +332,13 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
         P.cand_keep = A.cand_keep.as<uint8_t>();
         P.scorer = tamd::scorer_view(A.scorer);
         P.score_threshold = A.score_threshold;
+        if (A.verdict_cap > 0u) {
+            A.verdict_log.ensure(8u * size_t(A.verdict_cap));
+            A.verdict_n.ensure(4u);
+            P.verdict_log = A.verdict_log.as<uint64_t>();
+            P.verdict_n = A.verdict_n.as<uint32_t>();
+            P.verdict_cap = A.verdict_cap;
+        }
     }
     return P;
 }
@@ -411,6 +421,9 @@ uint32_t run_steps(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::laun
     std::memcpy(A.h_ctl, &init, sizeof(init));
     hip_check(hipMemcpyAsync(P.ctl, A.h_ctl, sizeof(init), hipMemcpyHostToDevice, A.stream),
               "ctl upload");
+    if (P.verdict_n != nullptr) {
+        hip_check(hipMemsetAsync(P.verdict_n, 0, 4u, A.stream), "memset");
+    }
     hip_check(hipStreamSynchronize(A.stream), "sync");
 
     if (A.timing) {
@@ -926,6 +939,17 @@ ckf_device_event* upload_event(const ckf_device_detector& D, const ckf_measureme
                 row[s + 1u] = off + ub;
                 for (uint32_t m = row[s]; m < row[s + 1u]; ++m) meas_sf[m] = s;
             }
+            // every measurement must sit on a detector surface: one whose
+            // identifier is unknown would fall into the next surface's range,
+            // or (past the last surface) into none
+            if (row[S] != off + M.size) {
+                throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "measurement on an unknown surface");
+            }
+            for (uint32_t i = 0; i < M.size; ++i) {
+                if (D.ids[meas_sf[off + i]] != M.surface_link[i]) {
+                    throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "measurement on an unknown surface");
+                }
+            }
             // the candidate search may bisect a surface's measurements on
             // loc0 when they are sorted by it (traccc's measurement ordering)
             // and every one of them measures loc0
@@ -1420,6 +1444,41 @@ ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, dou
         if (ms_propagate) *ms_propagate = alg->t_prop;
         if (ms_other) *ms_other = alg->t_other;
         if (n_steps) *n_steps = alg->n_prop_launches;
+    });
+}
+
+ckf_status ckf_release_cached_memory(void) {
+    return guarded([&] { result_cache().release(); });
+}
+
+ckf_status ckf_debug_set_verdict_log(ckf_algorithm* alg, uint32_t capacity) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        alg->verdict_cap = capacity;
+        if (capacity == 0u) {
+            alg->verdict_log.release();
+            alg->verdict_n.release();
+        }
+    });
+}
+
+ckf_status ckf_debug_get_verdicts(const ckf_algorithm* alg, uint64_t* keys, uint32_t capacity,
+                                  uint32_t* n_records) {
+    return guarded([&] {
+        if (alg == nullptr || n_records == nullptr) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
+        }
+        *n_records = 0u;
+        if (alg->verdict_cap == 0u || alg->verdict_n.p == nullptr) return;
+        uint32_t n = 0u;
+        hip_check(hipStreamSynchronize(alg->stream), "sync");
+        hip_check(hipMemcpy(&n, alg->verdict_n.p, 4u, hipMemcpyDeviceToHost), "verdict count");
+        *n_records = n;
+        const uint32_t m = std::min({n, capacity, alg->verdict_cap});
+        if (m > 0u && keys != nullptr) {
+            hip_check(hipMemcpy(keys, alg->verdict_log.p, 8u * size_t(m), hipMemcpyDeviceToHost),
+                      "verdict download");
+        }
     });
 }
 
