@@ -56,10 +56,16 @@ ckf_status ckf_debug_get_schedule(const ckf_algorithm* alg, uint32_t* lanes_per_
 ckf_status ckf_debug_mfma_layout(const float* A, const float* B, float* D);
 ckf_status ckf_debug_kernel_times(const ckf_algorithm* alg, double* ms_find, double* ms_dedup,
                                   double* ms_propagate, double* ms_other, uint32_t* n_steps);
-/* Device time of the branch scorer's candidate-list kernel and of its MLP
- * kernel in the last timed call (0 without a scorer); not included in
- * ms_find. */
-ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list, double* ms_score);
+/* The find group of the last timed call split into its two kernels: the
+ * find kernel (plain, or scorer-pruned: candidate lists, the MLP and the
+ * updates in one kernel) and its link compaction; their sum is ms_find. */
+ckf_status ckf_debug_find_split(const ckf_algorithm* alg, double* ms_find_kernel,
+                                double* ms_find_emit);
+/* The propagation group of the last timed call split into the propagation
+ * kernel (k_propagate at the schedule's team size) and its compaction; their
+ * sum is ms_propagate. */
+ckf_status ckf_debug_propagate_split(const ckf_algorithm* alg, double* ms_propagate_kernel,
+                                     double* ms_propagate_emit);
 
 /* Candidate-verdict log of scorer-pruned calls, for replay by a CPU
  * checker: while the capacity is above 0 every ckf_find_tracks call with a

@@ -105,7 +105,7 @@ struct step_ctl {
                                 // progressive filter, [2] free, [3] build, [4] find emit,
                                 // [5] propagate emit, [6] dedup emit, [7] free, [8] list emit
     line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
-    line_u32 n_rows;      // scorer rows listed this step (written by k_list_emit)
+    line_u32 n_rows;      // scorer rows scored this step (written by k_find_scored)
     line_u64 n_props;     // propagations so far
     line_u32 xcd_ticket[4][8];  // per-XCD work queues (one counter per XCD): find, scorer
                                 // list, propagate, dedup mark
@@ -195,21 +195,14 @@ struct ckf_params {
     float* log_feat;              // [link_cap][12] or null
     uint8_t* log_label;           // [link_cap] or null
     uint64_t* seed_best;          // [n_seeds] best kept tip of each seed (logging)
-    // scorer-pruned finding: per input the list_len best candidates in
-    // (chi2, index) order with their features, scored before the updates
-    uint32_t scored;              // 1: k_find takes its candidates from the list
+    // scorer-pruned finding (k_find_scored): per input the list_len best
+    // candidates in (chi2, index) order are scored before the updates
+    uint32_t scored;              // 1: the scorer-pruned find runs
     uint32_t list_len;
-    uint32_t* list_m;             // [in_cap][list_len] measurement
-    float* list_c;                // [in_cap][list_len] chi2
     uint32_t* find_count;         // [in_cap] branches each input keeps this step
     uint8_t* prop_flag;           // [state_cap] propagation outcome: bit 0 next input, bits 1-2 tips
-    uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
-    uint32_t* list_seed;          // [in_cap] seed of each listed input
-    uint32_t* list_n;             // [in_cap] entries
-    uint32_t* row_slot;           // [in_cap * list_len] list slot (input * list_len + q) of each
-                                  // scorer row, rows packed over the step by k_list_emit
-    float* list_feat;             // [in_cap][list_len][12] features of each listed candidate
-    uint8_t* cand_keep;           // [in_cap][list_len] scorer verdict
+    uint64_t* seed_run_min;       // [2][seeds] scorer feature 9: lowest running chi2 / ndf of
+                                  // the seed's links, by step parity
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
     // candidate-verdict log (diagnostics): one key per scored candidate,

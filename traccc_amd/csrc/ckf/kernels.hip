@@ -403,35 +403,201 @@ __device__ inline uint32_t emit_chunks(uint32_t n) {
     return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
 }
 
-// Scorer-pruned finding, first kernel: per input the list_len best candidates
-// in (chi2, index) order and their features, at the input's fixed slots
-// input * list_len + q; k_list_emit then packs the rows for the MLP.
-__global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
-    step_ctl* ctl = P.ctl;
-    if (ctl->done) {
+// Diagnostics: one scored candidate's verdict as a key a CPU checker replays.
+// The key names the candidate by its branch's history, so it does not depend
+// on where the branch sits in the step's input order: splitmix64 chained over
+// the seed, the step, the measurement of every earlier link of the branch
+// (latest first, holes as all-ones) and the candidate's measurement; low 63
+// bits, bit 63 = kept.
+__device__ inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
+    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t seed,
+                                   uint32_t prev_g, uint32_t meas, bool keep) {
+    const uint32_t at = atomicAdd(P.verdict_n, 1u);
+    if (at >= P.verdict_cap) {
         return;
     }
+    uint64_t h = verdict_mix(0u, seed);
+    h = verdict_mix(h, step);
+    uint32_t g = prev_g;
+    for (uint32_t k = 0; k < step && g < P.link_cap; ++k) {
+        const dlink L = P.links[g];
+        h = verdict_mix(h, L.meas);
+        if (link_step(L) == 0u) {
+            break;
+        }
+        g = L.prev;
+    }
+    h = verdict_mix(h, meas);
+    P.verdict_log[at] = (h & 0x7fffffffffffffffull) | (keep ? (1ull << 63) : 0ull);
+}
+
+// Scorer-pruned find, one kernel per step.  A team of kListTeam lanes takes
+// one input: the candidate stream gives its list_len best candidates in
+// (chi2, index) order (candidate q held by team lane q % kListTeam in slot
+// q / kListTeam); their features, normalised and rounded to BF16, are packed
+// densely into the wave's LDS rows (teams in input order, so a wave's 16
+// inputs fill ceil(rows / 16) sub-tiles); the MLP scores the rows 16 at a
+// time on the matrix cores, and each sub-tile's verdicts go into a
+// wave-uniform bit mask by one ballot; then each team updates its kept
+// candidates in list order -- one round of up to kListTeam candidates (one
+// per lane) at a time, in parallel -- until N succeed, and stages them as the
+// plain find does (k_find_emit builds the links).  Each input also offers the
+// lowest running chi2 / ndf of the links it makes to its seed's entry for the
+// next step (scorer feature 9).
+constexpr int kRowShorts = 16;  // a feature row: 12 BF16 features, 4 zero
+constexpr uint32_t kScoredPerWave = kWave / kListTeam;
+constexpr int kMaskWords = (kScoredPerWave * kListTeam * kListSlots + 63) / 64;
+
+// Dynamic LDS of the scored find: every wave's feature rows.
+inline size_t scored_find_lds(uint32_t list_len, uint32_t block) {
+    return static_cast<size_t>(block / kWave) * kScoredPerWave * list_len * kRowShorts *
+           sizeof(short);
+}
+
+// The scored find's update of one input (its team, after the MLP): the kept
+// candidates in list order, one round of kListTeam at a time (one per lane,
+// in parallel), until N updates succeed; staging as the plain find, and the
+// offer of the lowest running chi2 / ndf of the input's links for the next
+// step's feature 9.
+__device__ __forceinline__ void scored_update(const ckf_params& P, const find_input& in,
+                                              uint32_t step, uint32_t i, uint32_t epoch,
+                                              uint32_t N, uint32_t k, uint32_t base,
+                                              const uint64_t (&keep_bits)[kMaskWords],
+                                              const uint32_t (&sm)[kListSlots],
+                                              const float (&sc)[kListSlots], uint64_t* run_next) {
+    step_ctl* ctl = P.ctl;
+    const uint32_t l = lane();
+    const uint32_t team = l / kListTeam;
+    const uint32_t tl = l % kListTeam;
+    const bool leader = (tl == 0u);
+    auto kept_bit = [&](uint32_t r) {
+        uint64_t w = keep_bits[0];
+#pragma unroll
+        for (int x = 1; x < kMaskWords; ++x) {
+            if (static_cast<uint32_t>(x) == r / 64u) w = keep_bits[x];
+        }
+        return ((w >> (r % 64u)) & 1ull) != 0ull;
+    };
+    if (P.verdict_log != nullptr) {
+#pragma unroll
+        for (int j = 0; j < kListSlots; ++j) {
+            const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
+            if (q < k) {
+                log_verdict(P, step, in.seed, in.prev_g, sm[j], kept_bit(base + q));
+            }
+        }
+    }
+    uint32_t count = 0u;
+    float run_links = __int_as_float(0x7f800000);
+#pragma unroll
+    for (int j = 0; j < kListSlots; ++j) {
+        if (static_cast<uint32_t>(j * kListTeam) >= k || count >= N) {
+            break;  // team-uniform
+        }
+        const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
+        const bool kept = q < k && kept_bit(base + q);
+        bool ok = false;
+        float fx[6], fc[21];
+        uint32_t dims = 0u;
+        if (kept) {
+            const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
+            dims = md.dims;
+#pragma unroll
+            for (int e = 0; e < 6; ++e) fx[e] = in.x[e];
+#pragma unroll
+            for (int e = 0; e < 21; ++e) fc[e] = in.c[e];
+            int res = CKF_FIT_SUCCESS;
+            if (in.rule0) {
+                // step 0 on a surface without material: keep the seed,
+                // take the measurement variances
+                fc[sym_idx(0, 0)] = md.v0;
+                fc[sym_idx(1, 1)] = md.v1;
+            } else {
+                res = gain_update(md, fx, fc);
+            }
+            ok = res == CKF_FIT_SUCCESS;
+        }
+        // this round's successes in list order (the team's lanes)
+        const uint32_t tm =
+            static_cast<uint32_t>(wave_ballot(ok) >> (team * kListTeam)) & ((1u << kListTeam) - 1u);
+        const uint32_t before = __builtin_popcount(tm & ((1u << tl) - 1u));
+        if (ok && count + before < N) {
+            const uint32_t slot = i * N + count + before;
+            if (slot < P.flt_cap) {
+                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
+                P.stage_slot[slot] = make_stage_cand(sm[j], sc[j], dims);
+            } else {
+                set_overflow(ctl, 4u);
+            }
+            run_links = fminf(run_links, branch_run(in.chi2_prev + sc[j], in.ndf_prev + dims));
+        }
+        count += __builtin_popcount(tm);
+        count = count < N ? count : N;
+    }
+    if (count == 0u) {
+        run_links = branch_run(in.chi2_prev, in.ndf_prev);  // the input's hole link
+    }
+#pragma unroll
+    for (int d = kListTeam / 2; d >= 1; d >>= 1) {
+        run_links = fminf(run_links, __shfl_xor(run_links, d, kListTeam));
+    }
+    if (leader) {
+        P.stage_in[i] = make_stage_head(in, step, i);
+        if (count == 0u) {
+            // no candidate kept: the input continues as one hole link
+            const uint32_t slot = i * N;
+            if (slot < P.flt_cap) {
+                P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
+            } else {
+                set_overflow(ctl, 4u);
+            }
+        }
+        P.find_count[i] = count;
+        standing_offer(run_next, in.seed, epoch + 1u, run_links);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_find_scored(ckf_params P) {
+    extern __shared__ __attribute__((aligned(16))) short s_rows[];
+    __shared__ mlp_vecs V;
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;  // the whole grid returns: no barrier is left waiting
+    }
+    mlp_fill_vecs(P.scorer, V, threadIdx.x, blockDim.x);
+    __syncthreads();
+    mlp_frags F;
+    mlp_load_frags(P.scorer, F);
     const uint32_t step = ctl->step;
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t prev_base = ctl->prev_link_base;
-    const uint32_t team = lane() / kListTeam;
-    const uint32_t tl = lane() % kListTeam;
-    const uint32_t Lc = P.list_len;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t n_parts = (n_in + (kWave / kListTeam) - 1) / (kWave / kListTeam);
-    // dynamic partitions (dense surfaces make some inputs far heavier)
+    const uint32_t N = P.n_branch_surface;
+    const uint32_t Lc = P.list_len;
+    const uint32_t l = lane();
+    const uint32_t team = l / kListTeam;
+    const uint32_t tl = l % kListTeam;
+    const bool leader = (tl == 0u);
+    short* rows = s_rows + static_cast<size_t>(threadIdx.x / kWave) * kScoredPerWave * Lc * kRowShorts;
+    const uint64_t* run_now = standing_read(P, step);
+    uint64_t* run_next = standing_write(P, step);
+    const uint32_t n_parts = (n_in + kScoredPerWave - 1u) / kScoredPerWave;
     xcd_ticket_run queue(&ctl->xcd_ticket[1][0].v, n_parts);
     for (;;) {
         const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i0 = p * (kWave / kListTeam);
-        const uint32_t i = i0 + team;
+        const uint32_t i = p * kScoredPerWave + team;
         const bool active = i < n_in;
         find_input in;
         uint32_t n_pass = 0u, k = 0u;
-        // candidate q of the list is held by team lane q % kListTeam in slot q / kListTeam
         uint32_t sm[kListSlots];
         float sc[kListSlots];
         if (active) {
@@ -455,207 +621,76 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                 }
             }
         }
+        // the wave's rows: the teams' lists packed in input order
+        const uint32_t kk = (active && leader) ? k : 0u;
+        const uint32_t incl = wave_inclusive_scan(kk);
+        const uint32_t total = wave_broadcast(incl, kWave - 1);
+        const uint32_t base = __shfl(incl - kk, static_cast<int>(team * kListTeam), kWave);
         if (active) {
-            if (tl == 0u) {
-                P.list_n[i] = k;
-                P.list_seed[i] = in.seed;
-            }
-            // the team's lanes write the list and the features of their
-            // candidates, at the input's fixed slots i * list_len + q
+            const float run_min = standing_min(run_now, in.seed, epoch);
 #pragma unroll
             for (int j = 0; j < kListSlots; ++j) {
                 const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
                 if (q < k) {
-                    const size_t r = static_cast<size_t>(i) * Lc + q;
-                    P.list_m[r] = sm[j];
-                    P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
+                    float f[kFeatures];
                     candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
-                                       P.list_feat + r * kFeatures);
-                    if (q == 0u) {
-                        standing_offer(P.seed_run_min, in.seed, epoch,
-                                       branch_run(in.chi2_prev, in.ndf_prev, sc[j], md.dims));
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass, run_min, f);
+                    short* row = rows + static_cast<size_t>(base + q) * kRowShorts;
+                    bf16x4 v[4];
+#pragma unroll
+                    for (int e = 0; e < kRowShorts; ++e) {
+                        v[e / 4][e % 4] = e < kFeatures ? mlp_feature_bf(V, f[e], e) : short(0);
                     }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) reinterpret_cast<bf16x4*>(row)[e] = v[e];
                 }
             }
         }
-    }
-}
-
-// Scorer rows of the step: every input's listed candidates laid out densely
-// in input order (one look-back per partition of up to 1,024 inputs) as row
-// -> list slot, so k_score runs on full 64-row tiles.  Kept apart from
-// k_find_list so no wave of the list waits on a shared row counter.
-__global__ __launch_bounds__(kBlock) void k_list_emit(ckf_params P) {
-    step_ctl* ctl = P.ctl;
-    if (ctl->done) {
-        return;
-    }
-    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
-    const uint32_t epoch = ctl->epoch;
-    const uint32_t Lc = P.list_len;
-    const uint32_t kc = emit_chunks(n_in);
-    const uint32_t part = kWave * kc;
-    const uint32_t n_parts = (n_in + part - 1u) / part;
-    const bool takes = wave_takes_tickets(n_parts);
-    for (;;) {
-        const uint32_t p = takes ? wave_ticket(&ctl->ticket[8].v) : n_parts;
-        if (p >= n_parts) {
-            break;
-        }
-        const uint32_t i0 = p * part;
-        uint32_t cnt[kEmitChunks];
-        uint32_t total = 0u;
+        wave_lds_sync();
+        // the MLP over the rows, 16 per sub-tile; verdict bits by row
+        uint64_t keep_bits[kMaskWords];
 #pragma unroll
-        for (int ch = 0; ch < static_cast<int>(kEmitChunks); ++ch) {
-            const uint32_t i = i0 + static_cast<uint32_t>(ch) * kWave + lane();
-            cnt[ch] = (static_cast<uint32_t>(ch) < kc && i < n_in) ? P.list_n[i] : 0u;
-            total += wave_broadcast(wave_inclusive_scan(cnt[ch]), kWave - 1);
-        }
-        const uint64_t b = wave_lookback(P.lb_lo[4], nullptr, p, epoch, pack2(total, 0u));
-        if (p == n_parts - 1u && lane() == 0u) {
-            ctl->n_rows.v = lo32(b) + total;
-        }
-        uint32_t run = lo32(b);
-#pragma unroll 1
-        for (uint32_t ch = 0; ch < kc; ++ch) {
-            // select cnt[ch] without a runtime index into the array
-            uint32_t c = 0u;
+        for (int w = 0; w < kMaskWords; ++w) keep_bits[w] = 0u;
+        const uint32_t n_sub = (total + 15u) / 16u;
+        for (uint32_t t = 0; t < n_sub; ++t) {
+            const uint32_t r = t * 16u + (l % 16u);
+            bf16x4 xb = {0, 0, 0, 0};
+            if (r < total) {
+                xb = load4(rows + static_cast<size_t>(r) * kRowShorts + 4u * (l / 16u));
+            }
+            const float s = mlp_logit16_bf(F, V, xb);
+            const bool keep = l < 16u && r < total && s >= P.score_threshold;
+            const uint64_t bits = (wave_ballot(keep) & 0xffffull) << ((t % 4u) * 16u);
 #pragma unroll
-            for (int q = 0; q < static_cast<int>(kEmitChunks); ++q) {
-                if (static_cast<uint32_t>(q) == ch) c = cnt[q];
+            for (int w = 0; w < kMaskWords; ++w) {
+                if (static_cast<uint32_t>(w) == t / 4u) keep_bits[w] |= bits;
             }
-            const uint32_t i = i0 + ch * kWave + lane();
-            const uint32_t incl = wave_inclusive_scan(c);
-            const uint32_t first = run + incl - c;
-            for (uint32_t q = 0; q < c; ++q) {
-                P.row_slot[first + q] = i * Lc + q;
-            }
-            run += wave_broadcast(incl, kWave - 1);
+        }
+        wave_lds_sync();  // the next partition's rows overwrite these
+        if (l == 0u && total > 0u) {
+            atomicAdd(&ctl->n_rows.v, total);
+        }
+        if (active) {
+            scored_update(P, in, step, i, epoch, N, k, base, keep_bits, sm, sc, run_next);
         }
     }
 }
 
-// Diagnostics: one scored candidate's verdict as a key a CPU checker replays.
-// The key names the candidate by its branch's history, so it does not depend
-// on where the branch sits in the step's input order: splitmix64 chained over
-// the seed, the step, the measurement of every earlier link of the branch
-// (latest first, holes as all-ones) and the candidate's measurement; low 63
-// bits, bit 63 = kept.
-__device__ inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
-    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
 
-__device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t input,
-                                   uint32_t meas, bool keep) {
-    const uint32_t at = atomicAdd(P.verdict_n, 1u);
-    if (at >= P.verdict_cap) {
-        return;
-    }
-    uint64_t h = verdict_mix(0u, P.list_seed[input]);
-    h = verdict_mix(h, step);
-    if (step > 0u) {
-        uint32_t g = P.ctl->prev_link_base + P.live[input];
-        for (uint32_t k = 0; k < step && g < P.link_cap; ++k) {
-            const dlink L = P.links[g];
-            h = verdict_mix(h, L.meas);
-            if (link_step(L) == 0u) {
-                break;
-            }
-            g = L.prev;
-        }
-    }
-    h = verdict_mix(h, meas);
-    P.verdict_log[at] = (h & 0x7fffffffffffffffull) | (keep ? (1ull << 63) : 0ull);
-}
-
-// Scorer-pruned finding, second kernel: the MLP over every listed candidate
-// (64-row tiles on the matrix cores), keep = logit >= threshold.
-__global__ __launch_bounds__(kBlock) void k_score(ckf_params P) {
-    __shared__ mlp_vecs V;
-    const step_ctl* ctl = P.ctl;
-    if (ctl->done) {
-        return;  // the whole grid returns: no barrier is left waiting
-    }
-    mlp_fill_vecs(P.scorer, V, threadIdx.x, blockDim.x);
-    __syncthreads();
-    mlp_frags F;
-    mlp_load_frags(P.scorer, F);
-    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
-    const uint32_t cap = n_in * P.list_len;
-    const uint32_t rows = ctl->n_rows.v < cap ? ctl->n_rows.v : cap;
-    const uint32_t epoch = ctl->epoch;
-    const uint32_t Lc = P.list_len;
-    const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t l = lane();
-    const uint32_t r16 = l % 16u;
-    const uint32_t grp = l / 16u;  // features 4 grp .. 4 grp + 3
-    const uint32_t n_tiles = (rows + 63u) / 64u;
-    for (uint32_t t = blockIdx.x * (blockDim.x / kWave) + wave; t < n_tiles;
-         t += gridDim.x * (blockDim.x / kWave)) {
-        float x[4][4];
-        bool valid[4];
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            const uint32_t r = t * 64u + 16u * mt + r16;
-            valid[mt] = r < rows;
-            const uint32_t slot = valid[mt] ? P.row_slot[r] : 0u;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t k = 4u * grp + static_cast<uint32_t>(e);
-                x[mt][e] = (valid[mt] && k < static_cast<uint32_t>(kMlpIn))
-                               ? P.list_feat[static_cast<size_t>(slot) * kFeatures + k] : 0.f;
-            }
-            if (valid[mt] && grp == static_cast<uint32_t>(kFeatStanding / 4)) {
-                float& f = x[mt][kFeatStanding % 4];
-                f = standing_of(P.seed_run_min, P.list_seed[slot / Lc], epoch, f);
-            }
-        }
-        // one sub-tile at a time (the next ones shift down) keeps a single
-        // MLP evaluation live, and the wave at 3 waves / SIMD
-        float mine = 0.f;
-#pragma unroll 1
-        for (int mt = 0; mt < 4; ++mt) {
-            const float s = mlp_logit16(F, V, x[0], valid[0]);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                valid[q] = valid[q + 1];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) x[q][e] = x[q + 1][e];
-            }
-            if (grp == static_cast<uint32_t>(mt)) mine = s;  // row 64 t + l
-        }
-        const uint32_t r = t * 64u + l;
-        if (r < rows) {
-            const uint32_t slot = P.row_slot[r];
-            const bool keep = mine >= P.score_threshold;
-            P.cand_keep[slot] = keep ? 1u : 0u;
-            if (P.verdict_log != nullptr) {
-                log_verdict(P, ctl->step, slot / Lc, P.list_m[slot], keep);
-            }
-        }
-    }
-}
-
-// A team of kFindTeam lanes handles one input branch.  Plain: the lanes split
-// the surface's measurements (scan_next) and take candidates in (chi2, index)
-// order; logged: the same, with every filtered slot's features kept for the
-// branch log; scored: the candidates come from the scored list, pruned ones
-// skipped -- there is no scan left to split, so a scored team is one lane and
-// a wave holds 64 inputs.  The gain update runs redundantly on the team (so
-// it stays uniform) and the team leader writes.  A wave holds kWave / kT
-// inputs.
-enum find_mode : int { kFindPlain = 0, kFindLogged = 1, kFindScored = 2 };
+// A team of kFindTeam lanes handles one input branch: the lanes split the
+// surface's measurements (the candidate stream) and take candidates in
+// (chi2, index) order.  Logged: the same, with every filtered slot's features
+// kept for the branch log, and each input's lowest new running chi2 / ndf
+// offered to its seed for the next step's feature 9.  The gain update runs
+// redundantly on the team (so it stays uniform) and the team leader writes.
+// A wave holds kWave / kT inputs.  (The scorer-pruned find is k_find_scored.)
+enum find_mode : int { kFindPlain = 0, kFindLogged = 1 };
 
 template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
-    constexpr bool kScored = (kMode == kFindScored);
     constexpr bool logging = (kMode == kFindLogged);
-    constexpr int kT = kScored ? 1 : kFindTeam;
+    constexpr int kT = kFindTeam;
     constexpr uint32_t kPerWave = kWave / kT;
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -681,110 +716,63 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
         uint32_t count = 0u;
         if (valid) {
             find_input in;
-            load_input<!kScored>(P, step, i, prev_base, in);
+            load_input(P, step, i, prev_base, in);
             if (leader) {
                 P.stage_in[i] = make_stage_head(in, step, i);
             }
-            uint32_t n_pass = 0u;
-            uint32_t li = 0u;
-            const uint32_t ln = kScored ? P.list_n[i] : 0u;
-            const uint32_t row0 = kScored ? i * P.list_len : 0u;  // the input's list slots
-            if constexpr (kScored) {
-                // one lane per input: the kept list entries in order.  The
-                // verdicts are read all at once (independent loads, one
-                // memory latency) into a mask of the kept entries.
-                constexpr int kListMax = kListTeam * kListSlots;
-                static_assert(kListMax <= 32, "kept-entry mask is 32 bits");
-                uint32_t kept = 0u;
-#pragma unroll
-                for (int q = 0; q < kListMax; ++q) {
-                    if (static_cast<uint32_t>(q) < ln && P.cand_keep[row0 + q] != 0u) {
-                        kept |= 1u << q;
-                    }
+            // candidates in stream order; the gain update runs on every
+            // lane of the team (uniform), the leader writes
+            cand_stream<6, kFindTeam> cs;
+            stream_init(P, in, tl, cs);
+            const uint32_t n_pass = cs.n_pass;
+            float run_min = -1.f;
+            float run_links = __int_as_float(0x7f800000);
+            if constexpr (logging) {
+                run_min = standing_min(standing_read(P, step), in.seed, ctl->epoch);
+            }
+            while (count < N) {
+                float best_c = 0.f;
+                uint32_t best_m = kInvalid;
+                stream_next(P, in, tl, cs, best_c, best_m);
+                if (best_m == kInvalid) {
+                    break;
                 }
-                while (count < N && kept != 0u) {
-                    li = static_cast<uint32_t>(__builtin_ctz(kept));
-                    kept &= kept - 1u;
-                    const size_t r = static_cast<size_t>(i) * P.list_len + li;
-                    const uint32_t best_m = P.list_m[r];
-                    const float best_c = P.list_c[r];
-                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
-                    float fx[6], fc[21];
+                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+                float fx[6], fc[21];
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
+                for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
 #pragma unroll
-                    for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
-                    int res = CKF_FIT_SUCCESS;
-                    if (in.rule0) {
-                        fc[sym_idx(0, 0)] = md.v0;
-                        fc[sym_idx(1, 1)] = md.v1;
-                    } else {
-                        res = gain_update(md, fx, fc);
-                    }
-                    if (res == CKF_FIT_SUCCESS) {
-                        const uint32_t slot = i * N + count;
+                for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
+                int res = CKF_FIT_SUCCESS;
+                if (in.rule0) {
+                    // step 0 on a surface without material: keep the
+                    // seed, take the measurement variances
+                    fc[sym_idx(0, 0)] = md.v0;
+                    fc[sym_idx(1, 1)] = md.v1;
+                } else {
+                    res = gain_update(md, fx, fc);
+                }
+                if (res == CKF_FIT_SUCCESS) {
+                    const uint32_t slot = i * N + count;
+                    if (leader) {
                         if (slot < P.flt_cap) {
                             store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
                             P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
+                            if constexpr (logging) {
+                                candidate_features(P, md, in.x, in.c, in.sf, step, in.skip,
+                                                   in.consec, in.chi2_prev, in.ndf_prev, best_c,
+                                                   count, n_pass, run_min,
+                                                   P.feat_slot + static_cast<size_t>(slot) * kFeatures);
+                            }
                         } else {
                             set_overflow(ctl, 4u);
                         }
-                        ++count;
                     }
-                }
-            } else {
-                // candidates in stream order; the gain update runs on every
-                // lane of the team (uniform), the leader writes
-                cand_stream<6, kFindTeam> cs;
-                stream_init(P, in, tl, cs);
-                n_pass = cs.n_pass;
-                bool rank0 = true;
-                while (count < N) {
-                    float best_c = 0.f;
-                    uint32_t best_m = kInvalid;
-                    stream_next(P, in, tl, cs, best_c, best_m);
-                    if (best_m == kInvalid) {
-                        break;
-                    }
-                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
                     if constexpr (logging) {
-                        if (rank0 && leader) {
-                            standing_offer(P.seed_run_min, in.seed, ctl->epoch,
-                                           branch_run(in.chi2_prev, in.ndf_prev, best_c, md.dims));
-                        }
-                        rank0 = false;
+                        run_links = fminf(run_links,
+                                          branch_run(in.chi2_prev + best_c, in.ndf_prev + md.dims));
                     }
-                    float fx[6], fc[21];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
-#pragma unroll
-                    for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
-                    int res = CKF_FIT_SUCCESS;
-                    if (in.rule0) {
-                        // step 0 on a surface without material: keep the
-                        // seed, take the measurement variances
-                        fc[sym_idx(0, 0)] = md.v0;
-                        fc[sym_idx(1, 1)] = md.v1;
-                    } else {
-                        res = gain_update(md, fx, fc);
-                    }
-                    if (res == CKF_FIT_SUCCESS) {
-                        const uint32_t slot = i * N + count;
-                        if (leader) {
-                            if (slot < P.flt_cap) {
-                                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
-                                P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
-                                if constexpr (logging) {
-                                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                                       in.chi2_prev, in.ndf_prev, best_c, count, n_pass,
-                                                       P.feat_slot + static_cast<size_t>(slot) * kFeatures);
-                                }
-                            } else {
-                                set_overflow(ctl, 4u);
-                            }
-                        }
-                        ++count;
-                    }
+                    ++count;
                 }
             }
             if (count == 0u && leader) {
@@ -794,6 +782,14 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                     P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
                 } else {
                     set_overflow(ctl, 4u);
+                }
+            }
+            if constexpr (logging) {
+                if (leader) {
+                    if (count == 0u) {
+                        run_links = branch_run(in.chi2_prev, in.ndf_prev);
+                    }
+                    standing_offer(standing_write(P, step), in.seed, ctl->epoch + 1u, run_links);
                 }
             }
         }
@@ -889,11 +885,8 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
                         if (L.meas != kInvalid) {
 #pragma unroll
                             for (int q = 0; q < kFeatures; ++q) {
-                                float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
-                                if (q == kFeatStanding) {
-                                    v = standing_of(P.seed_run_min, seed, epoch, v);
-                                }
-                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
+                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] =
+                                    P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
                             }
                         }
                     }
@@ -1679,37 +1672,19 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
+hipError_t launch_find_work(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.scored) {
-        hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
-        hipLaunchKernelGGL(k_list_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_score(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    if (P.scored) {
-        hipLaunchKernelGGL(k_score, dim3(s.score_blocks), dim3(kBlock), 0, stream, P);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    const hipError_t e = launch_list(P, s, stream);
-    if (e != hipSuccess) {
-        return e;
-    }
-    return launch_score(P, s, stream);
-}
-
-hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    if (P.scored) {
-        hipLaunchKernelGGL(k_find<kFindScored>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_find_scored, dim3(s.find_blocks), dim3(kBlock),
+                           scored_find_lds(P.list_len, kBlock), stream, P);
     } else if (P.feat_slot != nullptr) {
         hipLaunchKernelGGL(k_find<kFindLogged>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     } else {
         hipLaunchKernelGGL(k_find<kFindPlain>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_find_emit(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.feat_slot != nullptr) {
         hipLaunchKernelGGL(k_find_emit<true>, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P);
     } else {
@@ -1719,11 +1694,11 @@ hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStr
 }
 
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
-    const hipError_t e = launch_score_lists(P, s, stream);
+    const hipError_t e = launch_find_work(P, s, stream);
     if (e != hipSuccess) {
         return e;
     }
-    return launch_find_update(P, s, stream);
+    return launch_find_emit(P, s, stream);
 }
 
 // Branch log compaction: measurement links [0, n) in order, 1024 per block.
@@ -1786,7 +1761,21 @@ hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t 
 
 hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                             bool fused_advance) {
-    const int fa = fused_advance ? 1 : 0;
+    const hipError_t e = launch_propagate_work(P, s, stream);
+    if (e != hipSuccess) {
+        return e;
+    }
+    return launch_prop_emit(P, s, stream, fused_advance);
+}
+
+hipError_t launch_prop_emit(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                            bool fused_advance) {
+    hipLaunchKernelGGL(k_prop_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P,
+                       fused_advance ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate_work(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     switch (s.prop_team) {
         case 16:
             hipLaunchKernelGGL(k_propagate<16>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
@@ -1804,7 +1793,6 @@ hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStrea
             hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
             break;
     }
-    hipLaunchKernelGGL(k_prop_emit, dim3(s.emit_blocks), dim3(kBlock), 0, stream, P, fa);
     return hipGetLastError();
 }
 

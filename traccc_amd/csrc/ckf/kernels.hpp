@@ -13,8 +13,6 @@ struct launch_shape {
     uint32_t prop_blocks;
     uint32_t build_blocks;
     uint32_t prop_team;  // lanes per link in k_propagate: 1, 2, 4, 8 or 16
-    uint32_t list_blocks;   // scorer-pruned finding: candidate lists
-    uint32_t score_blocks;  // and the MLP over them
     uint32_t emit_blocks;   // the compaction kernels (k_*_emit)
 };
 
@@ -27,15 +25,18 @@ struct launch_shape {
 hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                            bool with_dedup);
 // the same step, kernel group by kernel group (timed runs)
+// launch_find = the find kernel (plain, logged, or scorer-pruned: candidate
+// lists, MLP and updates in one kernel), then its link compaction
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-// launch_find = the scorer's list, row-packing and score kernels (scored runs
-// only), then the find / update kernel and its link compaction
-hipError_t launch_score_lists(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-hipError_t launch_list(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-hipError_t launch_score(const ckf_params& P, const launch_shape& s, hipStream_t stream);
-hipError_t launch_find_update(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_find_work(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_find_emit(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_dedup(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_propagate(const ckf_params& P, const launch_shape& s, hipStream_t stream,
+                            bool fused_advance);
+// launch_propagate = the propagation kernel (at the schedule's team size),
+// then its compaction (k_prop_emit)
+hipError_t launch_propagate_work(const ckf_params& P, const launch_shape& s, hipStream_t stream);
+hipError_t launch_prop_emit(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                             bool fused_advance);
 hipError_t launch_advance(const ckf_params& P, hipStream_t stream);
 

@@ -29,11 +29,6 @@
 #include "traccc_amd/scorer.h"
 
 namespace tamd {
-std::string& last_error() {
-    thread_local std::string s;
-    return s;
-}
-
 block_cache& result_cache() {
     static block_cache* c = new block_cache();  // lives until the process ends
     return *c;
@@ -101,8 +96,7 @@ struct ckf_algorithm {
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
-    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, row_slot, list_feat,
-        cand_keep, find_count, seed_run_min, list_seed, prop_flag;
+    dbuf feat_slot, log_feat, log_label, seed_best, find_count, seed_run_min, prop_flag;
     bool logging = false;
     // candidate-verdict log of scored calls (diagnostics)
     dbuf verdict_log, verdict_n;
@@ -122,7 +116,8 @@ struct ckf_algorithm {
     uint32_t schedule_override = 0;   // fixed lanes per link, 0 = autotuned
     std::map<uint32_t, uint32_t> tuned;  // log2(seeds) -> lanes per link
     bool timing = false;
-    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0, t_score = 0.0, t_list = 0.0;
+    double t_find = 0.0, t_dedup = 0.0, t_prop = 0.0, t_other = 0.0, t_find_work = 0.0,
+           t_find_emit = 0.0, t_prop_work = 0.0, t_prop_emit = 0.0;
     uint32_t n_prop_launches = 0;
 
     ~ckf_algorithm() {
@@ -136,9 +131,8 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_lo, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &list_m, &list_c, &list_n, &row_slot, &list_feat, &find_count,
-                       &seed_run_min, &list_seed, &prop_flag,
-                       &cand_keep, &verdict_log, &verdict_n};
+                       &log_label, &seed_best, &find_count, &seed_run_min, &prop_flag,
+                       &verdict_log, &verdict_n};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < tamd::kLookbacks; ++k) {
             lb_lo[k].release();
@@ -220,20 +214,13 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         A.seed_best.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
     }
     if (A.logging || A.scorer != nullptr) {
-        A.seed_run_min.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
+        // two halves (step parity) of one entry per seed
+        A.seed_run_min.ensure(16u * size_t(std::max(ev.n_seeds, 1u)));
         // every key of a real step compares below all-ones
         hip_check(hipMemsetAsync(A.seed_run_min.p, 0xff, A.seed_run_min.bytes, A.stream), "memset");
     }
     if (A.scorer != nullptr) {
         A.list_len = std::min(A.cfg.max_num_branches_per_surface + 4u, 16u);
-        const size_t rows = size_t(A.in_cap) * A.list_len;
-        A.list_seed.ensure(4u * size_t(A.in_cap));
-        A.row_slot.ensure(4u * rows);
-        A.list_m.ensure(4u * rows);
-        A.list_c.ensure(4u * rows);
-        A.list_feat.ensure(4u * 12u * rows);
-        A.cand_keep.ensure(rows);
-        A.list_n.ensure(4u * size_t(A.in_cap));
     }
     if (A.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
@@ -322,14 +309,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     }
     if (A.scorer != nullptr) {
         P.scored = 1u;
-        P.list_seed = A.list_seed.as<uint32_t>();
         P.list_len = A.list_len;
-        P.list_m = A.list_m.as<uint32_t>();
-        P.list_c = A.list_c.as<float>();
-        P.list_n = A.list_n.as<uint32_t>();
-        P.row_slot = A.row_slot.as<uint32_t>();
-        P.list_feat = A.list_feat.as<float>();
-        P.cand_keep = A.cand_keep.as<uint8_t>();
         P.scorer = tamd::scorer_view(A.scorer);
         P.score_threshold = A.score_threshold;
         if (A.verdict_cap > 0u) {
@@ -352,17 +332,13 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     tamd::launch_shape s;
     // grids are upper bounds for the largest step: a wave beyond a step's
     // partitions leaves at once.  The find is sized for 16 inputs per wave
-    // (its two-lane teams hold 32, the scored update 64).
+    // (the scored find's four-lane teams; the plain find's two-lane teams
+    // hold 32).
     s.find_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
     s.prop_team = A.prop_team;
     const uint32_t prop_parts = (A.state_cap + 64u / s.prop_team - 1u) / (64u / s.prop_team);
     s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
-    // scorer-pruned finding: 16 inputs per wave listing, 64 rows per scoring wave
-    s.list_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
-    // packed rows: about one per input (the grid strides over any excess)
-    const uint64_t rows = 2u * uint64_t(A.in_cap);
-    s.score_blocks = static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>((rows / 64u + 4u) / 4u, max_blocks)));
     s.build_blocks = blocks(A.tip_cap);
     // compaction waves do little per partition: four blocks per CU
     s.emit_blocks = std::min(blocks(A.in_cap), static_cast<uint32_t>(A.n_cu) * 4u);
@@ -373,19 +349,20 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
 uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd::launch_shape& s) {
     hipEvent_t ev[7];
     for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
-    A.t_find = A.t_dedup = A.t_prop = A.t_other = A.t_score = A.t_list = 0.0;
+    A.t_find = A.t_dedup = A.t_prop = A.t_other = A.t_find_work = A.t_find_emit = 0.0;
+    A.t_prop_work = A.t_prop_emit = 0.0;
     A.n_prop_launches = 0u;
     for (uint32_t k = 0; k < A.cfg.max_track_candidates_per_track + 1u; ++k) {
-        hip_check(hipEventRecord(ev[5], A.stream), "record");
-        hip_check(tamd::launch_list(P, s, A.stream), "list");
-        hip_check(hipEventRecord(ev[6], A.stream), "record");
-        hip_check(tamd::launch_score(P, s, A.stream), "score");
         hip_check(hipEventRecord(ev[0], A.stream), "record");
-        hip_check(tamd::launch_find_update(P, s, A.stream), "find");
+        hip_check(tamd::launch_find_work(P, s, A.stream), "find");
+        hip_check(hipEventRecord(ev[5], A.stream), "record");
+        hip_check(tamd::launch_find_emit(P, s, A.stream), "find emit");
         hip_check(hipEventRecord(ev[1], A.stream), "record");
         hip_check(tamd::launch_dedup(P, s, A.stream), "dedup");
         hip_check(hipEventRecord(ev[2], A.stream), "record");
-        hip_check(tamd::launch_propagate(P, s, A.stream, false), "propagate");
+        hip_check(tamd::launch_propagate_work(P, s, A.stream), "propagate");
+        hip_check(hipEventRecord(ev[6], A.stream), "record");
+        hip_check(tamd::launch_prop_emit(P, s, A.stream, false), "propagate emit");
         hip_check(hipEventRecord(ev[3], A.stream), "record");
         hip_check(tamd::launch_advance(P, A.stream), "advance");
         hip_check(hipEventRecord(ev[4], A.stream), "record");
@@ -393,12 +370,17 @@ uint32_t run_steps_timed(ckf_algorithm& A, const tamd::ckf_params& P, const tamd
                                  A.stream),
                   "ctl readback");
         hip_check(hipStreamSynchronize(A.stream), "sync");
-        float ms[4], ms_list = 0.f, ms_score = 0.f;
+        float ms[4], ms_work = 0.f, ms_emit = 0.f;
         for (int j = 0; j < 4; ++j) hip_check(hipEventElapsedTime(&ms[j], ev[j], ev[j + 1]), "elapsed");
-        hip_check(hipEventElapsedTime(&ms_list, ev[5], ev[6]), "elapsed");
-        hip_check(hipEventElapsedTime(&ms_score, ev[6], ev[0]), "elapsed");
-        A.t_list += ms_list;
-        A.t_score += ms_score;
+        hip_check(hipEventElapsedTime(&ms_work, ev[0], ev[5]), "elapsed");
+        hip_check(hipEventElapsedTime(&ms_emit, ev[5], ev[1]), "elapsed");
+        A.t_find_work += ms_work;
+        A.t_find_emit += ms_emit;
+        float ms_pw = 0.f, ms_pe = 0.f;
+        hip_check(hipEventElapsedTime(&ms_pw, ev[2], ev[6]), "elapsed");
+        hip_check(hipEventElapsedTime(&ms_pe, ev[6], ev[3]), "elapsed");
+        A.t_prop_work += ms_pw;
+        A.t_prop_emit += ms_pe;
         A.t_find += ms[0];
         A.t_dedup += ms[1];
         A.t_prop += ms[2];
@@ -1052,7 +1034,6 @@ extern "C" {
 
 uint32_t ckf_abi_version(void) { return CKF_ABI_VERSION; }
 
-const char* ckf_last_error(void) { return tamd::last_error().c_str(); }
 
 ckf_status ckf_algorithm_create(const ckf_finding_config* cfg, ckf_algorithm** out) {
     return guarded([&] {
@@ -1482,11 +1463,21 @@ ckf_status ckf_debug_get_verdicts(const ckf_algorithm* alg, uint64_t* keys, uint
     });
 }
 
-ckf_status ckf_debug_scorer_time(const ckf_algorithm* alg, double* ms_list, double* ms_score) {
+ckf_status ckf_debug_propagate_split(const ckf_algorithm* alg, double* ms_propagate_kernel,
+                                     double* ms_propagate_emit) {
     return guarded([&] {
         if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
-        if (ms_list) *ms_list = alg->t_list;
-        if (ms_score) *ms_score = alg->t_score;
+        if (ms_propagate_kernel) *ms_propagate_kernel = alg->t_prop_work;
+        if (ms_propagate_emit) *ms_propagate_emit = alg->t_prop_emit;
+    });
+}
+
+ckf_status ckf_debug_find_split(const ckf_algorithm* alg, double* ms_find_kernel,
+                                double* ms_find_emit) {
+    return guarded([&] {
+        if (alg == nullptr) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null algorithm");
+        if (ms_find_kernel) *ms_find_kernel = alg->t_find_work;
+        if (ms_find_emit) *ms_find_emit = alg->t_find_emit;
     });
 }
 

@@ -14,7 +14,7 @@
 
 namespace tamd {
 
-// the calling thread's last error message (defined in ckf/runtime.hip)
+// the calling thread's last error message (defined in common/last_error.cpp)
 std::string& last_error();
 
 struct ckf_error : std::runtime_error {

@@ -6,8 +6,12 @@
 //   dW3 = H2^T dS,  dZ2 = dS W3^T * GELU'(Z2),  dW2 = H1^T dZ2,
 //   dZ1 = (dZ2 W2^T) * GELU'(Z1),  dW1 = X^T dZ1
 // with dZ2 / dZ1 rounded to BF16 as MFMA operands.  Every wave writes its
-// tile's full gradient to its own slot and a second kernel sums the slots in
-// a fixed order, so training is bitwise reproducible.
+// tile's full gradient to its own slot; the step kernel sums the slots in a
+// fixed order (16 waves per 64 parameters, each over a fixed stride of
+// tiles, combined in wave order), so training is bitwise reproducible, and
+// applies AdamW and writes the BF16 views of its parameters in the same
+// pass.  The batch index, the Adam step count and the epoch's permutation
+// live on the device, so a run of steps is one captured graph replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +40,22 @@ struct perm_map {
         return static_cast<uint32_t>((a * i + b) % n);
     }
 };
+
+// Training state on the device: the epoch's sample order, the batch the next
+// step trains on, Adam's step count and the blocks of the step kernel that
+// have finished.
+struct train_state {
+    perm_map perm;
+    uint32_t batch_idx;
+    uint32_t adam_t;
+    uint32_t done_blocks;
+    uint32_t pad;
+};
+
+__device__ inline uint32_t batch_rows(const train_state& ts, uint32_t n, uint32_t batch) {
+    const uint64_t row0 = static_cast<uint64_t>(ts.batch_idx) * batch;
+    return row0 >= n ? 0u : static_cast<uint32_t>(n - row0 < batch ? n - row0 : batch);
+}
 
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(mlp_dev W, const float* feats, uint32_t n,
                                                        float* out) {
@@ -84,13 +104,19 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(mlp_dev W, const float* fe
     }
 }
 
-// Gradient of one 64-sample tile per wave into grad_part[tile][kMlpParams].
+// Gradient of one 64-sample tile per wave into grad_part[tile][kMlpParams],
+// for the batch the training state names.
 __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* feats,
-                                                      const uint8_t* labels, perm_map perm,
-                                                      uint32_t row0, uint32_t rows,
-                                                      float pos_weight, float inv_norm,
+                                                      const uint8_t* labels,
+                                                      const train_state* state, uint32_t n,
+                                                      uint32_t batch, float pos_weight,
                                                       float* grad_part, float* loss_part) {
     __shared__ short lds[kWavesPerBlock * kMlpLdsPerWave];
+    const train_state ts = *state;
+    const perm_map perm = ts.perm;
+    const uint32_t row0 = ts.batch_idx * batch;
+    const uint32_t rows = batch_rows(ts, n, batch);
+    const float inv_norm = rows > 0u ? 1.f / static_cast<float>(rows) : 0.f;
     const uint32_t wave = threadIdx.x / 64u;
     short* lds_x = lds + wave * kMlpLdsPerWave;
     short* lds_h = lds_x + 64 * kMlpInPad;
@@ -267,37 +293,95 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
     }
 }
 
-// grad[p] = sum over tiles in order; loss = sum of the tile losses.
-__global__ void k_mlp_reduce(const float* grad_part, const float* loss_part, uint32_t n_tiles,
-                             float* grad, float* loss) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < static_cast<uint32_t>(kMlpParams)) {
-        float acc = 0.f;
-        for (uint32_t t = 0; t < n_tiles; ++t) acc += grad_part[static_cast<size_t>(t) * kMlpParams + p];
-        grad[p] = acc;
-    }
-    if (p == 0u) {
-        float acc = 0.f;
-        for (uint32_t t = 0; t < n_tiles; ++t) acc += loss_part[t];
-        *loss = acc;
+struct adam_hyper {
+    float lr, beta1, beta2, eps, wd;
+};
+
+// One optimiser step.  Block b owns parameters 64 b .. 64 b + 63 (one per
+// lane): its 16 waves each sum a fixed stride of the batch's tile gradients
+// (coalesced 256 B rows), the block combines them in wave order, then --
+// with apply != 0 -- AdamW updates the parameter and writes its BF16 / FP32
+// views (the k_mlp_pack layouts).  Block 0 also sums the tile losses into
+// losses[batch_idx].  The last block to finish advances the training state.
+constexpr int kStepWaves = 16;
+
+__device__ inline void pack_param(uint32_t p, float w, short* w1t, short* w2, short* w2t,
+                                  float* b1, float* b2, float* w3, float* b3) {
+    if (p < static_cast<uint32_t>(kOffB1)) {
+        const uint32_t in = p / kMlpHidden, o = p % kMlpHidden;
+        w1t[o * kMlpInPad + in] = f2bf(w);
+    } else if (p < static_cast<uint32_t>(kOffW2)) {
+        b1[p - kOffB1] = w;
+    } else if (p < static_cast<uint32_t>(kOffB2)) {
+        const uint32_t r = (p - kOffW2) / kMlpHidden, c = (p - kOffW2) % kMlpHidden;
+        w2[r * kMlpHidden + c] = f2bf(w);   // [in r][out c]
+        w2t[c * kMlpHidden + r] = f2bf(w);  // [out c][in r]
+    } else if (p < static_cast<uint32_t>(kOffW3)) {
+        b2[p - kOffB2] = w;
+    } else if (p < static_cast<uint32_t>(kOffB3)) {
+        w3[p - kOffW3] = bf2f(f2bf(w));
+    } else {
+        b3[0] = w;
     }
 }
 
-struct adam_hyper {
-    float lr, beta1, beta2, eps, wd, bc1, bc2;
-};
-
-__global__ void k_mlp_adam(float* params, const float* grad, float* m, float* v, adam_hyper h) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= static_cast<uint32_t>(kMlpParams)) return;
-    const float g = grad[p];
-    const float mm = h.beta1 * m[p] + (1.f - h.beta1) * g;
-    const float vv = h.beta2 * v[p] + (1.f - h.beta2) * g * g;
-    m[p] = mm;
-    v[p] = vv;
-    const float mhat = mm / h.bc1;
-    const float vhat = vv / h.bc2;
-    params[p] -= h.lr * (mhat / (sqrtf(vhat) + h.eps) + h.wd * params[p]);
+__global__ __launch_bounds__(64 * kStepWaves) void k_mlp_step(
+    const float* grad_part, const float* loss_part, uint32_t n, uint32_t batch,
+    train_state* state, int apply, adam_hyper h, float* params, float* m, float* v,
+    float* grad_out, float* losses, short* w1t, short* w2, short* w2t, float* b1, float* b2,
+    float* w3, float* b3) {
+    __shared__ float s_part[kStepWaves][64];
+    const train_state ts = *state;
+    const uint32_t rows = batch_rows(ts, n, batch);
+    const uint32_t tiles = (rows + 63u) / 64u;
+    const uint32_t wave = threadIdx.x / 64u;
+    const uint32_t l = __lane_id();
+    const uint32_t p = blockIdx.x * 64u + l;
+    const bool real = p < static_cast<uint32_t>(kMlpParams);
+    float acc = 0.f;
+    for (uint32_t t = wave; t < tiles; t += kStepWaves) {
+        acc += real ? grad_part[static_cast<size_t>(t) * kMlpParams + p] : 0.f;
+    }
+    s_part[wave][l] = acc;
+    if (blockIdx.x == 0u && wave == 1u) {
+        float lo = 0.f;
+        for (uint32_t t = l; t < tiles; t += 64u) lo += loss_part[t];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) lo += __shfl_xor(lo, d, 64);
+        if (l == 0u) losses[ts.batch_idx] = lo;
+    }
+    __syncthreads();
+    if (wave == 0u && real) {
+        float g = 0.f;
+#pragma unroll
+        for (int w = 0; w < kStepWaves; ++w) g += s_part[w][l];
+        if (grad_out != nullptr) grad_out[p] = g;
+        if (apply) {
+            const float t = static_cast<float>(ts.adam_t + 1u);
+            const float bc1 = 1.f - powf(h.beta1, t);
+            const float bc2 = 1.f - powf(h.beta2, t);
+            const float mm = h.beta1 * m[p] + (1.f - h.beta1) * g;
+            const float vv = h.beta2 * v[p] + (1.f - h.beta2) * g * g;
+            m[p] = mm;
+            v[p] = vv;
+            const float w = params[p] - h.lr * ((mm / bc1) / (sqrtf(vv / bc2) + h.eps) +
+                                                h.wd * params[p]);
+            params[p] = w;
+            pack_param(p, w, w1t, w2, w2t, b1, b2, w3, b3);
+        }
+    }
+    // the last block to finish moves the state to the next batch (every
+    // block read the state at its start, before counting itself)
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        __threadfence();
+        const uint32_t before = atomicAdd(&state->done_blocks, 1u);
+        if (before + 1u == gridDim.x) {
+            state->done_blocks = 0u;
+            state->batch_idx = ts.batch_idx + 1u;
+            state->adam_t = ts.adam_t + (apply ? 1u : 0u);
+        }
+    }
 }
 
 // BF16 / FP32 views of the master parameters in the fragment layouts.
@@ -321,26 +405,62 @@ __global__ void k_mlp_pack(const float* params, short* w1t, short* w2, short* w2
     if (t == 0u) b3[0] = params[kOffB3];
 }
 
-// Per-feature sums of x and x^2 (double) for the normalisation.
-__global__ void k_mlp_moments(const float* feats, uint32_t n, double* acc) {
-    __shared__ double s[2 * kMlpIn];
-    for (int k = threadIdx.x; k < 2 * kMlpIn; k += blockDim.x) s[k] = 0.0;
-    __syncthreads();
-    double a[kMlpIn], b[kMlpIn];
-    for (int k = 0; k < kMlpIn; ++k) a[k] = b[k] = 0.0;
+// Per-feature count / mean / M2 for the normalisation in FP32 with Welford's
+// update per thread and Chan, Golub and LeVeque's pairwise combination
+// across the block (fixed order); one (count, mean[12], M2[12]) record per
+// block, combined the same way on the host.
+struct moments {
+    float n;
+    float mean[kMlpIn];
+    float m2[kMlpIn];
+};
+
+__device__ __host__ inline void moments_merge(moments& a, const moments& b) {
+    const float n = a.n + b.n;
+    if (b.n == 0.f) return;
+    if (a.n == 0.f) {
+        a = b;
+        return;
+    }
+    const float wb = b.n / n;
+    const float cross = a.n * b.n / n;
+    for (int k = 0; k < kMlpIn; ++k) {
+        const float d = b.mean[k] - a.mean[k];
+        a.mean[k] += d * wb;
+        a.m2[k] += b.m2[k] + d * d * cross;
+    }
+    a.n = n;
+}
+
+constexpr int kMomentThreads = 256;
+
+__global__ __launch_bounds__(kMomentThreads) void k_mlp_moments(const float* feats, uint32_t n,
+                                                                moments* out) {
+    __shared__ moments s[kMomentThreads];
+    moments me;
+    me.n = 0.f;
+    for (int k = 0; k < kMlpIn; ++k) me.mean[k] = me.m2[k] = 0.f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        me.n += 1.f;
+        const float inv = 1.f / me.n;
         for (int k = 0; k < kMlpIn; ++k) {
-            const double x = feats[static_cast<size_t>(i) * kMlpIn + k];
-            a[k] += x;
-            b[k] += x * x;
+            const float x = feats[static_cast<size_t>(i) * kMlpIn + k];
+            const float d = x - me.mean[k];
+            me.mean[k] += d * inv;
+            me.m2[k] += d * (x - me.mean[k]);
         }
     }
-    for (int k = 0; k < kMlpIn; ++k) {
-        atomicAdd(&s[k], a[k]);
-        atomicAdd(&s[kMlpIn + k], b[k]);
-    }
+    s[threadIdx.x] = me;
     __syncthreads();
-    for (int k = threadIdx.x; k < 2 * kMlpIn; k += blockDim.x) atomicAdd(&acc[k], s[k]);
+    for (uint32_t w = kMomentThreads / 2; w >= 1u; w >>= 1) {
+        if (threadIdx.x < w) {
+            moments a = s[threadIdx.x];
+            moments_merge(a, s[threadIdx.x + w]);
+            s[threadIdx.x] = a;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0u) out[blockIdx.x] = s[0];
 }
 
 // One 16x16x16 product through the documented fragment layout (diagnostics).
@@ -360,6 +480,7 @@ __global__ void k_mfma_layout(const float* A, const float* B, float* D) {
 }
 
 }  // namespace
+
 }  // namespace tamd
 
 // ---------------------------------------------------------------------------
@@ -381,6 +502,7 @@ struct ckf_scorer {
     float* b3 = nullptr;
     float* mean = nullptr;
     float* inv_std = nullptr;
+    tamd::train_state* state = nullptr;  // device training state
     float h_mean[tamd::kMlpIn];
     float h_std[tamd::kMlpIn];
     uint64_t adam_t = 0;
@@ -439,20 +561,29 @@ tamd::perm_map make_perm(uint64_t n, uint64_t& rng, bool identity) {
     return tamd::perm_map{a, splitmix(rng) % n, n};
 }
 
-// gradient of `rows` samples starting at position row0 of the permutation;
-// result in S.grad / S.loss (device)
-void grad_batch(ckf_scorer& S, const float* feats, const uint8_t* labels, tamd::perm_map perm,
-                uint32_t row0, uint32_t rows, float pos_weight, float* part, float* loss_part,
-                hipStream_t st) {
-    const uint32_t tiles = (rows + 63u) / 64u;
+// Set the device training state (sample order, batch index, Adam steps).
+void set_state(ckf_scorer& S, const tamd::perm_map& perm, uint32_t batch_idx, hipStream_t st) {
+    tamd::train_state ts{perm, batch_idx, static_cast<uint32_t>(S.adam_t), 0u, 0u};
+    hip_check(hipMemcpyAsync(S.state, &ts, sizeof(ts), hipMemcpyHostToDevice, st), "scorer state");
+    hip_check(hipStreamSynchronize(st), "scorer state");
+}
+
+// One optimiser step on the batch the device state names: the tile
+// gradients, then their fixed-order sum with (apply) the AdamW update and the
+// BF16 views; the state moves to the next batch.
+void launch_step(ckf_scorer& S, const float* feats, const uint8_t* labels, uint32_t n,
+                 uint32_t batch, float pos_weight, int apply, const tamd::adam_hyper& h,
+                 float* part, float* loss_part, float* losses, float* grad_out, hipStream_t st) {
+    const uint32_t tiles = (batch + 63u) / 64u;
     const uint32_t blocks = (tiles + tamd::kWavesPerBlock - 1) / tamd::kWavesPerBlock;
     hipLaunchKernelGGL(tamd::k_mlp_grad, dim3(blocks), dim3(tamd::kBlock), 0, st, S.view(), feats,
-                       labels, perm, row0, rows, pos_weight, 1.f / static_cast<float>(rows), part,
-                       loss_part);
+                       labels, S.state, n, batch, pos_weight, part, loss_part);
     hip_check(hipGetLastError(), "scorer grad");
-    hipLaunchKernelGGL(tamd::k_mlp_reduce, dim3((tamd::kMlpParams + 255) / 256), dim3(256), 0, st,
-                       part, loss_part, tiles, S.grad, S.loss);
-    hip_check(hipGetLastError(), "scorer reduce");
+    hipLaunchKernelGGL(tamd::k_mlp_step, dim3((tamd::kMlpParams + 63) / 64),
+                       dim3(64 * tamd::kStepWaves), 0, st, part, loss_part, n, batch, S.state,
+                       apply, h, S.params, S.m, S.v, grad_out, losses, S.w1t, S.w2, S.w2t, S.b1,
+                       S.b2, S.w3, S.b3);
+    hip_check(hipGetLastError(), "scorer step");
 }
 
 }  // namespace
@@ -479,8 +610,9 @@ ckf_status ckf_scorer_create(uint64_t seed, ckf_scorer** out) {
         if (!out) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
         auto* S = new ckf_scorer();
         const size_t P = tamd::kMlpParams;
-        // floats: params, m, v, grad, loss(4), b1, b2, w3, b3(4), mean, inv_std; shorts after
-        const size_t nf = 4 * P + 4 + 64 * 3 + 4 + 2 * tamd::kMlpIn;
+        // floats: params, m, v, grad, loss(4), b1, b2, w3, b3(4), mean, inv_std, the
+        // training state (12 words, 8-byte aligned); shorts after
+        const size_t nf = 4 * P + 4 + 64 * 3 + 4 + 2 * tamd::kMlpIn + 12;
         const size_t ns = 64 * 16 + 64 * 64 * 2;
         if (hipMalloc(&S->d, nf * 4 + ns * 2 + 256) != hipSuccess) {
             delete S;
@@ -498,6 +630,12 @@ ckf_status ckf_scorer_create(uint64_t seed, ckf_scorer** out) {
         S->b3 = f; f += 4;
         S->mean = f; f += tamd::kMlpIn;
         S->inv_std = f; f += tamd::kMlpIn;
+        S->state = reinterpret_cast<tamd::train_state*>(f); f += 12;
+        static_assert(sizeof(tamd::train_state) <= 12 * sizeof(float), "state words");
+        if (reinterpret_cast<uintptr_t>(S->state) % alignof(tamd::train_state) != 0u) {
+            delete S;
+            throw ckf_error(CKF_ERR_INTERNAL, "training state misaligned");
+        }
         short* sh = reinterpret_cast<short*>(f);
         S->w1t = sh; sh += 64 * 16;
         S->w2 = sh; sh += 64 * 64;
@@ -558,23 +696,29 @@ ckf_status ckf_scorer_fit_normalisation(ckf_scorer* s, const float* d_features, 
         if (!s || (!d_features && n)) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "null argument");
         if (n == 0) return;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        double* acc = nullptr;
-        hip_check(hipMalloc(&acc, 2 * tamd::kMlpIn * sizeof(double)), "scorer moments");
-        hip_check(hipMemsetAsync(acc, 0, 2 * tamd::kMlpIn * sizeof(double), st), "scorer moments");
-        hipLaunchKernelGGL(tamd::k_mlp_moments, dim3(512), dim3(256), 0, st, d_features, n, acc);
-        double h[2 * tamd::kMlpIn];
+        constexpr uint32_t kBlocks = 512;
+        tamd::moments* part = nullptr;
+        hip_check(hipMalloc(&part, kBlocks * sizeof(tamd::moments)), "scorer moments");
+        hipLaunchKernelGGL(tamd::k_mlp_moments, dim3(kBlocks), dim3(tamd::kMomentThreads), 0, st,
+                           d_features, n, part);
+        std::vector<tamd::moments> h(kBlocks);
         const hipError_t e1 = hipGetLastError();
-        const hipError_t e2 = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = hipMemcpyAsync(h.data(), part, kBlocks * sizeof(tamd::moments),
+                                             hipMemcpyDeviceToHost, st);
         const hipError_t e3 = hipStreamSynchronize(st);
-        (void)hipFree(acc);
+        (void)hipFree(part);
         hip_check(e1, "scorer moments");
         hip_check(e2, "scorer moments");
         hip_check(e3, "scorer moments");
+        // pairwise (tree) combination of the block records, as on the device
+        for (uint32_t w = kBlocks / 2; w >= 1u; w >>= 1) {
+            for (uint32_t b = 0; b < w; ++b) tamd::moments_merge(h[b], h[b + w]);
+        }
         for (int k = 0; k < tamd::kMlpIn; ++k) {
-            const double mu = h[k] / n;
-            const double var = std::max(h[tamd::kMlpIn + k] / n - mu * mu, 0.0);
-            s->h_mean[k] = static_cast<float>(mu);
-            s->h_std[k] = static_cast<float>(std::sqrt(var) > 1e-6 ? std::sqrt(var) : 1.0);
+            const float var = h[0].n > 0.f ? std::max(h[0].m2[k] / h[0].n, 0.f) : 0.f;
+            const float sd = std::sqrt(var);
+            s->h_mean[k] = h[0].mean[k];
+            s->h_std[k] = sd > 1e-6f ? sd : 1.f;
         }
         upload_norm(*s, st);
     });
@@ -604,11 +748,12 @@ ckf_status ckf_scorer_gradient(const ckf_scorer* s, const float* d_features,
         const uint32_t tiles = (n + 63u) / 64u;
         float* part = nullptr;
         hip_check(hipMalloc(&part, (size_t(tiles) * tamd::kMlpParams + tiles) * 4), "scorer grad");
-        uint64_t rng = 0;
         ckf_scorer& S = const_cast<ckf_scorer&>(*s);
         try {
-            grad_batch(S, d_features, d_labels, make_perm(n, rng, true), 0u, n, pos_weight, part,
-                       part + size_t(tiles) * tamd::kMlpParams, st);
+            uint64_t rng = 0;
+            set_state(S, make_perm(n, rng, true), 0u, st);
+            launch_step(S, d_features, d_labels, n, n, pos_weight, 0, tamd::adam_hyper{},
+                        part, part + size_t(tiles) * tamd::kMlpParams, S.loss, S.grad, st);
             if (grad) hip_check(hipMemcpyAsync(grad, s->grad, tamd::kMlpParams * 4, hipMemcpyDeviceToHost, st), "grad");
             if (loss) hip_check(hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToHost, st), "loss");
             hip_check(hipStreamSynchronize(st), "scorer grad sync");
@@ -629,43 +774,58 @@ ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_
         hipStream_t st = static_cast<hipStream_t>(stream);
         const uint32_t batch = std::min(cfg->batch, ((n + 63u) / 64u) * 64u);
         const uint32_t tiles = (batch + 63u) / 64u;
+        const uint32_t n_batches = (n + batch - 1) / batch;
+        // steps per captured graph: the step kernels read their batch from
+        // the device state, so one capture serves every replay
+        constexpr uint32_t kGraphSteps = 32;
         float* part = nullptr;
-        hip_check(hipMalloc(&part, (size_t(tiles) * tamd::kMlpParams + tiles) * 4), "scorer train");
+        hip_check(hipMalloc(&part, (size_t(tiles) * tamd::kMlpParams + tiles + n_batches) * 4),
+                  "scorer train");
+        float* loss_part = part + size_t(tiles) * tamd::kMlpParams;
+        float* losses = loss_part + tiles;
         hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipGraphExec_t graph = nullptr;
         try {
             hip_check(hipEventCreate(&e0), "event");
             hip_check(hipEventCreate(&e1), "event");
+            const tamd::adam_hyper h{cfg->learning_rate, cfg->beta1, cfg->beta2, cfg->eps,
+                                     cfg->weight_decay};
+            if (n_batches >= kGraphSteps) {
+                hipGraph_t g = nullptr;
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
+                for (uint32_t k = 0; k < kGraphSteps; ++k) {
+                    launch_step(*s, d_features, d_labels, n, batch, cfg->pos_weight, 1, h, part,
+                                loss_part, losses, nullptr, st);
+                }
+                hip_check(hipStreamEndCapture(st, &g), "capture");
+                const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                hip_check(e, "graph instantiate");
+            }
             uint64_t rng = cfg->shuffle_seed;
             uint32_t steps = 0;
             float first = 0.f, last = 0.f;
-            std::vector<float> losses;
+            std::vector<float> h_losses(n_batches, 0.f);
             hip_check(hipEventRecord(e0, st), "event");
             for (uint32_t ep = 0; ep < cfg->epochs; ++ep) {
-                const tamd::perm_map perm = make_perm(n, rng, false);
-                const uint32_t n_batches = (n + batch - 1) / batch;
-                losses.assign(n_batches, 0.f);
-                for (uint32_t bi = 0; bi < n_batches; ++bi) {
-                    const uint32_t row0 = bi * batch;
-                    const uint32_t rows = std::min(batch, n - row0);
-                    grad_batch(*s, d_features, d_labels, perm, row0, rows, cfg->pos_weight, part,
-                               part + size_t(tiles) * tamd::kMlpParams, st);
-                    s->adam_t += 1;
-                    tamd::adam_hyper h{cfg->learning_rate, cfg->beta1, cfg->beta2, cfg->eps,
-                                       cfg->weight_decay,
-                                       1.f - std::pow(cfg->beta1, static_cast<float>(s->adam_t)),
-                                       1.f - std::pow(cfg->beta2, static_cast<float>(s->adam_t))};
-                    hipLaunchKernelGGL(tamd::k_mlp_adam, dim3((tamd::kMlpParams + 255) / 256), dim3(256), 0, st,
-                                       s->params, s->grad, s->m, s->v, h);
-                    hip_check(hipGetLastError(), "scorer adam");
-                    repack(*s, st);
-                    hip_check(hipMemcpyAsync(&losses[bi], s->loss, 4, hipMemcpyDeviceToHost, st), "loss");
-                    ++steps;
+                set_state(*s, make_perm(n, rng, false), 0u, st);
+                uint32_t done = 0;
+                for (; graph != nullptr && done + kGraphSteps <= n_batches; done += kGraphSteps) {
+                    hip_check(hipGraphLaunch(graph, st), "graph launch");
                 }
+                for (; done < n_batches; ++done) {
+                    launch_step(*s, d_features, d_labels, n, batch, cfg->pos_weight, 1, h, part,
+                                loss_part, losses, nullptr, st);
+                }
+                s->adam_t += n_batches;
+                steps += n_batches;
+                hip_check(hipMemcpyAsync(h_losses.data(), losses, 4u * n_batches,
+                                         hipMemcpyDeviceToHost, st), "losses");
                 hip_check(hipStreamSynchronize(st), "scorer epoch");
                 double sum = 0.0, wsum = 0.0;
                 for (uint32_t bi = 0; bi < n_batches; ++bi) {
                     const uint32_t rows = std::min(batch, n - bi * batch);
-                    sum += double(losses[bi]) * rows;
+                    sum += double(h_losses[bi]) * rows;
                     wsum += rows;
                 }
                 const float mean_loss = static_cast<float>(sum / wsum);
@@ -686,11 +846,13 @@ ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_
                 stats->tflops = ms > 0.f ? flop / (ms * 1e-3) / 1e12 : 0.0;
             }
         } catch (...) {
+            if (graph) (void)hipGraphExecDestroy(graph);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             (void)hipFree(part);
             throw;
         }
+        if (graph) (void)hipGraphExecDestroy(graph);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipFree(part);

@@ -220,22 +220,14 @@ __device__ inline void mlp_fill_vecs(const mlp_dev& W, mlp_vecs& V, uint32_t tid
     if (tid == 0u) V.b3 = W.b3[0];
 }
 
-// Logit of sample l % 16 (returned in all four lanes of that column).  x:
-// the lane's four raw features; valid = false gives a zero row.
-__device__ inline float mlp_logit16(const mlp_frags& F, const mlp_vecs& V, const float (&x)[4],
-                                    bool valid) {
+// Logit of sample l % 16 (returned in all four lanes of that column) from
+// its normalised BF16 features: lane l holds features 4 (l / 16) .. + 3 of
+// sample l % 16 (the B fragment of layer 1).
+__device__ inline float mlp_logit16_bf(const mlp_frags& F, const mlp_vecs& V, bf16x4 xb) {
     const uint32_t g4 = 4u * (__lane_id() / 16u);
     // read the vectors from LDS at each use instead of holding 56 registers
     // across the caller's loop
     __asm__ volatile("" ::: "memory");
-    const f32x4 mu = *reinterpret_cast<const f32x4*>(&V.mean[g4]);
-    const f32x4 is = *reinterpret_cast<const f32x4*>(&V.inv_std[g4]);
-    bf16x4 xb;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const bool real = valid && g4 + static_cast<uint32_t>(e) < static_cast<uint32_t>(kMlpIn);
-        xb[e] = f2bf(real ? (x[e] - mu[e]) * is[e] : 0.f);
-    }
     // layer 1: H1^T tile nt = GELU(W1^T X^T + b1), straight into B fragments
     bf16x4 h[4];
 #pragma unroll
@@ -261,6 +253,28 @@ __device__ inline float mlp_logit16(const mlp_frags& F, const mlp_vecs& V, const
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
     return part + V.b3;
+}
+
+// Normalised BF16 value of raw feature f (index k); padded inputs are zero.
+__device__ inline short mlp_feature_bf(const mlp_vecs& V, float f, uint32_t k) {
+    return f2bf(k < static_cast<uint32_t>(kMlpIn) ? (f - V.mean[k]) * V.inv_std[k] : 0.f);
+}
+
+// Logit of sample l % 16 from raw features.  x: the lane's four raw
+// features (4 (l / 16) .. + 3); valid = false gives a zero row.
+__device__ inline float mlp_logit16(const mlp_frags& F, const mlp_vecs& V, const float (&x)[4],
+                                    bool valid) {
+    const uint32_t g4 = 4u * (__lane_id() / 16u);
+    __asm__ volatile("" ::: "memory");
+    const f32x4 mu = *reinterpret_cast<const f32x4*>(&V.mean[g4]);
+    const f32x4 is = *reinterpret_cast<const f32x4*>(&V.inv_std[g4]);
+    bf16x4 xb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool real = valid && g4 + static_cast<uint32_t>(e) < static_cast<uint32_t>(kMlpIn);
+        xb[e] = f2bf(real ? (x[e] - mu[e]) * is[e] : 0.f);
+    }
+    return mlp_logit16_bf(F, V, xb);
 }
 
 // Scores of a 64-row tile: the logit of lane l's row returned in lane l.

@@ -15,7 +15,7 @@ from .ckf import LIB, CKFError, _check
 
 N_FEATURES = 12
 FEATURE_NAMES = ("pull_loc0", "pull_loc1", "chi2", "layer", "qop", "theta", "holes", "depth",
-                 "chi2_per_ndf", "consecutive_holes", "rank", "log_candidates")
+                 "chi2_per_ndf", "standing", "rank", "log_candidates")
 
 
 class TrainConfig(C.Structure):
