@@ -15,9 +15,9 @@
  *   0 pull in loc0   1 pull in loc1 (0 for 1-D)   2 predicted chi2
  *   3 layer index / layer count   4 q/p (1/GeV)   5 theta
  *   6 holes so far   7 branch depth (CKF step)   8 chi2 sum / ndf sum so far
- *   9 standing: the branch's running chi2 / ndf (feature 8) minus the
- *     lowest running chi2 / ndf over the links its seed made at the
- *     previous step (0 for the seed's leading branch, 0 at step 0)
+ *   9 standing: the branch's running chi2 / ndf with this candidate minus
+ *     the lowest such value over the rank-0 candidates of all branches of
+ *     the seed at this step (0 for the seed's best branch)
  *   10 rank of the candidate among the branch's candidates by chi2
  *   11 log(1 + candidates passing the chi2 cut)
  *

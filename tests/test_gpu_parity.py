@@ -298,7 +298,7 @@ def test_window_search_matches_full_scan():
     assert np.array_equal(a["offsets"], b["offsets"])
     assert np.array_equal(a["constituents"], perm[np.asarray(b["constituents"])])
     assert np.array_equal(a["chi2"], b["chi2"])
-    match, _, _, _ = _chain_compare(cfg, det, fld, rev, "C4 reversed surfaces")
+    match, _, _, _, _ = _chain_compare(cfg, det, fld, rev, "C4 reversed surfaces")
     assert match >= 0.995
 
 
@@ -319,13 +319,17 @@ def test_measurement_on_unknown_surface_rejected(where):
     det, fld, gun = S.workload("C1")
     ev = S.generate_event(det, fld, gun)
     sl = ev.surface_link.copy()
+    ids = det.identifiers
     if where == "between":
-        k = len(sl) // 2
-        sl[k] = sl[k] - 1              # identifiers are spaced: no surface has this one
-        assert sl[k] not in set(det.identifiers.tolist())
+        # the first measurement of a later layer moves to the identifier just
+        # past its layer's last surface: still sorted, named by no surface
+        k = int(np.nonzero((sl[1:] >> np.uint64(24)) != (sl[:-1] >> np.uint64(24)))[0][0]) + 1
+        layer_ids = ids[(ids >> np.uint64(24)) == (sl[k - 1] >> np.uint64(24))]
+        sl[k] = layer_ids.max() + np.uint64(1)
+        assert sl[k] not in set(ids.tolist())
         assert np.all(np.diff(sl.astype(np.int64)) >= 0)
     else:
-        sl[-1] = det.identifiers[-1] + 1
+        sl[-1] = ids[-1] + np.uint64(1)
     ev.surface_link = np.ascontiguousarray(sl)
     alg = ckf.combinatorial_kalman_filter_algorithm(S.finding_config_for("C1"))
     with pytest.raises(ValueError):

@@ -7,7 +7,8 @@
   * the local_key bit rule (2 = loc0, 4 = loc1, 6 = both) on hand-written
     rows; malformed files are rejected with the line number;
   * write -> read round trip of a generated event is exact, and the CKF
-    finds the same tracks from the re-read measurements.
+    finds the same tracks from the re-read measurements (the oracle on the
+    CPU; the HIP CKF against the oracle on the GPU).
 """
 import os
 
@@ -98,3 +99,47 @@ def test_round_trip_and_finding(tmp_path):
     b = oracle.run(cfg, det, fld, ev2)
     assert np.array_equal(a.offsets, b.offsets)
     assert np.array_equal(a.constituents, b.constituents)
+
+
+@pytest.mark.gpu
+def test_csv_measurements_through_the_hip_ckf(tmp_path, gpu_available):
+    """A C2 event written to CSV with its rows shuffled, read back with the
+    traccc ordering restored, runs through the HIP CKF: the same tracks as the
+    FP64 oracle finds on the re-read event, and as the HIP CKF finds on the
+    generator's own event (constituents mapped through the reader's
+    old -> new row map)."""
+    import oracle
+    from traccc_amd import ckf
+    det, fld, gun = S.workload("C2", n_particles=2000, seed=15)
+    ev = S.generate_event(det, fld, gun)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(ev.n_meas)
+    lp, lv = ev.local_position.reshape(-1, 2), ev.local_variance.reshape(-1, 2)
+    sh = IO.Measurements(lp[perm].copy(), lv[perm].copy(), ev.dimensions[perm].copy(),
+                         ev.time[perm].copy(), ev.identifier[perm].copy(),
+                         ev.surface_link[perm].copy(), ev.subspace.reshape(-1, 2)[perm].copy())
+    p = tmp_path / "c2.csv"
+    IO.write_measurements_csv(str(p), sh)
+    m, idx = IO.read_measurements_csv(str(p), do_sort=True)
+    assert m.n_meas == ev.n_meas
+    ev2 = S.Event(**{**ev.__dict__,
+                     "local_position": np.ascontiguousarray(m.local_position, np.float32),
+                     "local_variance": np.ascontiguousarray(m.local_variance, np.float32),
+                     "dimensions": np.ascontiguousarray(m.dimensions, np.uint32),
+                     "time": np.ascontiguousarray(m.time, np.float32),
+                     "identifier": np.ascontiguousarray(m.identifier, np.uint32),
+                     "surface_link": np.ascontiguousarray(m.surface_link, np.uint64),
+                     "subspace": np.ascontiguousarray(m.subspace, np.uint8)})
+    cfg = S.finding_config_for("C2")
+    g = ckf.combinatorial_kalman_filter_algorithm(cfg)(det, fld, ev2)
+    o = oracle.run(cfg, det, fld, ev2, 64)
+    assert len(g) > 1500
+    assert np.array_equal(g.offsets, o.offsets)
+    assert np.array_equal(g.constituents, o.constituents)
+    # the generator's event is already in traccc order: sorted row k of the
+    # re-read event is original row perm[old] where idx[old] = k
+    orig_of_sorted = np.empty(ev.n_meas, np.int64)
+    orig_of_sorted[idx] = perm
+    g0 = ckf.combinatorial_kalman_filter_algorithm(cfg)(det, fld, ev)
+    assert np.array_equal(g0.offsets, g.offsets)
+    assert np.array_equal(g0.constituents, orig_of_sorted[g.constituents])

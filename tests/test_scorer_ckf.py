@@ -64,7 +64,7 @@ def test_branch_log_closed_forms():
     assert np.all((f[:, 2] >= 0) & (f[:, 2] <= cfg.chi2_max))          # chi2
     assert np.all((f[:, 7] >= 0) & (f[:, 7] < st.n_steps))             # depth
     assert np.all(f[:, 10] == 0)                                        # rank (one branch)
-    assert np.all(f[:, 9] == 0)         # standing: a seed's only branch leads it
+    assert np.all(f[:, 9] == 0)         # standing: a seed's only branch is its best
     assert np.all(f[:, 11] >= np.log(2.0) - 1e-6)                       # >= 1 candidate
     assert np.all(f[:, 6] <= cfg.max_num_skipping_per_cand)            # holes
     # three branches per surface: a branch never stands below its seed's best

@@ -18,8 +18,8 @@ namespace tamd {
 
 constexpr int kStateCols = 28;  // 6 + 21 + 1
 constexpr int kTickets = 9;     // work queues of the control block
-constexpr int kLookbacks = 5;   // compactions with their own look-back words:
-                                // find emit, dedup, propagate emit, build, list emit
+constexpr int kLookbacks = 4;   // compactions with their own look-back words:
+                                // find emit, dedup, propagate emit, build
 constexpr uint32_t kInvalid = 0xffffffffu;
 
 // state_ref encoding: where a link's (filtered) state lives
@@ -103,9 +103,9 @@ struct step_ctl {
     uint64_t n_scored;        // scorer rows of the finished steps
     line_u32 ticket[kTickets];  // work-queue tickets: [0] progressive filter, [1] votes /
                                 // progressive filter, [2] free, [3] build, [4] find emit,
-                                // [5] propagate emit, [6] dedup emit, [7] free, [8] list emit
+                                // [5] propagate emit, [6] dedup emit, [7] free, [8] free
     line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
-    line_u32 n_rows;      // scorer rows scored this step (written by k_find_scored)
+    line_u32 n_rows;      // scorer rows scored this step (added up by k_score_update)
     line_u64 n_props;     // propagations so far
     line_u32 xcd_ticket[4][8];  // per-XCD work queues (one counter per XCD): find, scorer
                                 // list, propagate, dedup mark
@@ -195,14 +195,18 @@ struct ckf_params {
     float* log_feat;              // [link_cap][12] or null
     uint8_t* log_label;           // [link_cap] or null
     uint64_t* seed_best;          // [n_seeds] best kept tip of each seed (logging)
-    // scorer-pruned finding (k_find_scored): per input the list_len best
-    // candidates in (chi2, index) order are scored before the updates
-    uint32_t scored;              // 1: the scorer-pruned find runs
+    // scorer-pruned finding: per input the list_len best candidates in
+    // (chi2, index) order with their features, scored before the updates
+    uint32_t scored;              // 1: k_find takes its candidates from the list
     uint32_t list_len;
+    uint32_t* list_m;             // [in_cap][list_len] measurement
+    float* list_c;                // [in_cap][list_len] chi2
     uint32_t* find_count;         // [in_cap] branches each input keeps this step
     uint8_t* prop_flag;           // [state_cap] propagation outcome: bit 0 next input, bits 1-2 tips
-    uint64_t* seed_run_min;       // [2][seeds] scorer feature 9: lowest running chi2 / ndf of
-                                  // the seed's links, by step parity
+    uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
+    uint32_t* list_seed;          // [in_cap] seed of each listed input
+    uint32_t* list_n;             // [in_cap] entries
+    float* list_feat;             // [in_cap][list_len][12] features of each listed candidate
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
     // candidate-verdict log (diagnostics): one key per scored candidate,

@@ -403,201 +403,35 @@ __device__ inline uint32_t emit_chunks(uint32_t n) {
     return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
 }
 
-// Diagnostics: one scored candidate's verdict as a key a CPU checker replays.
-// The key names the candidate by its branch's history, so it does not depend
-// on where the branch sits in the step's input order: splitmix64 chained over
-// the seed, the step, the measurement of every earlier link of the branch
-// (latest first, holes as all-ones) and the candidate's measurement; low 63
-// bits, bit 63 = kept.
-__device__ inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
-    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-
-__device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t seed,
-                                   uint32_t prev_g, uint32_t meas, bool keep) {
-    const uint32_t at = atomicAdd(P.verdict_n, 1u);
-    if (at >= P.verdict_cap) {
-        return;
-    }
-    uint64_t h = verdict_mix(0u, seed);
-    h = verdict_mix(h, step);
-    uint32_t g = prev_g;
-    for (uint32_t k = 0; k < step && g < P.link_cap; ++k) {
-        const dlink L = P.links[g];
-        h = verdict_mix(h, L.meas);
-        if (link_step(L) == 0u) {
-            break;
-        }
-        g = L.prev;
-    }
-    h = verdict_mix(h, meas);
-    P.verdict_log[at] = (h & 0x7fffffffffffffffull) | (keep ? (1ull << 63) : 0ull);
-}
-
-// Scorer-pruned find, one kernel per step.  A team of kListTeam lanes takes
-// one input: the candidate stream gives its list_len best candidates in
-// (chi2, index) order (candidate q held by team lane q % kListTeam in slot
-// q / kListTeam); their features, normalised and rounded to BF16, are packed
-// densely into the wave's LDS rows (teams in input order, so a wave's 16
-// inputs fill ceil(rows / 16) sub-tiles); the MLP scores the rows 16 at a
-// time on the matrix cores, and each sub-tile's verdicts go into a
-// wave-uniform bit mask by one ballot; then each team updates its kept
-// candidates in list order -- one round of up to kListTeam candidates (one
-// per lane) at a time, in parallel -- until N succeed, and stages them as the
-// plain find does (k_find_emit builds the links).  Each input also offers the
-// lowest running chi2 / ndf of the links it makes to its seed's entry for the
-// next step (scorer feature 9).
-constexpr int kRowShorts = 16;  // a feature row: 12 BF16 features, 4 zero
-constexpr uint32_t kScoredPerWave = kWave / kListTeam;
-constexpr int kMaskWords = (kScoredPerWave * kListTeam * kListSlots + 63) / 64;
-
-// Dynamic LDS of the scored find: every wave's feature rows.
-inline size_t scored_find_lds(uint32_t list_len, uint32_t block) {
-    return static_cast<size_t>(block / kWave) * kScoredPerWave * list_len * kRowShorts *
-           sizeof(short);
-}
-
-// The scored find's update of one input (its team, after the MLP): the kept
-// candidates in list order, one round of kListTeam at a time (one per lane,
-// in parallel), until N updates succeed; staging as the plain find, and the
-// offer of the lowest running chi2 / ndf of the input's links for the next
-// step's feature 9.
-__device__ __forceinline__ void scored_update(const ckf_params& P, const find_input& in,
-                                              uint32_t step, uint32_t i, uint32_t epoch,
-                                              uint32_t N, uint32_t k, uint32_t base,
-                                              const uint64_t (&keep_bits)[kMaskWords],
-                                              const uint32_t (&sm)[kListSlots],
-                                              const float (&sc)[kListSlots], uint64_t* run_next) {
-    step_ctl* ctl = P.ctl;
-    const uint32_t l = lane();
-    const uint32_t team = l / kListTeam;
-    const uint32_t tl = l % kListTeam;
-    const bool leader = (tl == 0u);
-    auto kept_bit = [&](uint32_t r) {
-        uint64_t w = keep_bits[0];
-#pragma unroll
-        for (int x = 1; x < kMaskWords; ++x) {
-            if (static_cast<uint32_t>(x) == r / 64u) w = keep_bits[x];
-        }
-        return ((w >> (r % 64u)) & 1ull) != 0ull;
-    };
-    if (P.verdict_log != nullptr) {
-#pragma unroll
-        for (int j = 0; j < kListSlots; ++j) {
-            const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
-            if (q < k) {
-                log_verdict(P, step, in.seed, in.prev_g, sm[j], kept_bit(base + q));
-            }
-        }
-    }
-    uint32_t count = 0u;
-    float run_links = __int_as_float(0x7f800000);
-#pragma unroll
-    for (int j = 0; j < kListSlots; ++j) {
-        if (static_cast<uint32_t>(j * kListTeam) >= k || count >= N) {
-            break;  // team-uniform
-        }
-        const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
-        const bool kept = q < k && kept_bit(base + q);
-        bool ok = false;
-        float fx[6], fc[21];
-        uint32_t dims = 0u;
-        if (kept) {
-            const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
-            dims = md.dims;
-#pragma unroll
-            for (int e = 0; e < 6; ++e) fx[e] = in.x[e];
-#pragma unroll
-            for (int e = 0; e < 21; ++e) fc[e] = in.c[e];
-            int res = CKF_FIT_SUCCESS;
-            if (in.rule0) {
-                // step 0 on a surface without material: keep the seed,
-                // take the measurement variances
-                fc[sym_idx(0, 0)] = md.v0;
-                fc[sym_idx(1, 1)] = md.v1;
-            } else {
-                res = gain_update(md, fx, fc);
-            }
-            ok = res == CKF_FIT_SUCCESS;
-        }
-        // this round's successes in list order (the team's lanes)
-        const uint32_t tm =
-            static_cast<uint32_t>(wave_ballot(ok) >> (team * kListTeam)) & ((1u << kListTeam) - 1u);
-        const uint32_t before = __builtin_popcount(tm & ((1u << tl) - 1u));
-        if (ok && count + before < N) {
-            const uint32_t slot = i * N + count + before;
-            if (slot < P.flt_cap) {
-                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
-                P.stage_slot[slot] = make_stage_cand(sm[j], sc[j], dims);
-            } else {
-                set_overflow(ctl, 4u);
-            }
-            run_links = fminf(run_links, branch_run(in.chi2_prev + sc[j], in.ndf_prev + dims));
-        }
-        count += __builtin_popcount(tm);
-        count = count < N ? count : N;
-    }
-    if (count == 0u) {
-        run_links = branch_run(in.chi2_prev, in.ndf_prev);  // the input's hole link
-    }
-#pragma unroll
-    for (int d = kListTeam / 2; d >= 1; d >>= 1) {
-        run_links = fminf(run_links, __shfl_xor(run_links, d, kListTeam));
-    }
-    if (leader) {
-        P.stage_in[i] = make_stage_head(in, step, i);
-        if (count == 0u) {
-            // no candidate kept: the input continues as one hole link
-            const uint32_t slot = i * N;
-            if (slot < P.flt_cap) {
-                P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
-            } else {
-                set_overflow(ctl, 4u);
-            }
-        }
-        P.find_count[i] = count;
-        standing_offer(run_next, in.seed, epoch + 1u, run_links);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_find_scored(ckf_params P) {
-    extern __shared__ __attribute__((aligned(16))) short s_rows[];
-    __shared__ mlp_vecs V;
+// Scorer-pruned finding, first kernel: per input the list_len best candidates
+// in (chi2, index) order and their features, at the input's fixed slots
+// input * list_len + q; k_list_emit then packs the rows for the MLP.
+__global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
-        return;  // the whole grid returns: no barrier is left waiting
+        return;
     }
-    mlp_fill_vecs(P.scorer, V, threadIdx.x, blockDim.x);
-    __syncthreads();
-    mlp_frags F;
-    mlp_load_frags(P.scorer, F);
     const uint32_t step = ctl->step;
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t prev_base = ctl->prev_link_base;
-    const uint32_t epoch = ctl->epoch;
-    const uint32_t N = P.n_branch_surface;
+    const uint32_t team = lane() / kListTeam;
+    const uint32_t tl = lane() % kListTeam;
     const uint32_t Lc = P.list_len;
-    const uint32_t l = lane();
-    const uint32_t team = l / kListTeam;
-    const uint32_t tl = l % kListTeam;
-    const bool leader = (tl == 0u);
-    short* rows = s_rows + static_cast<size_t>(threadIdx.x / kWave) * kScoredPerWave * Lc * kRowShorts;
-    const uint64_t* run_now = standing_read(P, step);
-    uint64_t* run_next = standing_write(P, step);
-    const uint32_t n_parts = (n_in + kScoredPerWave - 1u) / kScoredPerWave;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t n_parts = (n_in + (kWave / kListTeam) - 1) / (kWave / kListTeam);
+    // dynamic partitions (dense surfaces make some inputs far heavier)
     xcd_ticket_run queue(&ctl->xcd_ticket[1][0].v, n_parts);
     for (;;) {
         const uint32_t p = queue.next();
         if (p >= n_parts) {
             break;
         }
-        const uint32_t i = p * kScoredPerWave + team;
+        const uint32_t i0 = p * (kWave / kListTeam);
+        const uint32_t i = i0 + team;
         const bool active = i < n_in;
         find_input in;
         uint32_t n_pass = 0u, k = 0u;
+        // candidate q of the list is held by team lane q % kListTeam in slot q / kListTeam
         uint32_t sm[kListSlots];
         float sc[kListSlots];
         if (active) {
@@ -621,70 +455,243 @@ __global__ __launch_bounds__(kBlock) void k_find_scored(ckf_params P) {
                 }
             }
         }
-        // the wave's rows: the teams' lists packed in input order
-        const uint32_t kk = (active && leader) ? k : 0u;
-        const uint32_t incl = wave_inclusive_scan(kk);
-        const uint32_t total = wave_broadcast(incl, kWave - 1);
-        const uint32_t base = __shfl(incl - kk, static_cast<int>(team * kListTeam), kWave);
         if (active) {
-            const float run_min = standing_min(run_now, in.seed, epoch);
+            if (tl == 0u) {
+                P.list_n[i] = k;
+                P.list_seed[i] = in.seed;
+            }
+            // the team's lanes write the list and the features of their
+            // candidates, at the input's fixed slots i * list_len + q
 #pragma unroll
             for (int j = 0; j < kListSlots; ++j) {
                 const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
                 if (q < k) {
+                    const size_t r = static_cast<size_t>(i) * Lc + q;
+                    P.list_m[r] = sm[j];
+                    P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
-                    float f[kFeatures];
                     candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass, run_min, f);
-                    short* row = rows + static_cast<size_t>(base + q) * kRowShorts;
-                    bf16x4 v[4];
-#pragma unroll
-                    for (int e = 0; e < kRowShorts; ++e) {
-                        v[e / 4][e % 4] = e < kFeatures ? mlp_feature_bf(V, f[e], e) : short(0);
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
+                                       P.list_feat + r * kFeatures);
+                    if (q == 0u) {
+                        standing_offer(P.seed_run_min, in.seed, epoch,
+                                       branch_run(in.chi2_prev, in.ndf_prev, sc[j], md.dims));
                     }
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) reinterpret_cast<bf16x4*>(row)[e] = v[e];
                 }
             }
-        }
-        wave_lds_sync();
-        // the MLP over the rows, 16 per sub-tile; verdict bits by row
-        uint64_t keep_bits[kMaskWords];
-#pragma unroll
-        for (int w = 0; w < kMaskWords; ++w) keep_bits[w] = 0u;
-        const uint32_t n_sub = (total + 15u) / 16u;
-        for (uint32_t t = 0; t < n_sub; ++t) {
-            const uint32_t r = t * 16u + (l % 16u);
-            bf16x4 xb = {0, 0, 0, 0};
-            if (r < total) {
-                xb = load4(rows + static_cast<size_t>(r) * kRowShorts + 4u * (l / 16u));
-            }
-            const float s = mlp_logit16_bf(F, V, xb);
-            const bool keep = l < 16u && r < total && s >= P.score_threshold;
-            const uint64_t bits = (wave_ballot(keep) & 0xffffull) << ((t % 4u) * 16u);
-#pragma unroll
-            for (int w = 0; w < kMaskWords; ++w) {
-                if (static_cast<uint32_t>(w) == t / 4u) keep_bits[w] |= bits;
-            }
-        }
-        wave_lds_sync();  // the next partition's rows overwrite these
-        if (l == 0u && total > 0u) {
-            atomicAdd(&ctl->n_rows.v, total);
-        }
-        if (active) {
-            scored_update(P, in, step, i, epoch, N, k, base, keep_bits, sm, sc, run_next);
         }
     }
 }
 
+// Diagnostics: one scored candidate's verdict as a key a CPU checker replays.
+// The key names the candidate by its branch's history, so it does not depend
+// on where the branch sits in the step's input order: splitmix64 chained over
+// the seed, the step, the measurement of every earlier link of the branch
+// (latest first, holes as all-ones) and the candidate's measurement; low 63
+// bits, bit 63 = kept.
+__device__ inline uint64_t verdict_mix(uint64_t h, uint64_t v) {
+    uint64_t z = (h ^ v) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
 
-// A team of kFindTeam lanes handles one input branch: the lanes split the
-// surface's measurements (the candidate stream) and take candidates in
-// (chi2, index) order.  Logged: the same, with every filtered slot's features
-// kept for the branch log, and each input's lowest new running chi2 / ndf
-// offered to its seed for the next step's feature 9.  The gain update runs
-// redundantly on the team (so it stays uniform) and the team leader writes.
-// A wave holds kWave / kT inputs.  (The scorer-pruned find is k_find_scored.)
+__device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t input,
+                                   uint32_t meas, bool keep) {
+    const uint32_t at = atomicAdd(P.verdict_n, 1u);
+    if (at >= P.verdict_cap) {
+        return;
+    }
+    uint64_t h = verdict_mix(0u, P.list_seed[input]);
+    h = verdict_mix(h, step);
+    if (step > 0u) {
+        uint32_t g = P.ctl->prev_link_base + P.live[input];
+        for (uint32_t k = 0; k < step && g < P.link_cap; ++k) {
+            const dlink L = P.links[g];
+            h = verdict_mix(h, L.meas);
+            if (link_step(L) == 0u) {
+                break;
+            }
+            g = L.prev;
+        }
+    }
+    h = verdict_mix(h, meas);
+    P.verdict_log[at] = (h & 0x7fffffffffffffffull) | (keep ? (1ull << 63) : 0ull);
+}
+
+// Scorer-pruned finding, second kernel, one input per lane (a wave holds 64
+// inputs).  The wave packs its inputs' listed candidates (k_find_list's
+// fixed slots) densely into LDS rows -- the features normalised and rounded
+// to BF16, feature 9 finalised from its seed's minimum -- kRows at a time,
+// scores them 16 rows per matrix-core pass, and keeps one 16-bit verdict mask
+// per sub-tile in LDS; then each lane walks its input's kept entries in list
+// order, runs the gain update until N succeed and stages the links as the
+// plain find does (k_find_emit builds them).
+constexpr int kRowShorts = 16;     // a feature row: 12 BF16 features, 4 zero
+constexpr uint32_t kRows = 128;    // LDS rows of a wave per packing pass
+constexpr int kListMax = kListTeam * kListSlots;
+constexpr uint32_t kSubTiles = (kWave * kListMax) / 16u;  // sub-tiles of a full partition
+static_assert(kListMax <= 32, "kept-entry mask is 32 bits");
+
+// The update of one input after the MLP: its kept entries in list order (a
+// mask read from the sub-tiles' verdicts), the gain update until N succeed,
+// staging as the plain find.
+__device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t i, uint32_t k,
+                                                   uint32_t base, const uint16_t* keep16,
+                                                   uint32_t step, uint32_t prev_base) {
+    step_ctl* ctl = P.ctl;
+    const uint32_t N = P.n_branch_surface;
+    const size_t slot0 = static_cast<size_t>(i) * P.list_len;
+    uint32_t kept = 0u;
+    for (uint32_t q = 0; q < k; ++q) {
+        const uint32_t r = base + q;
+        if ((keep16[r / 16u] >> (r % 16u)) & 1u) {
+            kept |= 1u << q;
+        }
+    }
+    if (P.verdict_log != nullptr) {
+        for (uint32_t q = 0; q < k; ++q) {
+            log_verdict(P, step, i, P.list_m[slot0 + q], ((kept >> q) & 1u) != 0u);
+        }
+    }
+    find_input in;
+    load_input<false>(P, step, i, prev_base, in);
+    P.stage_in[i] = make_stage_head(in, step, i);
+    uint32_t count = 0u;
+    while (count < N && kept != 0u) {
+        const uint32_t li = static_cast<uint32_t>(__builtin_ctz(kept));
+        kept &= kept - 1u;
+        const uint32_t best_m = P.list_m[slot0 + li];
+        const float best_c = P.list_c[slot0 + li];
+        const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
+        float fx[6], fc[21];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) fx[e] = in.x[e];
+#pragma unroll
+        for (int e = 0; e < 21; ++e) fc[e] = in.c[e];
+        int res = CKF_FIT_SUCCESS;
+        if (in.rule0) {
+            // step 0 on a surface without material: keep the seed, take the
+            // measurement variances
+            fc[sym_idx(0, 0)] = md.v0;
+            fc[sym_idx(1, 1)] = md.v1;
+        } else {
+            res = gain_update(md, fx, fc);
+        }
+        if (res == CKF_FIT_SUCCESS) {
+            const uint32_t slot = i * N + count;
+            if (slot < P.flt_cap) {
+                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
+                P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
+            } else {
+                set_overflow(ctl, 4u);
+            }
+            ++count;
+        }
+    }
+    if (count == 0u) {
+        // no candidate kept: the input continues as one hole link
+        const uint32_t slot = i * N;
+        if (slot < P.flt_cap) {
+            P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
+        } else {
+            set_overflow(ctl, 4u);
+        }
+    }
+    P.find_count[i] = count;
+}
+
+// Three waves per SIMD: the compiler's own allocation lands four registers
+// above the three-wave budget, and the hint costs no spill.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
+void k_score_update(ckf_params P) {
+    __shared__ __attribute__((aligned(16))) short s_rows[kBlock / kWave][kRows * kRowShorts];
+    __shared__ uint16_t s_keep[kBlock / kWave][kSubTiles];
+    __shared__ mlp_vecs V;
+    __shared__ mlp_frags_lds F;
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;  // the whole grid returns: no barrier is left waiting
+    }
+    mlp_fill_vecs(P.scorer, V, threadIdx.x, blockDim.x);
+    mlp_fill_frags(P.scorer, F, threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint32_t step = ctl->step;
+    const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
+    const uint32_t prev_base = ctl->prev_link_base;
+    const uint32_t epoch = ctl->epoch;
+    const uint32_t Lc = P.list_len;
+    const uint32_t l = lane();
+    const uint32_t wave = threadIdx.x / kWave;
+    short* rows = s_rows[wave];
+    uint16_t* keep16 = s_keep[wave];
+    const uint32_t n_parts = (n_in + kWave - 1u) / kWave;
+    xcd_ticket_run queue(&ctl->xcd_ticket[0][0].v, n_parts);
+    for (;;) {
+        const uint32_t p = queue.next();
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t i = p * kWave + l;
+        const bool valid = i < n_in;
+        const uint32_t k = valid ? P.list_n[i] : 0u;
+        const uint32_t seed = valid ? P.list_seed[i] : 0u;
+        const uint32_t incl = wave_inclusive_scan(k);
+        const uint32_t total = wave_broadcast(incl, kWave - 1);
+        const uint32_t base = incl - k;
+        const size_t slot0 = static_cast<size_t>(i) * Lc;
+        // rows [r0, r0 + kRows) per pass: pack, then score 16 at a time
+        for (uint32_t r0 = 0u; r0 < total; r0 += kRows) {
+            for (uint32_t q = 0; q < k; ++q) {
+                const uint32_t r = base + q;
+                if (r < r0 || r >= r0 + kRows) {
+                    continue;
+                }
+                const float* f = P.list_feat + (slot0 + q) * kFeatures;
+                bf16x4 v[4];
+#pragma unroll
+                for (int e = 0; e < kRowShorts; ++e) {
+                    float x = e < kFeatures ? f[e] : 0.f;
+                    if (e == kFeatStanding) {
+                        x = standing_of(P.seed_run_min, seed, epoch, x);
+                    }
+                    v[e / 4][e % 4] = e < kFeatures ? mlp_feature_bf(V, x, e) : short(0);
+                }
+                bf16x4* row = reinterpret_cast<bf16x4*>(rows + (r - r0) * kRowShorts);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) row[e] = v[e];
+            }
+            wave_lds_sync();
+            const uint32_t n_rows = total - r0 < kRows ? total - r0 : kRows;
+            for (uint32_t t = 0; t < (n_rows + 15u) / 16u; ++t) {
+                const uint32_t r = t * 16u + (l % 16u);
+                bf16x4 xb = {0, 0, 0, 0};
+                if (r < n_rows) {
+                    xb = load4(rows + r * kRowShorts + 4u * (l / 16u));
+                }
+                const float s = mlp_logit16_lds(F, V, xb);
+                const uint64_t bits = wave_ballot(l < 16u && r < n_rows && s >= P.score_threshold);
+                if (l == 0u) {
+                    keep16[r0 / 16u + t] = static_cast<uint16_t>(bits & 0xffffull);
+                }
+            }
+            wave_lds_sync();  // the next pass overwrites the rows
+        }
+        if (l == 0u && total > 0u) {
+            atomicAdd(&ctl->n_rows.v, total);
+        }
+        if (valid) {
+            score_update_input(P, i, k, base, keep16, step, prev_base);
+        }
+    }
+}
+
+// A team of kFindTeam lanes handles one input branch.  Plain: the lanes split
+// the surface's measurements (the candidate stream) and take candidates in
+// (chi2, index) order; logged: the same, with every filtered slot's features
+// kept for the branch log.  The gain update runs redundantly on the team (so
+// it stays uniform) and the team leader writes.  A wave holds kWave / kT
+// inputs.  (The scorer-pruned update is k_score_update.)
 enum find_mode : int { kFindPlain = 0, kFindLogged = 1 };
 
 template <int kMode>
@@ -720,59 +727,60 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             if (leader) {
                 P.stage_in[i] = make_stage_head(in, step, i);
             }
-            // candidates in stream order; the gain update runs on every
-            // lane of the team (uniform), the leader writes
-            cand_stream<6, kFindTeam> cs;
-            stream_init(P, in, tl, cs);
-            const uint32_t n_pass = cs.n_pass;
-            float run_min = -1.f;
-            float run_links = __int_as_float(0x7f800000);
-            if constexpr (logging) {
-                run_min = standing_min(standing_read(P, step), in.seed, ctl->epoch);
-            }
-            while (count < N) {
-                float best_c = 0.f;
-                uint32_t best_m = kInvalid;
-                stream_next(P, in, tl, cs, best_c, best_m);
-                if (best_m == kInvalid) {
-                    break;
-                }
-                const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
-                float fx[6], fc[21];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
-#pragma unroll
-                for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
-                int res = CKF_FIT_SUCCESS;
-                if (in.rule0) {
-                    // step 0 on a surface without material: keep the
-                    // seed, take the measurement variances
-                    fc[sym_idx(0, 0)] = md.v0;
-                    fc[sym_idx(1, 1)] = md.v1;
-                } else {
-                    res = gain_update(md, fx, fc);
-                }
-                if (res == CKF_FIT_SUCCESS) {
-                    const uint32_t slot = i * N + count;
-                    if (leader) {
-                        if (slot < P.flt_cap) {
-                            store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
-                            P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
-                            if constexpr (logging) {
-                                candidate_features(P, md, in.x, in.c, in.sf, step, in.skip,
-                                                   in.consec, in.chi2_prev, in.ndf_prev, best_c,
-                                                   count, n_pass, run_min,
-                                                   P.feat_slot + static_cast<size_t>(slot) * kFeatures);
-                            }
-                        } else {
-                            set_overflow(ctl, 4u);
-                        }
+            uint32_t n_pass = 0u;
+            {
+                // candidates in stream order; the gain update runs on every
+                // lane of the team (uniform), the leader writes
+                cand_stream<6, kFindTeam> cs;
+                stream_init(P, in, tl, cs);
+                n_pass = cs.n_pass;
+                bool rank0 = true;
+                while (count < N) {
+                    float best_c = 0.f;
+                    uint32_t best_m = kInvalid;
+                    stream_next(P, in, tl, cs, best_c, best_m);
+                    if (best_m == kInvalid) {
+                        break;
                     }
+                    const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
                     if constexpr (logging) {
-                        run_links = fminf(run_links,
-                                          branch_run(in.chi2_prev + best_c, in.ndf_prev + md.dims));
+                        if (rank0 && leader) {
+                            standing_offer(P.seed_run_min, in.seed, ctl->epoch,
+                                           branch_run(in.chi2_prev, in.ndf_prev, best_c, md.dims));
+                        }
+                        rank0 = false;
                     }
-                    ++count;
+                    float fx[6], fc[21];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) fx[k] = in.x[k];
+#pragma unroll
+                    for (int k = 0; k < 21; ++k) fc[k] = in.c[k];
+                    int res = CKF_FIT_SUCCESS;
+                    if (in.rule0) {
+                        // step 0 on a surface without material: keep the
+                        // seed, take the measurement variances
+                        fc[sym_idx(0, 0)] = md.v0;
+                        fc[sym_idx(1, 1)] = md.v1;
+                    } else {
+                        res = gain_update(md, fx, fc);
+                    }
+                    if (res == CKF_FIT_SUCCESS) {
+                        const uint32_t slot = i * N + count;
+                        if (leader) {
+                            if (slot < P.flt_cap) {
+                                store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
+                                P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
+                                if constexpr (logging) {
+                                    candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
+                                                       in.chi2_prev, in.ndf_prev, best_c, count, n_pass,
+                                                       P.feat_slot + static_cast<size_t>(slot) * kFeatures);
+                                }
+                            } else {
+                                set_overflow(ctl, 4u);
+                            }
+                        }
+                        ++count;
+                    }
                 }
             }
             if (count == 0u && leader) {
@@ -782,14 +790,6 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                     P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
                 } else {
                     set_overflow(ctl, 4u);
-                }
-            }
-            if constexpr (logging) {
-                if (leader) {
-                    if (count == 0u) {
-                        run_links = branch_run(in.chi2_prev, in.ndf_prev);
-                    }
-                    standing_offer(standing_write(P, step), in.seed, ctl->epoch + 1u, run_links);
                 }
             }
         }
@@ -885,8 +885,11 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
                         if (L.meas != kInvalid) {
 #pragma unroll
                             for (int q = 0; q < kFeatures; ++q) {
-                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] =
-                                    P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
+                                float v = P.feat_slot[static_cast<size_t>(slot) * kFeatures + q];
+                                if (q == kFeatStanding) {
+                                    v = standing_of(P.seed_run_min, seed, epoch, v);
+                                }
+                                P.log_feat[static_cast<size_t>(g) * kFeatures + q] = v;
                             }
                         }
                     }
@@ -1674,8 +1677,8 @@ hipError_t launch_debug_propagate(const ckf_params& P, uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 hipError_t launch_find_work(const ckf_params& P, const launch_shape& s, hipStream_t stream) {
     if (P.scored) {
-        hipLaunchKernelGGL(k_find_scored, dim3(s.find_blocks), dim3(kBlock),
-                           scored_find_lds(P.list_len, kBlock), stream, P);
+        hipLaunchKernelGGL(k_find_list, dim3(s.list_blocks), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_score_update, dim3(s.score_blocks), dim3(kBlock), 0, stream, P);
     } else if (P.feat_slot != nullptr) {
         hipLaunchKernelGGL(k_find<kFindLogged>, dim3(s.find_blocks), dim3(kBlock), 0, stream, P);
     } else {

@@ -13,6 +13,8 @@ struct launch_shape {
     uint32_t prop_blocks;
     uint32_t build_blocks;
     uint32_t prop_team;  // lanes per link in k_propagate: 1, 2, 4, 8 or 16
+    uint32_t list_blocks;   // scorer-pruned finding: candidate lists
+    uint32_t score_blocks;  // and the MLP + updates over them (64 inputs per wave)
     uint32_t emit_blocks;   // the compaction kernels (k_*_emit)
 };
 
@@ -25,8 +27,8 @@ struct launch_shape {
 hipError_t launch_ckf_step(const ckf_params& P, const launch_shape& s, hipStream_t stream,
                            bool with_dedup);
 // the same step, kernel group by kernel group (timed runs)
-// launch_find = the find kernel (plain, logged, or scorer-pruned: candidate
-// lists, MLP and updates in one kernel), then its link compaction
+// launch_find = the find kernel (plain or logged; scorer-pruned: the
+// candidate lists, then the MLP and the updates), then its link compaction
 hipError_t launch_find(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_find_work(const ckf_params& P, const launch_shape& s, hipStream_t stream);
 hipError_t launch_find_emit(const ckf_params& P, const launch_shape& s, hipStream_t stream);

@@ -96,7 +96,8 @@ struct ckf_algorithm {
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
-    dbuf feat_slot, log_feat, log_label, seed_best, find_count, seed_run_min, prop_flag;
+    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_feat,
+        find_count, seed_run_min, list_seed, prop_flag;
     bool logging = false;
     // candidate-verdict log of scored calls (diagnostics)
     dbuf verdict_log, verdict_n;
@@ -131,8 +132,8 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_lo, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &find_count, &seed_run_min, &prop_flag,
-                       &verdict_log, &verdict_n};
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_feat, &find_count,
+                       &seed_run_min, &list_seed, &prop_flag, &verdict_log, &verdict_n};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < tamd::kLookbacks; ++k) {
             lb_lo[k].release();
@@ -214,13 +215,18 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         A.seed_best.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
     }
     if (A.logging || A.scorer != nullptr) {
-        // two halves (step parity) of one entry per seed
-        A.seed_run_min.ensure(16u * size_t(std::max(ev.n_seeds, 1u)));
+        A.seed_run_min.ensure(8u * size_t(std::max(ev.n_seeds, 1u)));
         // every key of a real step compares below all-ones
         hip_check(hipMemsetAsync(A.seed_run_min.p, 0xff, A.seed_run_min.bytes, A.stream), "memset");
     }
     if (A.scorer != nullptr) {
         A.list_len = std::min(A.cfg.max_num_branches_per_surface + 4u, 16u);
+        const size_t rows = size_t(A.in_cap) * A.list_len;
+        A.list_seed.ensure(4u * size_t(A.in_cap));
+        A.list_m.ensure(4u * rows);
+        A.list_c.ensure(4u * rows);
+        A.list_feat.ensure(4u * 12u * rows);
+        A.list_n.ensure(4u * size_t(A.in_cap));
     }
     if (A.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&A.h_ctl), sizeof(tamd::step_ctl),
@@ -309,7 +315,12 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     }
     if (A.scorer != nullptr) {
         P.scored = 1u;
+        P.list_seed = A.list_seed.as<uint32_t>();
         P.list_len = A.list_len;
+        P.list_m = A.list_m.as<uint32_t>();
+        P.list_c = A.list_c.as<float>();
+        P.list_n = A.list_n.as<uint32_t>();
+        P.list_feat = A.list_feat.as<float>();
         P.scorer = tamd::scorer_view(A.scorer);
         P.score_threshold = A.score_threshold;
         if (A.verdict_cap > 0u) {
@@ -332,13 +343,15 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     tamd::launch_shape s;
     // grids are upper bounds for the largest step: a wave beyond a step's
     // partitions leaves at once.  The find is sized for 16 inputs per wave
-    // (the scored find's four-lane teams; the plain find's two-lane teams
-    // hold 32).
+    // (its two-lane teams hold 32, the scored update 64).
     s.find_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.dedup_blocks = blocks(A.state_cap);
     s.prop_team = A.prop_team;
     const uint32_t prop_parts = (A.state_cap + 64u / s.prop_team - 1u) / (64u / s.prop_team);
     s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
+    // scorer-pruned finding: 16 inputs per wave listing, 64 per scoring wave
+    s.list_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
+    s.score_blocks = std::max(1u, std::min((((A.in_cap + 63u) / 64u) + 3u) / 4u, max_blocks));
     s.build_blocks = blocks(A.tip_cap);
     // compaction waves do little per partition: four blocks per CU
     s.emit_blocks = std::min(blocks(A.in_cap), static_cast<uint32_t>(A.n_cu) * 4u);

@@ -771,7 +771,22 @@ ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_
     return guarded([&] {
         if (!s || !d_features || !d_labels || !cfg || n == 0) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "invalid argument");
         if (cfg->batch == 0 || cfg->batch % 64u != 0) throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "batch must be a positive multiple of 64");
+        // the steps are captured into a graph, which the legacy default
+        // stream cannot be: without a caller stream the training runs on a
+        // stream of its own, after everything queued on the default stream
         hipStream_t st = static_cast<hipStream_t>(stream);
+        hipStream_t own = nullptr;
+        if (st == nullptr) {
+            hip_check(hipDeviceSynchronize(), "scorer train");
+            hip_check(hipStreamCreateWithFlags(&own, hipStreamNonBlocking), "scorer stream");
+            st = own;
+        }
+        struct stream_guard {
+            hipStream_t s;
+            ~stream_guard() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } own_guard{own};
         const uint32_t batch = std::min(cfg->batch, ((n + 63u) / 64u) * 64u);
         const uint32_t tiles = (batch + 63u) / 64u;
         const uint32_t n_batches = (n + batch - 1) / batch;

@@ -206,6 +206,26 @@ __device__ inline void mlp_load_frags(const mlp_dev& W, mlp_frags& F) {
     }
 }
 
+// The same fragments kept once per block in LDS (20 per lane position),
+// read at each use so they hold no registers between uses.
+struct alignas(16) mlp_frags_lds {
+    bf16x4 f[20][64];
+};
+
+__device__ inline void mlp_fill_frags(const mlp_dev& W, mlp_frags_lds& L, uint32_t tid,
+                                      uint32_t n) {
+    for (uint32_t i = tid; i < 20u * 64u; i += n) {
+        const uint32_t k = i / 64u, l = i % 64u;
+        const uint32_t r16 = l % 16u, g4 = 4u * (l / 16u);
+        if (k < 4u) {
+            L.f[k][l] = load4(W.w1t + (16u * k + r16) * kMlpInPad + g4);
+        } else {
+            const uint32_t nt = (k - 4u) / 4u, kt = (k - 4u) % 4u;
+            L.f[k][l] = load4(W.w2t + (16u * nt + r16) * kMlpHidden + 16u * kt + g4);
+        }
+    }
+}
+
 // Fill V from the block's threads (tid of n); a block barrier must follow.
 __device__ inline void mlp_fill_vecs(const mlp_dev& W, mlp_vecs& V, uint32_t tid, uint32_t n) {
     for (uint32_t k = tid; k < kMlpHidden; k += n) {
@@ -245,6 +265,35 @@ __device__ inline float mlp_logit16_bf(const mlp_frags& F, const mlp_vecs& V, bf
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) acc = mfma16(F.w2[nt][kt], h[kt], acc);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(&V.b2[16 * nt + g4]);
+        const f32x4 w = *reinterpret_cast<const f32x4*>(&V.w3[16 * nt + g4]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) part += gelu(acc[e] + b[e]) * w[e];
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + V.b3;
+}
+
+// mlp_logit16_bf with the weight fragments read from the block's LDS copy.
+__device__ inline float mlp_logit16_lds(const mlp_frags_lds& L, const mlp_vecs& V, bf16x4 xb) {
+    const uint32_t l = __lane_id();
+    const uint32_t g4 = 4u * (l / 16u);
+    bf16x4 h[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma16(L.f[nt][l], xb, acc);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(&V.b1[16 * nt + g4]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[nt][e] = f2bf(gelu(acc[e] + b[e]));
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(L.f[4 + 4 * nt + kt][l], h[kt], acc);
         const f32x4 b = *reinterpret_cast<const f32x4*>(&V.b2[16 * nt + g4]);
         const f32x4 w = *reinterpret_cast<const f32x4*>(&V.w3[16 * nt + g4]);
 #pragma unroll

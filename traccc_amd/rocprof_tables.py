@@ -54,7 +54,7 @@ def calls_table(db: str) -> str:
         seg = rows[prev:i + 1]
         prev = i + 1
         names = {_short(r[0]) for r in seg}
-        kind = ("pruned" if "k_find_scored" in names else
+        kind = ("pruned" if "k_find_list" in names else
                 "logged" if "k_find<1>" in names else "plain")
         last[kind] = seg
     out = []
