@@ -89,6 +89,21 @@ ckf_status ckf_scorer_train(ckf_scorer* s, const float* d_features, const uint8_
                             uint32_t n, const ckf_scorer_train_config* cfg,
                             ckf_scorer_train_stats* stats, void* stream);
 
+/* ckf_scorer_gradient without any device allocation -- no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, training loops that forbid hipMalloc).  Same gradient and loss,
+ * bit for bit, into caller DEVICE memory: d_grad [ckf_scorer_n_params()]
+ * floats, d_loss [1] float.  The per-tile partial gradients go to a fixed
+ * device array of the library (20.7 MB per GPU, held until the process
+ * ends), so n is limited to CKF_SCORER_NOALLOC_MAX_SAMPLES
+ * (CKF_ERR_INVALID_ARGUMENT above it).  Ordered on the stream, no host
+ * synchronisation.  Every call on one GPU shares that array, and each call
+ * resets the scorer's training state: calls that overlap on one GPU (or
+ * overlap a ckf_scorer_train of the same scorer) race. */
+#define CKF_SCORER_NOALLOC_MAX_SAMPLES 65536u
+ckf_status ckf_scorer_gradient_noalloc(const ckf_scorer* s, const float* d_features,
+                                       const uint8_t* d_labels, uint32_t n, float pos_weight,
+                                       float* d_grad, float* d_loss, void* stream);
 
 /* ---- in the CKF ----------------------------------------------------------- */
 /* Prune, before the Kalman update, every candidate whose logit is below

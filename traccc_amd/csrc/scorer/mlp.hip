@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -463,6 +464,18 @@ __global__ __launch_bounds__(kMomentThreads) void k_mlp_moments(const float* fea
     if (threadIdx.x == 0u) out[blockIdx.x] = s[0];
 }
 
+// Training state set on the stream (no host copy), for callers that record
+// the call into a graph.
+__global__ void k_set_state(train_state* s, perm_map perm, uint32_t batch_idx, uint32_t adam_t) {
+    if (blockIdx.x == 0u && threadIdx.x == 0u) {
+        s->perm = perm;
+        s->batch_idx = batch_idx;
+        s->adam_t = adam_t;
+        s->done_blocks = 0u;
+        s->pad = 0u;
+    }
+}
+
 // One 16x16x16 product through the documented fragment layout (diagnostics).
 __global__ void k_mfma_layout(const float* A, const float* B, float* D) {
     const uint32_t l = threadIdx.x;
@@ -480,6 +493,14 @@ __global__ void k_mfma_layout(const float* A, const float* B, float* D) {
 }
 
 }  // namespace
+
+// Scratch of the allocation-free gradient (ckf_scorer_gradient_noalloc): the
+// per-tile partial gradients and losses of up to kNoAllocMaxSamples samples
+// (1,024 tiles x 5,057 parameters + 1,024 losses, 20.7 MB).  Every GPU the
+// library runs on holds its own copy until the process ends.
+constexpr uint32_t kNoAllocMaxSamples = CKF_SCORER_NOALLOC_MAX_SAMPLES;
+constexpr uint32_t kNoAllocTiles = kNoAllocMaxSamples / 64u;
+static __device__ __align__(256) float scratch[size_t(kNoAllocTiles) * kMlpParams + kNoAllocTiles];
 
 }  // namespace tamd
 
@@ -559,6 +580,25 @@ tamd::perm_map make_perm(uint64_t n, uint64_t& rng, bool identity) {
         a = splitmix(rng) % n;
     } while (a == 0 || gcd64(a, n) != 1);
     return tamd::perm_map{a, splitmix(rng) % n, n};
+}
+
+// Device address of `scratch` on the current device, looked up (with that
+// device current) at the first call on each device and kept: the array's name
+// in host code is a host-side stand-in, not a device address.
+float* noalloc_scratch() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static float* ptrs[kMaxDevices] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw ckf_error(CKF_ERR_UNSUPPORTED, "device index");
+    std::lock_guard<std::mutex> lk(mu);
+    if (ptrs[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(tamd::scratch)), "scratch address");
+        ptrs[dev] = static_cast<float*>(p);
+    }
+    return ptrs[dev];
 }
 
 // Set the device training state (sample order, batch index, Adam steps).
@@ -762,6 +802,30 @@ ckf_status ckf_scorer_gradient(const ckf_scorer* s, const float* d_features,
             throw;
         }
         (void)hipFree(part);
+    });
+}
+
+ckf_status ckf_scorer_gradient_noalloc(const ckf_scorer* s, const float* d_features,
+                                       const uint8_t* d_labels, uint32_t n, float pos_weight,
+                                       float* d_grad, float* d_loss, void* stream) {
+    return guarded([&] {
+        if (!s || !d_features || !d_labels || !d_grad || !d_loss || n == 0) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "invalid argument");
+        }
+        if (n > tamd::kNoAllocMaxSamples) {
+            throw ckf_error(CKF_ERR_INVALID_ARGUMENT,
+                            "n exceeds CKF_SCORER_NOALLOC_MAX_SAMPLES (65536) samples");
+        }
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        float* part = noalloc_scratch();
+        const uint32_t tiles = (n + 63u) / 64u;
+        ckf_scorer& S = const_cast<ckf_scorer&>(*s);
+        uint64_t rng = 0;
+        hipLaunchKernelGGL(tamd::k_set_state, dim3(1), dim3(64), 0, st, S.state,
+                           make_perm(n, rng, true), 0u, static_cast<uint32_t>(S.adam_t));
+        hip_check(hipGetLastError(), "scorer state");
+        launch_step(S, d_features, d_labels, n, n, pos_weight, 0, tamd::adam_hyper{}, part,
+                    part + size_t(tiles) * tamd::kMlpParams, d_loss, d_grad, st);
     });
 }
 
