@@ -311,10 +311,12 @@ def propagate_roofline(ctx, kt, workload_tag):
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_tag}.json")
     if os.path.exists(tpath):
+        # HBM bytes of the kernel over one pruned call of the same workload
+        # (FETCH_SIZE + WRITE_SIZE, copy-calibrated), per working launch
         with open(tpath) as fh:
-            tk = json.load(fh).get("kernels", {})
+            tk = json.load(fh).get("calls", {}).get("pruned", {})
         if kernel in tk:
-            traffic = tk[kernel]["hbm_bytes_per_launch"]
+            traffic = int(tk[kernel]["hbm_bytes"] / launches)
     return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),

@@ -8,7 +8,10 @@
     python -m traccc_amd.rocprof_tables traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_mib>
         HBM bytes per launch from one FETCH_SIZE and one WRITE_SIZE counter
         run, each scaled by what the same counter reads for a device copy of
-        copy_mib MiB (read and written once), as JSON
+        copy_mib MiB (read and written once), as JSON: the mean over every
+        dispatch of the run, and the total of the last plain and the last
+        scorer-pruned CKF call (dispatches in order, calls end at
+        k_build_write)
 
 Host-side tooling for the committed profile summaries; nothing here touches
 a GPU.
@@ -91,6 +94,32 @@ def _counter_means(db: str) -> dict[str, tuple[int, float]]:
     return {k: (len(d), s / max(len(d), 1)) for k, (d, s) in acc.items()}
 
 
+def _call_sums(db: str) -> dict[str, dict[str, tuple[int, float]]]:
+    """Counter totals per kernel over the last plain and the last pruned CKF
+    call of the run: {kind: {kernel: (dispatches, summed value)}}."""
+    con = sqlite3.connect(db)
+    rows = list(con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                            "group by dispatch_id order by dispatch_id"))
+    last: dict[str, list] = {}
+    seg: list = []
+    for did, kn, v in rows:
+        seg.append((_short(kn), v))
+        if "k_build_write" in kn:
+            names = {n for n, _ in seg}
+            kind = ("pruned" if "k_find_list" in names else
+                    "logged" if "k_find<1>" in names else "plain")
+            last[kind] = seg
+            seg = []
+    out: dict[str, dict[str, tuple[int, float]]] = {}
+    for kind, sg in last.items():
+        acc: dict[str, list] = defaultdict(lambda: [0, 0.0])
+        for n, v in sg:
+            acc[n][0] += 1
+            acc[n][1] += v
+        out[kind] = {k: (c, t) for k, (c, t) in acc.items()}
+    return out
+
+
 def traffic(fetch_db: str, write_db: str, copy_fetch_db: str, copy_write_db: str,
             copy_mib: float) -> dict:
     kib = copy_mib * 1024.0
@@ -107,7 +136,15 @@ def traffic(fetch_db: str, write_db: str, copy_fetch_db: str, copy_write_db: str
         kernels[k] = {"dispatches": f[k][0], "fetch_kib_raw": round(f[k][1], 1),
                       "write_kib_raw": round(w[k][1], 1),
                       "hbm_bytes_per_launch": int(1024 * (fs * f[k][1] + ws * w[k][1]))}
-    return {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "kernels": kernels}
+    cf, cw = _call_sums(fetch_db), _call_sums(write_db)
+    calls = {}
+    for kind in sorted(set(cf) & set(cw)):
+        ks = set(cf[kind]) & set(cw[kind])
+        calls[kind] = {k: {"dispatches": cf[kind][k][0],
+                           "hbm_bytes": int(1024 * (fs * cf[kind][k][1] + ws * cw[kind][k][1]))}
+                       for k in sorted(ks, key=lambda k: -cf[kind][k][1])}
+    return {"fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "kernels": kernels,
+            "calls": calls}
 
 
 def main(argv: list[str]) -> int:
