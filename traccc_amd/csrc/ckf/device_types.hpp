@@ -141,10 +141,10 @@ struct ckf_params {
     // work buffers
     float* state[2];              // ping-pong propagated states, SoA [28][state_cap]
     uint32_t state_cap;           // links per step
-    float* flt;                   // filtered states SoA [28][flt_cap], slot = input * N + k
+    float* flt;                   // filtered states SoA [28][flt_cap], slot = k * in_cap + input
     stage_head* stage_in;         // [in_cap] the input's link bookkeeping, from k_find
-    stage_cand* stage_slot;       // [flt_cap] the filtered slot's candidate (slot input * N
-                                  // holds the hole when nothing was kept)
+    stage_cand* stage_slot;       // [flt_cap] the filtered slot's candidate (slot k = 0 of
+                                  // an input holds the hole when nothing was kept)
     uint32_t flt_cap;
     uint32_t in_cap;              // inputs per step
     dlink* links;
@@ -206,7 +206,8 @@ struct ckf_params {
     uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
     uint32_t* list_seed;          // [in_cap] seed of each listed input
     uint32_t* list_n;             // [in_cap] entries
-    float* list_feat;             // [in_cap][list_len][12] features of each listed candidate
+    short* list_row;              // [in_cap][list_len][16] each listed candidate's MLP row:
+                                  // features normalised to BF16, feature 9 raw FP32 in 12-13
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned
     // candidate-verdict log (diagnostics): one key per scored candidate,

@@ -362,6 +362,15 @@ __device__ inline stage_cand make_stage_cand(uint32_t m, float chi2, uint32_t di
     return c;
 }
 
+// Filtered-state slot of an input's k-th kept candidate: plane k of in_cap
+// slots, so the common case of one kept candidate per input fills
+// consecutive slots (coalesced SoA stores here and loads in k_propagate).
+// Past flt_cap it returns kInvalid, which every caller treats as overflow.
+__device__ inline uint32_t flt_slot(const ckf_params& P, uint32_t i, uint32_t k) {
+    const uint64_t s = static_cast<uint64_t>(k) * P.in_cap + i;
+    return s < P.flt_cap ? static_cast<uint32_t>(s) : kInvalid;
+}
+
 // The link a staged slot becomes (a hole link carries the input's own state).
 __device__ inline dlink staged_link(const stage_head& h, const stage_cand& c, uint32_t step,
                                     uint32_t slot) {
@@ -403,9 +412,34 @@ __device__ inline uint32_t emit_chunks(uint32_t n) {
     return c < 1u ? 1u : (c > kEmitChunks ? kEmitChunks : c);
 }
 
+// A listed candidate's MLP row (16 shorts, 32 B): features 0-8, 10 and 11
+// normalised and rounded to BF16 exactly as the scorer takes them, short 9
+// zero, and feature 9 (standing) as its raw FP32 running value in shorts
+// 12-13 -- it needs every seed's minimum of the step, so k_score_update
+// finalises it.  Shorts 14-15 are zero.
+constexpr int kRowShorts = 16;
+
+__device__ inline void store_list_row(const mlp_dev& W, short* row, const float (&f)[kFeatures]) {
+    uint32_t w[8];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int a = 2 * e, b = 2 * e + 1;
+        const uint16_t lo = a == kFeatStanding ? 0u
+                                               : static_cast<uint16_t>(f2bf((f[a] - W.mean[a]) * W.inv_std[a]));
+        const uint16_t hi = b == kFeatStanding ? 0u
+                                               : static_cast<uint16_t>(f2bf((f[b] - W.mean[b]) * W.inv_std[b]));
+        w[e] = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+    }
+    w[6] = __float_as_uint(f[kFeatStanding]);
+    w[7] = 0u;
+    uint4* dst = reinterpret_cast<uint4*>(row);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 // Scorer-pruned finding, first kernel: per input the list_len best candidates
-// in (chi2, index) order and their features, at the input's fixed slots
-// input * list_len + q; k_list_emit then packs the rows for the MLP.
+// in (chi2, index) order and their MLP rows, at the input's fixed slots
+// input * list_len + q; k_score_update scores and updates them.
 __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -470,12 +504,12 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                     P.list_m[r] = sm[j];
                     P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
+                    float f[kFeatures];
                     candidate_features(P, md, in.x, in.c, in.sf, step, in.skip, in.consec,
-                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass,
-                                       P.list_feat + r * kFeatures);
+                                       in.chi2_prev, in.ndf_prev, sc[j], q, n_pass, f);
+                    store_list_row(P.scorer, P.list_row + r * kRowShorts, f);
                     if (q == 0u) {
-                        standing_offer(P.seed_run_min, in.seed, epoch,
-                                       branch_run(in.chi2_prev, in.ndf_prev, sc[j], md.dims));
+                        standing_offer(P.seed_run_min, in.seed, epoch, f[kFeatStanding]);
                     }
                 }
             }
@@ -527,7 +561,6 @@ __device__ inline void log_verdict(const ckf_params& P, uint32_t step, uint32_t 
 // per sub-tile in LDS; then each lane walks its input's kept entries in list
 // order, runs the gain update until N succeed and stages the links as the
 // plain find does (k_find_emit builds them).
-constexpr int kRowShorts = 16;     // a feature row: 12 BF16 features, 4 zero
 constexpr uint32_t kRows = 128;    // LDS rows of a wave per packing pass
 constexpr int kListMax = kListTeam * kListSlots;
 constexpr uint32_t kSubTiles = (kWave * kListMax) / 16u;  // sub-tiles of a full partition
@@ -579,7 +612,7 @@ __device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t
             res = gain_update(md, fx, fc);
         }
         if (res == CKF_FIT_SUCCESS) {
-            const uint32_t slot = i * N + count;
+            const uint32_t slot = flt_slot(P, i, count);
             if (slot < P.flt_cap) {
                 store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
                 P.stage_slot[slot] = make_stage_cand(best_m, best_c, md.dims);
@@ -591,7 +624,7 @@ __device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t
     }
     if (count == 0u) {
         // no candidate kept: the input continues as one hole link
-        const uint32_t slot = i * N;
+        const uint32_t slot = flt_slot(P, i, 0u);
         if (slot < P.flt_cap) {
             P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
         } else {
@@ -647,19 +680,19 @@ void k_score_update(ckf_params P) {
                 if (r < r0 || r >= r0 + kRows) {
                     continue;
                 }
-                const float* f = P.list_feat + (slot0 + q) * kFeatures;
-                bf16x4 v[4];
-#pragma unroll
-                for (int e = 0; e < kRowShorts; ++e) {
-                    float x = e < kFeatures ? f[e] : 0.f;
-                    if (e == kFeatStanding) {
-                        x = standing_of(P.seed_run_min, seed, epoch, x);
-                    }
-                    v[e / 4][e % 4] = e < kFeatures ? mlp_feature_bf(V, x, e) : short(0);
-                }
-                bf16x4* row = reinterpret_cast<bf16x4*>(rows + (r - r0) * kRowShorts);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) row[e] = v[e];
+                const uint4* src = reinterpret_cast<const uint4*>(P.list_row + (slot0 + q) * kRowShorts);
+                const uint4 a = src[0];
+                uint4 b = src[1];
+                // feature 9 (short 9, the high half of b.x) from the raw
+                // running value and the seed's minimum; shorts 12-15 zero
+                const float st = standing_of(P.seed_run_min, seed, epoch, __uint_as_float(b.z));
+                const uint16_t bf9 = static_cast<uint16_t>(mlp_feature_bf(V, st, kFeatStanding));
+                b.x = (b.x & 0xffffu) | (static_cast<uint32_t>(bf9) << 16);
+                b.z = 0u;
+                b.w = 0u;
+                uint4* row = reinterpret_cast<uint4*>(rows + (r - r0) * kRowShorts);
+                row[0] = a;
+                row[1] = b;
             }
             wave_lds_sync();
             const uint32_t n_rows = total - r0 < kRows ? total - r0 : kRows;
@@ -765,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
                         res = gain_update(md, fx, fc);
                     }
                     if (res == CKF_FIT_SUCCESS) {
-                        const uint32_t slot = i * N + count;
+                        const uint32_t slot = flt_slot(P, i, count);
                         if (leader) {
                             if (slot < P.flt_cap) {
                                 store_soa(P.flt, P.flt_cap, slot, fx, fc, in.sf);
@@ -785,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_find(ckf_params P) {
             }
             if (count == 0u && leader) {
                 // no candidate kept: the input continues as one hole link
-                const uint32_t slot = i * N;
+                const uint32_t slot = flt_slot(P, i, 0u);
                 if (slot < P.flt_cap) {
                     P.stage_slot[slot] = stage_cand{kInvalid, 0.f};
                 } else {
@@ -816,7 +849,6 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
     const uint32_t kc = emit_chunks(n_in);
     const uint32_t part = kWave * kc;
     const uint32_t n_parts = (n_in + part - 1u) / part;
-    const uint32_t N = P.n_branch_surface;
     auto input_seed = [&](uint32_t j) -> uint32_t { return P.stage_in[j].seed; };
     const bool takes = wave_takes_tickets(n_parts);
     for (;;) {
@@ -873,7 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_find_emit(ckf_params P) {
                 P.seed_first[seed] = first;
             }
             for (uint32_t k = 0; k < no; ++k) {
-                const uint32_t slot = i * N + k;
+                const uint32_t slot = flt_slot(P, i, k);
                 if (slot >= P.flt_cap) {
                     break;  // k_find raised the overflow that discards this step
                 }

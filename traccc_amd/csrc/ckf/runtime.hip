@@ -96,7 +96,7 @@ struct ckf_algorithm {
     // measurement votes (max_num_tracks_per_measurement > 0)
     dbuf vote_slots, vote_key, vote_keep, vote_rank, vote_cpos, vote_misc;
     // scorer training data (branch log) and scorer-pruned finding
-    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_feat,
+    dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row,
         find_count, seed_run_min, list_seed, prop_flag;
     bool logging = false;
     // candidate-verdict log of scored calls (diagnostics)
@@ -132,7 +132,7 @@ struct ckf_algorithm {
                        &vote_key, &vote_keep, &vote_rank, &vote_cpos, &vote_misc,
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_lo, &feat_slot, &log_feat,
-                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_feat, &find_count,
+                       &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &find_count,
                        &seed_run_min, &list_seed, &prop_flag, &verdict_log, &verdict_n};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < tamd::kLookbacks; ++k) {
@@ -225,7 +225,7 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
         A.list_seed.ensure(4u * size_t(A.in_cap));
         A.list_m.ensure(4u * rows);
         A.list_c.ensure(4u * rows);
-        A.list_feat.ensure(4u * 12u * rows);
+        A.list_row.ensure(2u * 16u * rows);
         A.list_n.ensure(4u * size_t(A.in_cap));
     }
     if (A.h_ctl == nullptr) {
@@ -320,7 +320,7 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
         P.list_m = A.list_m.as<uint32_t>();
         P.list_c = A.list_c.as<float>();
         P.list_n = A.list_n.as<uint32_t>();
-        P.list_feat = A.list_feat.as<float>();
+        P.list_row = A.list_row.as<short>();
         P.scorer = tamd::scorer_view(A.scorer);
         P.score_threshold = A.score_threshold;
         if (A.verdict_cap > 0u) {
