@@ -275,7 +275,8 @@ def plain_vs_pruned(a, ckf, S, torch, wl, n_particles, n_test, label_events, n_v
         torch.cuda.synchronize()
         dt_c = (time.perf_counter() - t0) / 5
         nt, ns = fitted.sizes()
-        outc = fitted.download().fit_outcome
+        fd = fitted.download()
+        outc = fd.fit_outcome
         ok = float(np.mean(outc == FT.OUTCOME_SUCCESS)) if len(outc) else 0.0
         log(f"[bench] {wl} fit: {nt} tracks, {ns} states in {dt_f * 1e3:.2f} ms "
             f"({nt / dt_f:.4g} fitted tracks/s, {ok:.4f} fitted); pruned CKF + fit "
@@ -285,7 +286,55 @@ def plain_vs_pruned(a, ckf, S, torch, wl, n_particles, n_test, label_events, n_v
                        "success_fraction": round(ok, 5)}
         line["pruned_ckf_plus_fit"] = {"value": round(n_seeds / dt_c, 1), "unit": "tracks/s",
                                        "ms_per_call": round(dt_c * 1e3, 3)}
+        line["ambiguity"] = ambiguity_line(wl, dev, out_s, fitted, fd, stream, torch)
     return line, ctx
+
+
+def ambiguity_line(wl, dev, cands, fitted, fd, stream, torch):
+    """Greedy ambiguity resolution of the fitted tracks (the step after the
+    fit), timed per operator call on the host clock (the call returns with
+    the accepted flags on the host); the oracle timed on a few events of the
+    same tracks on one host core."""
+    import oracle
+    from traccc_amd import ambiguity as AMB
+    res = AMB.greedy_ambiguity_resolution_algorithm(stream=stream)
+    res.run_fitted(dev, fitted)
+    times = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kept = res.run_fitted(dev, fitted)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    st = res.last_stats()
+    nt = len(fd)
+    # the oracle on the first events' tracks (identifiers: measurement index
+    # in the batch, as the resolver takes them)
+    ev_off = cands.event_offsets()
+    n_ev = min(4, len(ev_off) - 1)
+    t1 = int(ev_off[n_ev])
+    s1 = int(fd.state_offsets[t1])
+    meas_off = np.zeros(len(ev_off), np.uint64)
+    meas_off[1:] = np.cumsum(dev.event_n_meas)
+    ids = fd.measurement_index[:s1].astype(np.uint64)
+    for e in range(n_ev):
+        a, b = fd.state_offsets[ev_off[e]], fd.state_offsets[ev_off[e + 1]]
+        ids[a:b] += meas_off[e]
+    t0 = time.perf_counter()
+    keep, _ = oracle.greedy_resolve(fd.state_offsets[:t1 + 1], ids.astype(np.uint32), fd.pval[:t1])
+    cpu_s = time.perf_counter() - t0
+    agree = bool(np.array_equal(np.nonzero(keep)[0], kept[kept < t1]))
+    log(f"[bench] {wl} ambiguity resolution: {nt} fitted tracks -> {st.n_accepted} kept "
+        f"({st.n_removed} removed in {st.n_rounds} rounds) in {dt * 1e3:.2f} ms "
+        f"({nt / dt:.4g} tracks/s); oracle on {n_ev} events ({t1} tracks) {cpu_s:.2f} s on 1 core "
+        f"({t1 / cpu_s:.4g} tracks/s), same kept set: {agree}")
+    return {"tracks": int(nt), "kept": int(st.n_accepted), "removed": int(st.n_removed),
+            "rounds": int(st.n_rounds), "ms_per_call": round(dt * 1e3, 3),
+            "tracks_per_s": round(nt / dt, 1),
+            "timing": "median of 5 operator calls, host clock, flags returned to the host",
+            "cpu_oracle": {"tracks": t1, "events": n_ev, "seconds": round(cpu_s, 3),
+                           "tracks_per_s": round(t1 / cpu_s, 1), "cores": 1},
+            "oracle_agrees": agree}
 
 
 def propagate_roofline(ctx, kt, workload_tag):

@@ -117,6 +117,10 @@ def lib() -> C.CDLL:
         L.oracle_propagate_direct.restype = C.c_int
         L.oracle_rk_step.argtypes = [C.POINTER(F.BField), C.c_double, C.c_double, C.c_double,
                                      dptr, dptr, dptr]
+        L.oracle_greedy_resolve.argtypes = [C.c_uint32, F.c_u32_p, F.c_u32_p, F.c_float_p,
+                                            C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]
+        L.oracle_greedy_resolve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -357,3 +361,27 @@ def fit(fcfg, det, fld, event, seed_surface_link, seed_vector, seed_covariance, 
     finally:
         L.oracle_fitted_free(C.byref(out))
     return r
+
+
+def greedy_resolve(offsets, meas_ids, pval, min_meas_per_track: int = 3,
+                   max_iterations: int = 0xffffffff, max_shared_meas: int = 1):
+    """traccc's host greedy ambiguity resolution restated
+    (core/src/ambiguity_resolution/greedy_ambiguity_resolution_algorithm.cpp:19-254).
+    Track t owns the measurement identifiers meas_ids[offsets[t]:offsets[t+1]].
+    Returns (accepted flags as a bool array, number of tracks removed)."""
+    L = lib()
+    off = np.ascontiguousarray(offsets, np.uint32)
+    ids = np.ascontiguousarray(meas_ids, np.uint32)
+    pv = np.ascontiguousarray(pval, np.float32)
+    n = len(off) - 1
+    if len(pv) != n or (n > 0 and len(ids) < off[-1]):
+        raise ValueError("inconsistent track arrays")
+    acc = np.zeros(max(n, 1), np.uint8)
+    nr = C.c_uint32()
+    rc = L.oracle_greedy_resolve(n, F.ptr(off, C.c_uint32), F.ptr(ids, C.c_uint32),
+                                 F.ptr(pv, C.c_float), min_meas_per_track, max_iterations,
+                                 max_shared_meas, acc.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 C.byref(nr))
+    if rc != 0:
+        raise ValueError(f"oracle_greedy_resolve failed ({rc})")
+    return acc[:n].astype(bool), nr.value

@@ -18,6 +18,7 @@
 
 #include "traccc_amd/ckf.h"
 #include "traccc_amd/fitting.h"
+#include "traccc_amd/seeding.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -128,6 +129,39 @@ int oracle_propagate_direct(const ckf_finding_config* cfg, const ckf_detector* d
                             const ckf_bfield* field, uint32_t start, uint32_t target,
                             const double vec[6], const double cov[36], double out_vec[6],
                             double out_cov[36]);
+
+/* Greedy ambiguity resolution (ambiguity_oracle.cpp), traccc host
+ * greedy_ambiguity_resolution_algorithm restated.  Tracks t own the
+ * measurement identifiers meas_ids[offsets[t], offsets[t+1]).  accepted[t]
+ * = 1 for every track kept; *n_removed = tracks removed by the iterations.
+ * 0 on success. */
+int oracle_greedy_resolve(uint32_t n_tracks, const uint32_t* offsets, const uint32_t* meas_ids,
+                          const float* pval, uint32_t min_meas_per_track, uint32_t max_iterations,
+                          uint32_t max_shared_meas, uint8_t* accepted, uint32_t* n_removed);
+
+/* Seeding (seeding_oracle.cpp), traccc host seeding_algorithm restated in
+ * single precision.  Seeds in the reference's order (grid bin, middle
+ * spacepoint, then the filter's order). */
+typedef struct oracle_seeds {
+    uint32_t n_seeds;
+    uint32_t n_phi_bins, n_z_bins;
+    uint32_t* bottom;
+    uint32_t* middle;
+    uint32_t* top;
+    float* weight;
+} oracle_seeds;
+void oracle_seedfinder_setup(ckf_seedfinder_config* cfg);
+/* 0 ok, 2 = a grid configuration the reference rejects (std::domain_error) */
+int oracle_seeding(const ckf_seedfinder_config* finder, const ckf_spacepoint_grid_config* grid,
+                   const ckf_seedfilter_config* filter, uint32_t n_sp, const float* global,
+                   const float* var_r, const float* var_z, oracle_seeds* out);
+void oracle_seeds_free(oracle_seeds* s);
+/* Track parameters of one seed (bottom p0, middle p1, top p2) in double
+ * precision: vec = (loc0, loc1, phi, theta, q/p, t), cov_diag = the
+ * covariance diagonal. */
+int oracle_track_params(const ckf_track_params_estimation_config* cfg, const double b[3],
+                        const double p0[3], const double p1[3], const double p2[3],
+                        const double bottom_local[2], double vec[6], double cov_diag[6]);
 
 /* One RK4 step of a free state [x, y, z, t, Tx, Ty, Tz, q/p] with the
  * transport matrix d(free_out)/d(free_in); for tests. */

@@ -1,6 +1,7 @@
 // Host code of the project under AddressSanitizer and
 // UndefinedBehaviorSanitizer (built by tests/test_sanitizers.py with g++,
-// no device code): the CPU oracle (CKF in FP64 and FP32, the Kalman fit),
+// no device code): the CPU oracle (CKF in FP64 and FP32, the Kalman fit,
+// the greedy ambiguity resolution),
 // the synthetic-input toolkit (detectors, solenoid map, particle gun) and the
 // CSV measurement reader / writer, including malformed files -- the reader
 // parses text from outside the program.  Exits 0 when every call behaved;
@@ -128,6 +129,39 @@ void run_oracle(uint32_t preset, float x0, bool map, uint32_t n_particles, uint3
     tamd_detector_free(&det);
 }
 
+// Greedy ambiguity resolution on random track sets: duplicated identifiers,
+// empty tracks, every configuration path.
+void run_ambiguity() {
+    uint64_t rng = 12345;
+    auto next = [&rng](uint32_t mod) {
+        rng = rng * 6364136223846793005ull + 1442695040888963407ull;
+        return static_cast<uint32_t>((rng >> 33) % mod);
+    };
+    for (int rep = 0; rep < 6; ++rep) {
+        const uint32_t n = 50 + next(400);
+        std::vector<uint32_t> off(n + 1, 0), ids;
+        std::vector<float> pv(n);
+        for (uint32_t t = 0; t < n; ++t) {
+            const uint32_t len = next(12);
+            for (uint32_t j = 0; j < len; ++j) ids.push_back(next(rep % 2 ? 60 : 600));
+            off[t + 1] = static_cast<uint32_t>(ids.size());
+            pv[t] = static_cast<float>(next(1000)) / 1000.f;
+        }
+        std::vector<uint8_t> acc(n);
+        const uint32_t cfgs[4][3] = {{3, 0xffffffffu, 1}, {0, 0xffffffffu, 1}, {3, 7, 1}, {2, 0xffffffffu, 2}};
+        for (const auto& c : cfgs) {
+            uint32_t removed = 0;
+            expect(oracle_greedy_resolve(n, off.data(), ids.data(), pv.data(), c[0], c[1], c[2],
+                                         acc.data(), &removed) == 0,
+                   "greedy resolve");
+        }
+    }
+    uint8_t a = 0;
+    expect(oracle_greedy_resolve(0, nullptr, nullptr, nullptr, 3, 10, 1, &a, nullptr) == 0, "empty");
+    expect(oracle_greedy_resolve(0, nullptr, nullptr, nullptr, 3, 10, 0, &a, nullptr) != 0,
+           "zero threshold rejected");
+}
+
 void write_file(const std::string& path, const std::string& text) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) {
@@ -204,6 +238,7 @@ int main(int argc, char** argv) {
     run_oracle(TAMD_DET_BARREL4_DISK4, 0.01f, false, 300, 3);
     run_oracle(TAMD_DET_BARREL5_DISK6, 0.02f, true, 300, 3);
     run_oracle(TAMD_DET_TELESCOPE, 0.005f, false, 20, 2);
+    run_ambiguity();
     run_csv(argv[1], argv[2]);
     std::printf("sanitized run: %d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;

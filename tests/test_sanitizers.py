@@ -4,7 +4,8 @@ Builds tests/cpp/sanitize_driver.cpp with g++ -fsanitize=address,undefined
 together with the CPU oracle, the synthetic-input toolkit and the CSV
 measurement reader / writer (no device code on the line), then runs it: the
 oracle's CKF in FP64 and FP32, its Kalman fit and measurement votes on the
-C1-, C2-, C5-like and telescope geometries (solenoid map included), the CSV
+C1-, C2-, C5-like and telescope geometries (solenoid map included), its
+greedy ambiguity resolution on random track sets, the CSV
 round trip of the reference's mock measurement file and a set of malformed
 CSV files.  Any sanitizer finding aborts the driver with its report.
 """
@@ -15,7 +16,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SOURCES = ["tests/cpp/sanitize_driver.cpp", "oracle/ckf_oracle.cpp",
+SOURCES = ["tests/cpp/sanitize_driver.cpp", "oracle/ckf_oracle.cpp", "oracle/ambiguity_oracle.cpp",
            "traccc_amd/csrc/sim/simulation.cpp", "traccc_amd/csrc/common/config.cpp",
            "traccc_amd/csrc/common/last_error.cpp", "traccc_amd/csrc/io/csv.cpp"]
 

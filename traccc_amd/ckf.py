@@ -140,6 +140,17 @@ class DeviceTracks:
         _check(_lib().ckf_tracks_size(self._h, C.byref(nt), C.byref(nc)))
         return nt.value, nc.value
 
+    def event_offsets(self) -> np.ndarray:
+        """[n_events + 1] first track of each event in the batch."""
+        lib = _lib()
+        ne = C.c_uint32()
+        _check(lib.ckf_tracks_event_offsets(self._h, C.byref(ne), None, None))
+        eto = np.zeros(ne.value + 1, dtype=np.uint32)
+        eco = np.zeros(ne.value + 1, dtype=np.uint32)
+        _check(lib.ckf_tracks_event_offsets(self._h, C.byref(ne), F.ptr(eto, C.c_uint32),
+                                            F.ptr(eco, C.c_uint32)))
+        return eto
+
     def download(self) -> TrackContainer:
         lib = _lib()
         nt, nc = self.sizes()
@@ -203,6 +214,7 @@ class DeviceEvents:
         self._L = _lib()
         self.n_events = n
         self.n_seeds = sum(e.n_seeds for e in events)
+        self.event_n_meas = [e.n_meas for e in events]
 
     def __del__(self):
         # the library handle is held by the object: module globals may

@@ -151,6 +151,22 @@ struct ckf_device_event {
     }
 };
 
+// Fitted tracks in HBM (ckf_fit_tracks): per-track columns, the
+// candidates' constituents (event-relative measurement indices) and the
+// per-state filtered / smoothed records.
+struct ckf_device_fitted {
+    uint32_t n_tracks = 0, n_states = 0;
+    std::vector<uint64_t> ids;            // surface index -> identifier
+    std::vector<uint32_t> ev_track_off;   // [n_events + 1] tracks of each event
+    tamd::dbuf outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons, flt, smt, flags,
+        state_sf;
+    ~ckf_device_fitted() {
+        tamd::dbuf* bs[] = {&outcome, &surface, &vec, &cov, &ndf, &chi2, &pval, &nholes,
+                            &offsets, &cons, &flt, &smt, &flags, &state_sf};
+        for (tamd::dbuf* b : bs) b->release();
+    }
+};
+
 struct ckf_device_tracks {
     uint32_t n_events = 0, n_tracks = 0, n_const = 0;
     std::vector<uint32_t> ev_track_off, ev_const_off;

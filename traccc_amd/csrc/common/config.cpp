@@ -9,6 +9,7 @@
 
 #include "traccc_amd/ckf.h"
 #include "traccc_amd/fitting.h"
+#include "traccc_amd/seeding.h"
 
 extern "C" void ckf_finding_config_default(ckf_finding_config* c) {
     if (c == nullptr) {
@@ -70,4 +71,108 @@ extern "C" void ckf_fitting_config_default(ckf_fitting_config* cfg) {
     cfg->surface_sequence_size_factor = 5u;
     cfg->min_surface_sequence_capacity = 100u;
     cfg->do_precise_hole_count = 0u;
+}
+
+// seedfinder_config defaults (core/include/traccc/seeding/detail/seeding_config.hpp:21-110),
+// in traccc units: mm, GeV, field in GeV / (e mm) (1 T = 0.000299792458).
+extern "C" void ckf_seedfinder_config_default(ckf_seedfinder_config* c) {
+    if (c == nullptr) {
+        return;
+    }
+    std::memset(c, 0, sizeof(*c));
+    c->zMin = -2000.f;
+    c->zMax = 2000.f;
+    c->rMax = 200.f;
+    c->rMin = 33.f;
+    c->collisionRegionMin = -250.f;
+    c->collisionRegionMax = 250.f;
+    c->phiMin = static_cast<float>(-M_PI);
+    c->phiMax = static_cast<float>(M_PI);
+    c->minPt = 500.f * 0.001f;
+    c->cotThetaMax = 27.2845f;
+    c->deltaRMin = 20.f;
+    c->deltaRMax = 80.f;
+    c->deltaZMax = 450.f;
+    c->impactMax = 10.f;
+    c->sigmaScattering = 3.0f;
+    c->maxPtScattering = 10.f;
+    c->maxSeedsPerSpM = 5u;
+    c->bFieldInZ = 1.99724f * 0.000299792458f;
+    c->radLengthPerSeed = 0.05f;
+    c->sigmaError = 5.f;
+    c->phiBinDeflectionCoverage = 1;
+    c->neighbor_scope[0] = 1u;
+    c->neighbor_scope[1] = 1u;
+    ckf_seedfinder_config_setup(c);
+}
+
+// seedfinder_config::setup (seeding_config.hpp:119-134); MeV = 0.001 GeV
+extern "C" void ckf_seedfinder_config_setup(ckf_seedfinder_config* c) {
+    if (c == nullptr) {
+        return;
+    }
+    c->highland = 13.6f * 0.001f * std::sqrt(c->radLengthPerSeed) *
+                  (1.f + 0.038f * std::log(c->radLengthPerSeed));
+    const float maxScatteringAngle = c->highland / c->minPt;
+    c->maxScatteringAngle2 = maxScatteringAngle * maxScatteringAngle;
+    c->pTPerHelixRadius = c->bFieldInZ;
+    c->minHelixDiameter2 = std::pow(c->minPt * 2.f / c->pTPerHelixRadius, 2.f);
+    c->minHelixRadius = std::sqrt(c->minHelixDiameter2) / 2.f;
+    c->pT2perRadius = std::pow(c->highland / c->pTPerHelixRadius, 2.f);
+}
+
+// spacepoint_grid_config(const seedfinder_config&) (seeding_config.hpp:147-160)
+extern "C" void ckf_spacepoint_grid_config_from(const ckf_seedfinder_config* f,
+                                                ckf_spacepoint_grid_config* g) {
+    if (f == nullptr || g == nullptr) {
+        return;
+    }
+    g->bFieldInZ = f->bFieldInZ;
+    g->minPt = f->minPt;
+    g->rMax = f->rMax;
+    g->zMax = f->zMax;
+    g->zMin = f->zMin;
+    g->deltaRMax = f->deltaRMax;
+    g->cotThetaMax = f->cotThetaMax;
+    g->impactMax = f->impactMax;
+    g->phiMin = f->phiMin;
+    g->phiMax = f->phiMax;
+    g->phiBinDeflectionCoverage = f->phiBinDeflectionCoverage;
+}
+
+// seedfilter_config defaults (seeding_config.hpp:202-238)
+extern "C" void ckf_seedfilter_config_default(ckf_seedfilter_config* c) {
+    if (c == nullptr) {
+        return;
+    }
+    c->deltaInvHelixDiameter = 0.00003f;
+    c->impactWeightFactor = 1.f;
+    c->compatSeedWeight = 200.f;
+    c->deltaRMin = 5.f;
+    c->compatSeedLimit = 2u;
+    c->good_spB_min_radius = 150.f;
+    c->good_spB_weight_increase = 400.f;
+    c->good_spT_max_radius = 150.f;
+    c->good_spT_weight_increase = 200.f;
+    c->good_spB_min_weight = 380.f;
+    c->seed_min_weight = 200.f;
+    c->spB_min_radius = 43.f;
+}
+
+// track_params_estimation_config defaults
+// (core/include/traccc/seeding/detail/track_params_estimation_config.hpp:18-31);
+// 1 degree = pi / 180, 1 ns = 1 (traccc's time unit)
+extern "C" void ckf_track_params_estimation_config_default(ckf_track_params_estimation_config* c) {
+    if (c == nullptr) {
+        return;
+    }
+    const float deg = static_cast<float>(M_PI / 180.0);
+    const float sig[6] = {1.f, 1.f, deg, deg, 0.f, 1.f};
+    const float infl[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 100.f};
+    for (int i = 0; i < 6; ++i) {
+        c->initial_sigma[i] = sig[i];
+        c->initial_inflation[i] = infl[i];
+    }
+    c->initial_sigma_qopt = 0.1f;
+    c->initial_sigma_pt_rel = 0.1f;
 }

@@ -498,18 +498,6 @@ struct ckf_fitter {
     tamd::dbuf seq, fw_info, ev_track_off;
 };
 
-struct ckf_device_fitted {
-    uint32_t n_tracks = 0, n_states = 0;
-    std::vector<uint64_t> ids;  // surface index -> identifier
-    tamd::dbuf outcome, surface, vec, cov, ndf, chi2, pval, nholes, offsets, cons, flt, smt, flags,
-        state_sf;
-    ~ckf_device_fitted() {
-        tamd::dbuf* bs[] = {&outcome, &surface, &vec, &cov, &ndf, &chi2, &pval, &nholes,
-                            &offsets, &cons, &flt, &smt, &flags, &state_sf};
-        for (tamd::dbuf* b : bs) b->release();
-    }
-};
-
 namespace {
 
 void validate(const ckf_fitting_config& c) {
@@ -599,6 +587,7 @@ ckf_status ckf_fit_tracks(ckf_fitter* f, const ckf_device_detector* det, const c
             R->n_tracks = n;
             R->n_states = ns;
             R->ids = det->ids;
+            R->ev_track_off = cand->ev_track_off;
             R->outcome.ensure(2u * std::max(n, 1u));
             R->surface.ensure(4u * std::max(n, 1u));
             R->vec.ensure(24u * std::max(n, 1u));
