@@ -138,7 +138,8 @@ def train_scorer(a, ckf, S, torch, det, fld, label_events, cfg, tag):
     t_log = time.perf_counter() - t0
     sc = SC.Scorer(seed=7)
     sc.fit_normalisation(X)
-    st = sc.train(X, Y, SC.train_config(epochs=a.scorer_epochs, batch=16384, learning_rate=2e-3))
+    st = sc.train(X, Y, SC.train_config(epochs=a.scorer_epochs, batch=a.scorer_batch,
+                                        learning_rate=a.scorer_lr))
     n_samples = int(X.shape[0])
     survive = float(Y.float().mean())
     # MFMA utilisation of inference: the scorer's forward kernel over the
@@ -447,6 +448,52 @@ def cpu_baseline_c5(a, ctx, oracle):
                       f"per thread, {secs:.2f} s wall"}
 
 
+def seeding_line(a, torch, stream):
+    """Triplet seeding (the step before the CKF) on the spacepoints of one
+    C2-configuration event (10,000 muons, spacepoints already in HBM):
+    median of HIP-event-timed calls; the oracle on the same spacepoints on
+    one host core; both seed lists compared."""
+    import oracle
+    from traccc_amd import ckf
+    from traccc_amd import seeding as SD
+    from traccc_amd import simulation as S
+    det, fld, gun = S.workload("C2", seed=42)
+    ev = S.generate_event(det, fld, gun)
+    ddet = ckf.DeviceDetector(det, fld)
+    dev = ckf.DeviceEvents(ddet, ev)
+    xyz, midx = SD.spacepoint_formation(ddet, dev, 0)
+    finder = SD.seedfinder_config(bFieldInZ=2.0 * SD.TESLA)
+    grid = SD.spacepoint_grid_config(finder)
+    filt = SD.seedfilter_config()
+    alg = SD.seeding_algorithm(finder, grid, filt, stream=stream)
+    d_xyz = torch.from_numpy(xyz).cuda()
+    got = alg.run_device(d_xyz)
+    times = []
+    for _ in range(max(a.steps, 10)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        got = alg.run_device(d_xyz)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    t0 = time.perf_counter()
+    b, m, t, w, _ = oracle.seeding(finder, grid, filt, xyz)
+    cpu_s = time.perf_counter() - t0
+    same = (len(b) == len(got) and np.array_equal(b, got.bottom) and np.array_equal(m, got.middle)
+            and np.array_equal(t, got.top) and np.array_equal(w.view(np.uint32), got.weight.view(np.uint32)))
+    log(f"[bench] seeding: {len(xyz)} spacepoints -> {len(got)} seeds in {ms:.3f} ms "
+        f"({len(xyz) / ms * 1e3:.4g} spacepoints/s); oracle {cpu_s:.2f} s on 1 core; same seeds: {same}")
+    return {"workload": "spacepoints of one C2-configuration event (10,000 muons, 4 barrel + 2x4 "
+                        "disk layers, 5% noise), traccc default seedfinder config at 2 T",
+            "spacepoints": int(len(xyz)), "seeds": int(len(got)), "ms_per_call": round(ms, 3),
+            "spacepoints_per_s": round(len(xyz) / ms * 1e3, 1),
+            "timing": "median of the timed calls; the call ends with the seed count on the host",
+            "cpu_oracle": {"seconds": round(cpu_s, 3), "cores": 1,
+                           "spacepoints_per_s": round(len(xyz) / cpu_s, 1)},
+            "oracle_agrees": bool(same)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,7 +512,10 @@ def main() -> int:
     ap.add_argument("--c4-label-events", type=int, default=336,
                     help="C2-configuration events the C4 scorer's labels are logged on")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-seeding", action="store_true")
     ap.add_argument("--scorer-epochs", type=int, default=3)
+    ap.add_argument("--scorer-batch", type=int, default=16384, help="samples per Adam step")
+    ap.add_argument("--scorer-lr", type=float, default=2e-3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -500,6 +550,7 @@ def main() -> int:
     log(f"[bench] roofline {roofline}")
 
     c2 = None if a.no_c2 else c2_line(a, ckf, S, torch, stream)
+    seeding = None if a.no_seeding else seeding_line(a, torch, stream)
     c3 = c3_line(a, ckf, S, torch, stream) if a.batched_events > 0 else None
     c4 = None
     if a.c4_events > 0:
@@ -545,6 +596,7 @@ def main() -> int:
         "c2": c2,
         "batched": c3,
         "c4_plain_vs_pruned": c4,
+        "seeding": seeding,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -44,6 +44,12 @@ class OracleTracks(C.Structure):
     ]
 
 
+class OracleSeeds(C.Structure):
+    _fields_ = [("n_seeds", C.c_uint32), ("n_phi_bins", C.c_uint32), ("n_z_bins", C.c_uint32),
+                ("bottom", F.c_u32_p), ("middle", F.c_u32_p), ("top", F.c_u32_p),
+                ("weight", F.c_float_p)]
+
+
 class OracleFitted(C.Structure):
     _d = C.POINTER(C.c_double)
     _fields_ = [
@@ -121,6 +127,18 @@ def lib() -> C.CDLL:
                                             C.c_uint32, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]
         L.oracle_greedy_resolve.restype = C.c_int
+        from traccc_amd import seeding as SD  # struct layouts only
+        L.oracle_seedfinder_setup.argtypes = [C.POINTER(SD.SeedfinderConfig)]
+        L.oracle_seedfinder_setup.restype = None
+        L.oracle_seeding.argtypes = [C.POINTER(SD.SeedfinderConfig),
+                                     C.POINTER(SD.SpacepointGridConfig),
+                                     C.POINTER(SD.SeedfilterConfig), C.c_uint32, F.c_float_p,
+                                     F.c_float_p, F.c_float_p, C.POINTER(OracleSeeds)]
+        L.oracle_seeding.restype = C.c_int
+        L.oracle_seeds_free.argtypes = [C.POINTER(OracleSeeds)]
+        L.oracle_track_params.argtypes = [C.POINTER(SD.TrackParamsEstimationConfig), dptr, dptr,
+                                          dptr, dptr, dptr, dptr, dptr]
+        L.oracle_track_params.restype = C.c_int
         _lib = L
     return _lib
 
@@ -385,3 +403,50 @@ def greedy_resolve(offsets, meas_ids, pval, min_meas_per_track: int = 3,
     if rc != 0:
         raise ValueError(f"oracle_greedy_resolve failed ({rc})")
     return acc[:n].astype(bool), nr.value
+
+
+def seedfinder_setup(cfg):
+    """traccc seedfinder_config::setup restated (the derived fields)."""
+    lib().oracle_seedfinder_setup(C.byref(cfg))
+    return cfg
+
+
+def seeding(finder, grid, filt, global_xyz, var_r=None, var_z=None):
+    """traccc's host seeding_algorithm restated in single precision
+    (core/src/seeding/seed_finding.cpp:57-121 and its helpers).  Returns
+    (bottom, middle, top, weight, (n_phi_bins, n_z_bins))."""
+    L = lib()
+    g = np.ascontiguousarray(global_xyz, np.float32).reshape(-1, 3)
+    vr = None if var_r is None else np.ascontiguousarray(var_r, np.float32)
+    vz = None if var_z is None else np.ascontiguousarray(var_z, np.float32)
+    out = OracleSeeds()
+    rc = L.oracle_seeding(C.byref(finder), C.byref(grid), C.byref(filt), len(g),
+                          F.ptr(g, C.c_float), None if vr is None else F.ptr(vr, C.c_float),
+                          None if vz is None else F.ptr(vz, C.c_float), C.byref(out))
+    if rc == 2:
+        raise ValueError("grid configuration rejected (the reference throws std::domain_error)")
+    if rc != 0:
+        raise ValueError(f"oracle_seeding failed ({rc})")
+    try:
+        n = out.n_seeds
+        r = (F.np_from(out.bottom, n, np.uint32), F.np_from(out.middle, n, np.uint32),
+             F.np_from(out.top, n, np.uint32), F.np_from(out.weight, n, np.float32),
+             (out.n_phi_bins, out.n_z_bins))
+    finally:
+        L.oracle_seeds_free(C.byref(out))
+    return r
+
+
+def track_params(cfg, bfield, p0, p1, p2, bottom_local=(0.0, 0.0)):
+    """traccc's track parameters of one seed restated in double precision
+    (track_params_estimation_helper.hpp:47-125, track_params_estimation.cpp:60-87).
+    Returns (vector [6], covariance diagonal [6])."""
+    L = lib()
+    d = C.c_double * 3
+    vec = (C.c_double * 6)()
+    cov = (C.c_double * 6)()
+    rc = L.oracle_track_params(C.byref(cfg), d(*bfield), d(*p0), d(*p1), d(*p2),
+                               (C.c_double * 2)(*bottom_local), vec, cov)
+    if rc != 0:
+        raise ValueError("oracle_track_params failed")
+    return np.array(vec[:]), np.array(cov[:])

@@ -28,6 +28,7 @@ HIP_LIB = os.path.join(F.LIB_DIR, "libtraccc_amd.so")
                                         ("scorer.h", "libtraccc_amd.so"),
                                         ("fitting.h", "libtraccc_amd.so"),
                                         ("ambiguity.h", "libtraccc_amd.so"),
+                                        ("seeding.h", "libtraccc_amd.so"),
                                         ("io.h", "libtraccc_amd.so"),
                                         ("simulation.h", "libtraccc_amd_sim.so")])
 def test_every_declared_symbol_is_exported(header, lib):

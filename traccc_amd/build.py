@@ -31,6 +31,7 @@ HIP_SOURCES = [
     os.path.join(CSRC, "scorer", "mlp.hip"),
     os.path.join(CSRC, "fit", "fit.hip"),
     os.path.join(CSRC, "ambiguity", "greedy.hip"),
+    os.path.join(CSRC, "seeding", "seeding.hip"),
     os.path.join(CSRC, "common", "config.cpp"),
     os.path.join(CSRC, "common", "last_error.cpp"),
     os.path.join(CSRC, "io", "csv.cpp"),

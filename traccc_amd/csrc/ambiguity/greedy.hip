@@ -53,6 +53,7 @@
 #include "../ckf/handles.hpp"
 #include "../ckf/wave.hpp"
 #include "../common/errors.hpp"
+#include "../common/scan.hpp"
 #include "traccc_amd/ambiguity.h"
 
 namespace tamd {
@@ -60,8 +61,6 @@ namespace {
 
 constexpr uint32_t kAmbBlock = 256;
 constexpr uint32_t kSeqBlock = 1024;
-constexpr uint32_t kScanPer = 4;                          // items per thread of the scan
-constexpr uint32_t kScanTile = kAmbBlock * kScanPer;      // items per scan block
 constexpr uint32_t kDead = 0u, kAlive = 1u;
 constexpr uint32_t kNone = 0xffffffffu;
 
@@ -186,68 +185,6 @@ __global__ __launch_bounds__(kAmbBlock) void k_amb_count(amb_params P) {
             }
         }
         wave_count(&P.ctl->n_candidates, cand);
-    }
-}
-
-// Exclusive scan of n u32 into out[0..n] (out[n] = total), three passes.
-__global__ __launch_bounds__(kAmbBlock) void k_scan_local(const uint32_t* in, uint32_t* out,
-                                                          uint32_t* sums, uint32_t n) {
-    __shared__ uint32_t s_w[kAmbBlock / kWave];
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-    uint32_t v[kScanPer], sum = 0u;
-#pragma unroll
-    for (uint32_t k = 0; k < kScanPer; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        sum += v[k];
-    }
-    const uint32_t incl = wave_inclusive_scan(sum);
-    const uint32_t w = threadIdx.x / kWave;
-    if (lane() == kWave - 1) s_w[w] = incl;
-    __syncthreads();
-    uint32_t woff = 0u;
-    for (uint32_t q = 0; q < w; ++q) woff += s_w[q];
-    uint32_t ex = woff + incl - sum;
-#pragma unroll
-    for (uint32_t k = 0; k < kScanPer; ++k) {
-        if (base + k < n) out[base + k] = ex;
-        ex += v[k];
-    }
-    if (threadIdx.x == blockDim.x - 1u) sums[blockIdx.x] = woff + incl;
-}
-
-__global__ __launch_bounds__(kSeqBlock) void k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* total,
-                                                         uint32_t* total_copy) {
-    __shared__ uint32_t s_w[kSeqBlock / kWave];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0u) carry = 0u;
-    __syncthreads();
-    const uint32_t w = threadIdx.x / kWave;
-    for (uint32_t c0 = 0; c0 < nb; c0 += kSeqBlock) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t v = i < nb ? sums[i] : 0u;
-        const uint32_t incl = wave_inclusive_scan(v);
-        if (lane() == kWave - 1) s_w[w] = incl;
-        __syncthreads();
-        uint32_t woff = 0u;
-        for (uint32_t q = 0; q < w; ++q) woff += s_w[q];
-        const uint32_t ex = carry + woff + incl - v;
-        if (i < nb) sums[i] = ex;
-        __syncthreads();
-        if (threadIdx.x == kSeqBlock - 1u) carry = ex + v;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0u) {
-        *total = carry;
-        *total_copy = carry;
-    }
-}
-
-__global__ __launch_bounds__(kAmbBlock) void k_scan_add(uint32_t* out, const uint32_t* sums, uint32_t n) {
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-    const uint32_t add = sums[blockIdx.x];
-#pragma unroll
-    for (uint32_t k = 0; k < kScanPer; ++k) {
-        if (base + k < n) out[base + k] += add;
     }
 }
 
@@ -516,13 +453,7 @@ void setup(ckf_resolver& R, tamd::amb_params& P, uint32_t n_entries) {
     hip_check(hipMemsetAsync(P.best_of, 0xff, 4u * size_t(B), s), "memset");
     hipLaunchKernelGGL(k_amb_count, dim3(grid_for(P.n_tracks)), dim3(kAmbBlock), 0, s, P);
     hip_check(hipGetLastError(), "count");
-    const uint32_t nb = (B + kScanTile - 1u) / kScanTile;
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kAmbBlock), 0, s, P.n_acc, P.list_off,
-                       R.sums.as<uint32_t>(), B);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSeqBlock), 0, s, R.sums.as<uint32_t>(), nb,
-                       P.list_off + B, &P.ctl->scan_total);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kAmbBlock), 0, s, P.list_off, R.sums.as<uint32_t>(), B);
-    hip_check(hipGetLastError(), "scan");
+    exclusive_scan(P.n_acc, P.list_off, R.sums.as<uint32_t>(), B, &P.ctl->scan_total, s);
     read_ctl(R);
     if (R.h_ctl->bad_input & 1u) {
         throw ckf_error(CKF_ERR_INVALID_ARGUMENT, "constituent offsets decrease or pass offsets[n_tracks]");
@@ -578,7 +509,7 @@ void resolve_device(ckf_resolver& R, uint32_t n, const uint32_t* d_off, const ui
     R.best_of.ensure(4u * size_t(B));
     R.hot0.ensure(4u * size_t(B));
     R.hot1.ensure(4u * size_t(B));
-    R.sums.ensure(4u * size_t((B + kScanTile - 1u) / kScanTile + 1u));
+    R.sums.ensure(4u * size_t(scan_sums_size(B)));
     R.ctl.ensure(sizeof(amb_ctl));
     if (R.h_ctl == nullptr) {
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&R.h_ctl), sizeof(amb_ctl)), "hipHostMalloc");

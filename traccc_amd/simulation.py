@@ -301,6 +301,18 @@ def generate_event(det: DetectorData, fld: FieldData, spec: GunSpec) -> Event:
 # ---------------------------------------------------------------------------
 # Workload definitions
 # ---------------------------------------------------------------------------
+def with_seeds(ev: Event, surface_link, vector, covariance, particle) -> Event:
+    """A copy of ev whose seeds are the given bound parameters (surface
+    identifiers, [n, 6] vectors, [n, 6, 6] covariances) and their particles."""
+    import dataclasses
+    n = len(surface_link)
+    return dataclasses.replace(
+        ev, seed_surface_link=np.ascontiguousarray(surface_link, np.uint64),
+        seed_vector=np.ascontiguousarray(np.asarray(vector, np.float32).reshape(n, 6)),
+        seed_covariance=np.ascontiguousarray(np.asarray(covariance, np.float32).reshape(n, 6, 6)),
+        seed_particle=np.ascontiguousarray(particle, np.uint32), _keep=[])
+
+
 def workload(name: str, seed: int = 42, n_particles: int | None = None):
     """(detector, field, gun spec) of a named workload.
 
