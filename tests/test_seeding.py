@@ -230,14 +230,17 @@ def test_gpu_track_params(gpu):
 @pytest.mark.gpu
 def test_gpu_seeds_drive_the_ckf(gpu):
     """Spacepoints -> seeds -> parameters -> the CKF: the reconstructed
-    seeds find most particles, as the truth seeds do."""
+    seeds find most particles, as the truth seeds do.  The spacepoints carry
+    the hits' 50 um resolution as variances (the formation's own are zero,
+    as the reference's, which leaves smeared stiff tracks unseeded)."""
     from traccc_amd import ckf
     from traccc_amd import performance as PERF
     from traccc_amd import simulation as S
     det, fld, ev = _event("C2", 1000, 23, eta_min=-1.0, eta_max=1.0, pt_a=1.0)
     xyz, midx = _spacepoints(det, ev)
     finder, grid, filt = _c2_configs()
-    seeds = SD.seeding_algorithm(finder, grid, filt)(xyz)
+    var = np.full(len(xyz), 0.05 ** 2, np.float32)
+    seeds = SD.seeding_algorithm(finder, grid, filt)(xyz, var, var)
     sl, vec, cov = SD.track_params_estimation()(xyz, midx, seeds, ev, (0.0, 0.0, 2.0 * SD.TESLA))
     ev2 = S.with_seeds(ev, sl, vec, cov, ev.meas_particle[midx[seeds.middle]])
     cfg = S.finding_config_for("C2")
