@@ -514,8 +514,9 @@ def main() -> int:
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-seeding", action="store_true")
     ap.add_argument("--scorer-epochs", type=int, default=3)
-    ap.add_argument("--scorer-batch", type=int, default=16384, help="samples per Adam step")
-    ap.add_argument("--scorer-lr", type=float, default=2e-3)
+    ap.add_argument("--scorer-batch", type=int, default=65536,
+                    help="samples per Adam step (1,024 64-sample tiles: one wave per SIMD)")
+    ap.add_argument("--scorer-lr", type=float, default=4e-3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
