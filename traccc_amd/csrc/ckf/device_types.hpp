@@ -116,6 +116,8 @@ struct ckf_params {
     const ckf_surface* sfs;
     const ckf_layer* layers;
     uint32_t n_layers;
+    const layer_group* groups;  // build_layer_groups of the layers
+    uint32_t n_groups;
     uint32_t n_surfaces;
     field_view<float> field;
     nav_config<float> nav;

@@ -13,6 +13,7 @@
 
 #include "../common/errors.hpp"
 #include "traccc_amd/ckf.h"
+#include "../core/geometry.hpp"
 
 namespace tamd {
 
@@ -120,9 +121,13 @@ struct ckf_device_detector {
     ckf_bfield field{};
     ckf_surface* d_sfs = nullptr;
     ckf_layer* d_layers = nullptr;
+    // the navigator's layer groups (tamd::build_layer_groups)
+    std::vector<tamd::layer_group> h_groups;
+    tamd::layer_group* d_groups = nullptr;
     float* d_map = nullptr;
     ~ckf_device_detector() {
         if (d_sfs) (void)hipFree(d_sfs);
+        if (d_groups) (void)hipFree(d_groups);
         if (d_layers) (void)hipFree(d_layers);
         if (d_map) (void)hipFree(d_map);
     }

@@ -1109,12 +1109,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     if (ctl->done) {
         return;
     }
-    // the navigation layer table lives in LDS for the whole kernel
-    __shared__ ckf_layer s_layers[64];
-    for (uint32_t k = threadIdx.x; k < P.n_layers; k += blockDim.x) {
-        s_layers[k] = P.layers[k];
-    }
-    __syncthreads();
+    // the navigation layers and groups live in LDS for the whole kernel
+    __shared__ nav_tables s_nav;
+    stage_nav_tables(s_nav, P.layers, P.n_layers, P.groups, P.n_groups, P.nav.mask_tol_max);
     const uint32_t step = ctl->step;
     const uint32_t n = links_this_step(P, ctl);
     const uint32_t base = ctl->link_base;
@@ -1152,19 +1149,35 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
                 if (link_skipped(L) > P.max_skip || link_consec(L) > P.max_consec) {
                     ntip = 1u;
                 } else {
-                    float x[6], c[21];
+                    // the covariance is read after the navigation, which does
+                    // not need it: fewer registers live while navigating
+                    float x[6];
                     uint32_t sf = 0u;
-                    load_ref(P, L.state_ref, in_par, x, c, sf);
-                    if (step == 0u) {
-                        add_material(P, sf, x, c);
+                    {
+                        float c_later[21];
+                        load_ref(P, L.state_ref, in_par, x, c_later, sf);
                     }
-                    uint32_t n_rk = 0u;
-                    float leg = 0.f;
                     propagated = true;
-                    const int status = transport_state<kT>(P, s_layers, P.nav, x, c, sf, n_rk, leg, tl);
+                    const geo_view G{P.sfs, P.n_layers, P.field};
+                    navigator<float, true, kT> nav =
+                        make_navigator<kT>(G, s_nav.layers, P.nav, tl, &s_nav, P.n_groups);
+                    bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
+                    uint32_t end_sf = 0u;
+                    const int status = nav.propagate(static_cast<int>(sf), end_sf);
                     const bool success = (status == NAV_FOUND);
                     bool ok = success;
+                    float c[21];
                     if (success) {
+                        {
+                            float x_again[6];
+                            uint32_t sf_again = 0u;
+                            load_ref(P, L.state_ref, in_par, x_again, c, sf_again);
+                        }
+                        if (step == 0u) {
+                            add_material(P, sf, x, c);
+                        }
+                        finish_transport<kT>(G, P.nav, nav, end_sf, x, c);
+                        sf = end_sf;
                         add_material(P, sf, x, c);
                         const float th = x[e_theta];
                         if (th <= 0.f || th >= 6.28318530717958647692f) ok = false;
@@ -1621,6 +1634,8 @@ __global__ void k_debug_propagate(ckf_params P, uint32_t n, const uint32_t* star
                                   const float* cov, int apply_start_material, int32_t* status,
                                   uint32_t* end_surface, float* out_vec, float* out_cov, float* path,
                                   uint32_t* n_steps) {
+    __shared__ nav_tables s_nav;
+    stage_nav_tables(s_nav, P.layers, P.n_layers, P.groups, P.n_groups, P.nav.mask_tol_max);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) {
         return;
@@ -1638,8 +1653,12 @@ __global__ void k_debug_propagate(ckf_params P, uint32_t n, const uint32_t* star
     }
     navigator<float, true> nav;
     nav.sfs = P.sfs;
-    nav.layers = P.layers;
+    nav.layers = s_nav.layers;
     nav.n_layers = P.n_layers;
+    nav.groups = s_nav.groups;
+    nav.n_groups = P.n_groups;
+    nav.layer_boxes = s_nav.boxes;
+    nav.group_boxes = s_nav.boxes + 64;
     nav.field = P.field;
     nav.cfg = P.nav;
     bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);

@@ -35,7 +35,7 @@ __device__ inline bool momentum_above(const ckf_params& P, const float* x, float
 }
 
 // One seed's walk through the detector.
-__device__ inline void walk_seed(const ckf_params& P, const pkf_params& Q, const ckf_layer* s_layers,
+__device__ inline void walk_seed(const ckf_params& P, const pkf_params& Q, const nav_tables& s_nav,
                                  const nav_config<float>& leg_cfg, uint32_t s) {
     const float eps = FLT_EPSILON;
     float x[6], c[21];
@@ -97,7 +97,8 @@ __device__ inline void walk_seed(const ckf_params& P, const pkf_params& Q, const
             }
             uint32_t n_rk = 0u;
             float leg = 0.f;
-            const int status = transport_state(P, s_layers, leg_cfg, x, c, sf, n_rk, leg);
+            const int status = transport_state(P, s_nav.layers, leg_cfg, x, c, sf, n_rk, leg, 0u,
+                                               &s_nav, P.n_groups);
             path += leg;
             if (status == NAV_EXIT) {
                 good = true;  // left the detector
@@ -117,12 +118,9 @@ __device__ inline void walk_seed(const ckf_params& P, const pkf_params& Q, const
 }
 
 __global__ __launch_bounds__(kBlock) void k_pkf(ckf_params P, pkf_params Q) {
-    // the navigation layer table lives in LDS for the whole kernel
-    __shared__ ckf_layer s_layers[64];
-    for (uint32_t k = threadIdx.x; k < P.n_layers; k += blockDim.x) {
-        s_layers[k] = P.layers[k];
-    }
-    __syncthreads();
+    // the navigation layers and groups live in LDS for the whole kernel
+    __shared__ nav_tables s_nav;
+    stage_nav_tables(s_nav, P.layers, P.n_layers, P.groups, P.n_groups, P.nav.mask_tol_max);
     step_ctl* ctl = P.ctl;
     // legs between surfaces: no minimum step, no per-leg step budget (the
     // progressive filter's propagation has no ckf_aborter)
@@ -138,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_pkf(ckf_params P, pkf_params Q) {
         }
         const uint32_t s = p * kWave + lane();
         if (s < n) {
-            walk_seed(P, Q, s_layers, leg_cfg, s);
+            walk_seed(P, Q, s_nav, leg_cfg, s);
         }
     }
 }

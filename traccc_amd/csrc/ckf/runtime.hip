@@ -243,6 +243,8 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.sfs = D.d_sfs;
     P.layers = D.d_layers;
     P.n_layers = static_cast<uint32_t>(D.h_layers.size());
+    P.groups = D.d_groups;
+    P.n_groups = static_cast<uint32_t>(D.h_groups.size());
     P.n_surfaces = static_cast<uint32_t>(D.h_sfs.size());
     P.field = tamd::make_field_view<float>(D.field, D.d_map);
     P.nav = tamd::make_nav_config<float>(c);
@@ -1030,6 +1032,9 @@ ckf_device_detector* upload_detector(const ckf_detector* det, const ckf_bfield* 
         D->field = *field;
         D->d_sfs = dev_copy(det->surfaces, det->n_surfaces);
         D->d_layers = dev_copy(det->layers, det->n_layers);
+        D->h_groups.resize(tamd::kMaxGroups);
+        D->h_groups.resize(tamd::build_layer_groups(det->layers, det->n_layers, D->h_groups.data()));
+        D->d_groups = dev_copy(D->h_groups.data(), D->h_groups.size());
         if (field->kind == CKF_BFIELD_RZ_MAP) {
             D->d_map = dev_copy(field->map, size_t(2u) * field->n_r * field->n_z);
             D->field.map = D->d_map;

@@ -89,6 +89,35 @@ __device__ inline void add_material(const ckf_params& P, uint32_t sf, const floa
     add_material_sf(P.sfs[sf], P.nav.mass, P.nav.abs_q, x, c);
 }
 
+// The navigation layers, their groups and the boxes of both (for the
+// configuration's mask tolerance), staged in LDS by every block of a
+// transport kernel (64 layers and 16 groups at most).
+struct nav_tables {
+    nav_box boxes[64 + kMaxGroups];  // layers [0, n_layers), then the group slots
+    ckf_layer layers[64];
+    layer_group groups[kMaxGroups];
+    uint32_t n_layers;
+};
+
+__device__ inline void stage_nav_tables(nav_tables& t, const ckf_layer* layers, uint32_t n_layers,
+                                        const layer_group* groups, uint32_t n_groups,
+                                        float margin) {
+    for (uint32_t k = threadIdx.x; k < n_layers; k += blockDim.x) {
+        t.layers[k] = layers[k];
+    }
+    for (uint32_t k = threadIdx.x; k < n_groups; k += blockDim.x) {
+        t.groups[k] = groups[k];
+    }
+    for (uint32_t k = threadIdx.x; k < n_layers + kMaxGroups; k += blockDim.x) {
+        t.boxes[k < n_layers ? k : 64u + (k - n_layers)] =
+            nav_box_of(layers, n_layers, groups, n_groups, margin, k);
+    }
+    if (threadIdx.x == 0u) {
+        t.n_layers = n_layers;
+    }
+    __syncthreads();
+}
+
 // Geometry and field a transport needs.
 struct geo_view {
     const ckf_surface* sfs;
@@ -121,6 +150,44 @@ __device__ inline void transport_cov(const float (&Jb)[6][6], float* c) {
     }
 }
 
+// A navigator over geometry G for one transport (kT lanes per track, tl
+// this lane's index in the team), with the LDS navigation tables when given.
+template <int kT = 1>
+__device__ inline navigator<float, true, kT> make_navigator(const geo_view& G, const ckf_layer* layers,
+                                                          const nav_config<float>& cfg, uint32_t tl,
+                                                          const nav_tables* tabs,
+                                                          uint32_t n_groups) {
+    navigator<float, true, kT> nav;
+    nav.tl = tl;
+    nav.sfs = G.sfs;
+    nav.layers = layers;
+    nav.n_layers = G.n_layers;
+    if (tabs != nullptr) {
+        nav.groups = tabs->groups;
+        nav.n_groups = n_groups;
+        nav.layer_boxes = tabs->boxes;
+        nav.group_boxes = tabs->boxes + 64;
+    }
+    nav.field = G.field;
+    nav.cfg = cfg;
+    return nav;
+}
+
+// The bound state and covariance on the navigator's destination surface
+// end_sf after a NAV_FOUND propagation: x receives the bound vector, c (the
+// covariance at the start) is transported.
+template <int kT>
+__device__ inline void finish_transport(const geo_view& G, const nav_config<float>& cfg,
+                                        const navigator<float, true, kT>& nav, uint32_t end_sf,
+                                        float* x, float* c) {
+    float nx[6];
+    float Jb[6][6];
+    free_to_bound<float, true>(G.sfs[end_sf], nav.st, nav.J, G.field, cfg.mass, cfg.abs_q, nx, Jb);
+    transport_cov(Jb, c);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = nx[k];
+}
+
 // Transport (x, c) from surface sf to the next sensitive surface
 // (propagate_to_next_surface's stepper, navigator and parameter transport,
 // without the material of the destination).  On NAV_FOUND x, c and sf hold
@@ -132,14 +199,9 @@ template <int kT = 1>
 __device__ inline int transport_bound(const geo_view& G, const ckf_layer* layers,
                                       const nav_config<float>& cfg, float* x, float* c,
                                       uint32_t& sf, uint32_t& n_steps, float& path,
-                                      uint32_t tl = 0u) {
-    navigator<float, true, kT> nav;
-    nav.tl = tl;
-    nav.sfs = G.sfs;
-    nav.layers = layers;
-    nav.n_layers = G.n_layers;
-    nav.field = G.field;
-    nav.cfg = cfg;
+                                      uint32_t tl = 0u, const nav_tables* tabs = nullptr,
+                                      uint32_t n_groups = 0u) {
+    navigator<float, true, kT> nav = make_navigator<kT>(G, layers, cfg, tl, tabs, n_groups);
     bound_to_free<float, true>(G.sfs[sf], x, nav.st, nav.J);
     uint32_t end_sf = 0u;
     const int status = nav.propagate(static_cast<int>(sf), end_sf);
@@ -148,12 +210,7 @@ __device__ inline int transport_bound(const geo_view& G, const ckf_layer* layers
     if (status != NAV_FOUND) {
         return status;
     }
-    float nx[6];
-    float Jb[6][6];
-    free_to_bound<float, true>(G.sfs[end_sf], nav.st, nav.J, G.field, cfg.mass, cfg.abs_q, nx, Jb);
-    transport_cov(Jb, c);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) x[k] = nx[k];
+    finish_transport<kT>(G, cfg, nav, end_sf, x, c);
     sf = end_sf;
     return NAV_FOUND;
 }
@@ -162,9 +219,10 @@ template <int kT = 1>
 __device__ inline int transport_state(const ckf_params& P, const ckf_layer* layers,
                                       const nav_config<float>& cfg, float* x, float* c,
                                       uint32_t& sf, uint32_t& n_steps, float& path,
-                                      uint32_t tl = 0u) {
+                                      uint32_t tl = 0u, const nav_tables* tabs = nullptr,
+                                      uint32_t n_groups = 0u) {
     const geo_view G{P.sfs, P.n_layers, P.field};
-    return transport_bound<kT>(G, layers, cfg, x, c, sf, n_steps, path, tl);
+    return transport_bound<kT>(G, layers, cfg, x, c, sf, n_steps, path, tl, tabs, n_groups);
 }
 
 // Transport (x, c) from surface sf straight onto surface `target` (forwards

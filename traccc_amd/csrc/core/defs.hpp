@@ -14,10 +14,12 @@
 // full unrolling keeps small matrices in registers (a runtime index into a
 // register array sends it to scratch memory)
 #define TAMD_UNROLL _Pragma("unroll")
+#define TAMD_NO_UNROLL _Pragma("unroll 1")
 #else
 #define TAMD_HD inline
 #define TAMD_DEV inline
 #define TAMD_UNROLL
+#define TAMD_NO_UNROLL
 #endif
 
 #include "traccc_amd/ckf.h"

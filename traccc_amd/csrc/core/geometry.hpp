@@ -50,41 +50,14 @@ TAMD_HD bool inside_bounds(const ckf_surface& s, T l0, T l1, T tol) {
     return std::fabs(l0) <= half + tol;
 }
 
-// Straight-line distance along d at which the line enters the layer's (r, z)
-// envelope; 0 when p is already inside, +inf when the line never enters.
-// Crossing points are accepted within `margin` of the envelope's other extent.
+// Straight-line distance along d at which the line enters the (r, z)
+// envelope [r0, r1] x [z0, z1]; 0 when p is already inside, +inf when the line
+// never enters.  Crossing points are accepted within `margin` of the
+// envelope's other extent.
 template <typename T>
-TAMD_HD T layer_entry_distance(const ckf_layer& L, const vec3<T>& p,
-                               const vec3<T>& d, T margin) {
+TAMD_HD T envelope_entry_distance(T r0, T r1, T z0, T z1, const vec3<T>& p, const vec3<T>& d,
+                                  T margin) {
     const T inf = static_cast<T>(1e30);
-    if (L.kind == CKF_LAYER_PLANE) {
-        const vec3<T> n{static_cast<T>(L.normal[0]), static_cast<T>(L.normal[1]),
-                        static_cast<T>(L.normal[2])};
-        const T d0 = static_cast<T>(L.env_r[0]);
-        const T d1 = static_cast<T>(L.env_r[1]);
-        const T dp = dot(n, p);
-        if (dp >= d0 && dp <= d1) {
-            return T(0);
-        }
-        const T dn = dot(n, d);
-        if (std::fabs(dn) < static_cast<T>(1e-12)) {
-            return inf;
-        }
-        const T s0 = (d0 - dp) / dn;
-        const T s1 = (d1 - dp) / dn;
-        T best = inf;
-        if (s0 > T(0)) {
-            best = s0;
-        }
-        if (s1 > T(0) && s1 < best) {
-            best = s1;
-        }
-        return best;
-    }
-    const T r0 = static_cast<T>(L.env_r[0]);
-    const T r1 = static_cast<T>(L.env_r[1]);
-    const T z0 = static_cast<T>(L.env_z[0]);
-    const T z1 = static_cast<T>(L.env_z[1]);
     // Envelopes the line cannot reach, before any division: beyond the
     // widened z range and not moving towards it (z(s) only moves away, every
     // wall crossing then fails its z test and both z faces lie behind), or
@@ -144,6 +117,41 @@ TAMD_HD T layer_entry_distance(const ckf_layer& L, const vec3<T>& p,
     return best;
 }
 
+// Straight-line distance along d at which the line enters the layer's
+// envelope (envelope_entry_distance; a plane layer: its slab).
+template <typename T>
+TAMD_HD T layer_entry_distance(const ckf_layer& L, const vec3<T>& p,
+                               const vec3<T>& d, T margin) {
+    const T inf = static_cast<T>(1e30);
+    if (L.kind == CKF_LAYER_PLANE) {
+        const vec3<T> n{static_cast<T>(L.normal[0]), static_cast<T>(L.normal[1]),
+                        static_cast<T>(L.normal[2])};
+        const T d0 = static_cast<T>(L.env_r[0]);
+        const T d1 = static_cast<T>(L.env_r[1]);
+        const T dp = dot(n, p);
+        if (dp >= d0 && dp <= d1) {
+            return T(0);
+        }
+        const T dn = dot(n, d);
+        if (std::fabs(dn) < static_cast<T>(1e-12)) {
+            return inf;
+        }
+        const T s0 = (d0 - dp) / dn;
+        const T s1 = (d1 - dp) / dn;
+        T best = inf;
+        if (s0 > T(0)) {
+            best = s0;
+        }
+        if (s1 > T(0) && s1 < best) {
+            best = s1;
+        }
+        return best;
+    }
+    return envelope_entry_distance<T>(static_cast<T>(L.env_r[0]), static_cast<T>(L.env_r[1]),
+                                      static_cast<T>(L.env_z[0]), static_cast<T>(L.env_z[1]), p,
+                                      d, margin);
+}
+
 // A lower bound on layer_entry_distance(L, p, d, margin) for an (r, z)
 // envelope, from the slab the line must reach first: an accepted crossing
 // lies within the widened z range and at a radius of at least r0 - margin,
@@ -171,6 +179,152 @@ TAMD_HD T layer_entry_lower_bound(const ckf_layer& L, const vec3<T>& p, const ve
         lb = lr > lb ? lr : lb;
     }
     return lb;
+}
+
+// Navigation layer groups: up to four layers under one (r, z) box that
+// contains all their envelopes, so that a single bound can rule out every
+// member.  Plane layers go in groups without a box (unbounded).
+constexpr uint32_t kGroupSize = 4u;
+constexpr uint32_t kMaxGroups = 16u;  // 64 layers
+
+struct layer_group {
+    float env_r[2];  // union of the members' (r, z) envelopes
+    float env_z[2];
+    uint32_t bounded;  // 0: plane layers, no box
+    uint32_t n;
+    uint32_t member[kGroupSize];
+};
+
+// A layer's or a group's (r, z) box with the margin-dependent terms of its
+// lower bound, for one navigation configuration: 32 B, two 16-byte loads.
+enum nav_box_kind : uint32_t { kBoxEnvelope = 0u, kBoxNone = 1u, kBoxEmpty = 2u };
+
+struct alignas(16) nav_box {
+    float r0, r1, z0, z1;  // the envelope (a group: the union of its members')
+    float rlo2;            // (r0 - margin)^2, 0 when r0 <= margin
+    float inv2rlo;         // 1 / (2 (r0 - margin)), 0 when r0 <= margin
+    float r1m2;            // (r1 + margin)^2
+    uint32_t kind;         // kBoxEnvelope; kBoxNone (plane layers: bound 0); kBoxEmpty
+};
+
+TAMD_HD nav_box make_nav_box(float r0, float r1, float z0, float z1, uint32_t kind, float margin) {
+    nav_box B;
+    B.r0 = r0;
+    B.r1 = r1;
+    B.z0 = z0;
+    B.z1 = z1;
+    const float rlo = r0 - margin;
+    B.rlo2 = rlo > 0.f ? rlo * rlo : 0.f;
+    B.inv2rlo = rlo > 0.f ? 1.f / (2.f * rlo) : 0.f;
+    B.r1m2 = (r1 + margin) * (r1 + margin);
+    B.kind = kind;
+    return B;
+}
+
+// A lower bound on envelope_entry_distance over every envelope inside box B
+// (the line from p along d, rr = |p_T|^2, hb0 = p_T . d_T, inv_dz = 1 / d.z),
+// without branches: 1e30 when the line cannot reach the widened box (beyond
+// its z range and not moving towards it -- the members' own distances are
+// +inf by the same comparisons -- or outside its outer radius, with a
+// relative safety margin, and not moving inwards); otherwise the larger of
+// the z-slab bound (the z distance to the widened range over |d.z|) and the
+// radial bound (a line covers at least (rlo^2 - r^2) / (2 rlo) of path to
+// reach radius rlo from r < rlo).  A larger box gives a smaller bound.
+// kBoxNone: 0; kBoxEmpty: 1e30.
+template <typename T>
+TAMD_HD T nav_box_bound(const nav_box& B, const vec3<T>& p, const vec3<T>& d, T rr, T hb0, T margin,
+                        T inv_dz) {
+    const T zlo = static_cast<T>(B.z0) - margin;
+    const T zhi = static_cast<T>(B.z1) + margin;
+    const bool below = p.z < zlo;
+    const bool above = p.z > zhi;
+    const bool gone = (above && d.z >= T(0)) || (below && d.z <= T(0)) ||
+                      (rr > static_cast<T>(B.r1m2) * static_cast<T>(1.000001f) && hb0 >= T(0));
+    const T lz = below ? (zlo - p.z) * inv_dz : (above ? (zhi - p.z) * inv_dz : T(0));
+    const T rl2 = static_cast<T>(B.rlo2);
+    const T lr = rr < rl2 ? (rl2 - rr) * static_cast<T>(B.inv2rlo) : T(0);
+    T lb = lz > lr ? lz : lr;
+    lb = gone ? static_cast<T>(1e30) : lb;
+    lb = B.kind == kBoxNone ? T(0) : lb;
+    return B.kind == kBoxEmpty ? static_cast<T>(1e30) : lb;
+}
+
+// The boxes of the layers ([n_layers]) and of the groups (kMaxGroups slots,
+// kBoxEmpty past n_groups) for a margin; one record per index k < n_layers +
+// kMaxGroups (layers first), so a block of threads can fill them together.
+TAMD_HD nav_box nav_box_of(const ckf_layer* layers, uint32_t n_layers, const layer_group* groups,
+                           uint32_t n_groups, float margin, uint32_t k) {
+    if (k < n_layers) {
+        const ckf_layer& L = layers[k];
+        return make_nav_box(L.env_r[0], L.env_r[1], L.env_z[0], L.env_z[1],
+                            L.kind == CKF_LAYER_PLANE ? kBoxNone : kBoxEnvelope, margin);
+    }
+    const uint32_t g = k - n_layers;
+    if (g >= n_groups) {
+        return make_nav_box(0.f, 0.f, 0.f, 0.f, kBoxEmpty, margin);
+    }
+    const layer_group& G = groups[g];
+    return make_nav_box(G.env_r[0], G.env_r[1], G.env_z[0], G.env_z[1],
+                        G.bounded != 0u ? kBoxEnvelope : kBoxNone, margin);
+}
+
+// Host: group the layers for the navigator.  (r, z) envelope layers are
+// ordered by (kind, z centre, r centre) and cut into runs of kGroupSize, so a
+// group holds neighbours (a barrel's consecutive radii, a disk's rings);
+// plane layers follow in runs of their own.  Returns the number of groups
+// (n <= 64 layers).
+inline uint32_t build_layer_groups(const ckf_layer* L, uint32_t n, layer_group* out) {
+    uint32_t order[64];
+    uint32_t m = 0u;
+    for (uint32_t i = 0; i < n && i < 64u; ++i) {
+        if (L[i].kind != CKF_LAYER_PLANE) order[m++] = i;
+    }
+    const uint32_t n_box = m;
+    for (uint32_t i = 0; i < n && i < 64u; ++i) {
+        if (L[i].kind == CKF_LAYER_PLANE) order[m++] = i;
+    }
+    // insertion sort of the envelope layers (at most 64)
+    auto key_less = [&](uint32_t a, uint32_t b) {
+        if (L[a].kind != L[b].kind) return L[a].kind < L[b].kind;
+        const float za = L[a].env_z[0] + L[a].env_z[1], zb = L[b].env_z[0] + L[b].env_z[1];
+        if (za != zb) return za < zb;
+        const float ra = L[a].env_r[0] + L[a].env_r[1], rb = L[b].env_r[0] + L[b].env_r[1];
+        if (ra != rb) return ra < rb;
+        return a < b;
+    };
+    for (uint32_t i = 1; i < n_box; ++i) {
+        const uint32_t v = order[i];
+        uint32_t j = i;
+        while (j > 0u && key_less(v, order[j - 1u])) {
+            order[j] = order[j - 1u];
+            --j;
+        }
+        order[j] = v;
+    }
+    uint32_t g = 0u;
+    for (uint32_t start = 0; start < m;) {
+        // a group never mixes envelope and plane layers
+        const uint32_t end_run = start < n_box ? n_box : m;
+        const uint32_t stop = start + kGroupSize < end_run ? start + kGroupSize : end_run;
+        layer_group G{};
+        G.bounded = start < n_box ? 1u : 0u;
+        G.n = stop - start;
+        G.env_r[0] = 1e30f;
+        G.env_r[1] = -1e30f;
+        G.env_z[0] = 1e30f;
+        G.env_z[1] = -1e30f;
+        for (uint32_t k = 0; k < G.n; ++k) {
+            const ckf_layer& l = L[order[start + k]];
+            G.member[k] = order[start + k];
+            G.env_r[0] = l.env_r[0] < G.env_r[0] ? l.env_r[0] : G.env_r[0];
+            G.env_r[1] = l.env_r[1] > G.env_r[1] ? l.env_r[1] : G.env_r[1];
+            G.env_z[0] = l.env_z[0] < G.env_z[0] ? l.env_z[0] : G.env_z[0];
+            G.env_z[1] = l.env_z[1] > G.env_z[1] ? l.env_z[1] : G.env_z[1];
+        }
+        out[g++] = G;
+        start = stop;
+    }
+    return g;
 }
 
 // Module grid bin of a global point: phi bin (wrapped) and long bin (clamped).

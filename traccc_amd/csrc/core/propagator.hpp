@@ -359,6 +359,13 @@ struct navigator {
     field_view<T> field;
     nav_config<T> cfg;
     uint32_t tl = 0u;
+    // layer groups (build_layer_groups) and the boxes of the layers and the
+    // groups (nav_box_of, this configuration's margin) for the one-lane layer
+    // search; none: every layer is scanned
+    const layer_group* groups = nullptr;
+    uint32_t n_groups = 0u;
+    const nav_box* layer_boxes = nullptr;
+    const nav_box* group_boxes = nullptr;  // kMaxGroups slots
 
     free_state<T> st;
     transport_jac<T> J;
@@ -486,33 +493,226 @@ TAMD_UNROLL
         }
     }
 
+    // One pass of the step onto the plane of `S`: the signed straight-line
+    // distance to the plane decides.  2: plane parallel to the track; 0: within
+    // on_surface_tol, h = that distance (the last, sub-tolerance step, none
+    // when it is 0); 1: further away, h = the next RK step towards it.
+    TAMD_HD int plane_step(const ckf_surface& S, T& h) const {
+        const vec3<T> w = sf_w<T>(S);
+        const T tw = dot(st.dir, w);
+        if (std::fabs(tw) < static_cast<T>(1e-8)) {
+            return 2;
+        }
+        const T sd = -dot(rel_to(st, sf_center<T>(S)), w) / tw;
+        if (std::fabs(sd) <= cfg.on_surface_tol) {
+            h = sd;
+            return 0;
+        }
+        const T hl = step_limit();
+        h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
+        return 1;
+    }
+
     // Step onto the plane of `S`: RK steps until |distance| <= on_surface_tol,
     // then one last (sub-tolerance) RK step of that distance, which keeps the
     // direction, time and transport consistent with the position.
     // 0: on the plane, 1: abort, 2: plane parallel to the track.
     TAMD_HD int step_to_plane(const ckf_surface& S) {
-        const vec3<T> c = sf_center<T>(S);
-        const vec3<T> w = sf_w<T>(S);
-        T sd = T(0);
         for (;;) {
-            const T tw = dot(st.dir, w);
-            if (std::fabs(tw) < static_cast<T>(1e-8)) {
+            T h = T(0);
+            const int r = plane_step(S, h);
+            if (r == 2) {
                 return 2;
             }
-            sd = -dot(rel_to(st, c), w) / tw;
-            if (std::fabs(sd) <= cfg.on_surface_tol) {
-                break;
+            if (r == 0) {
+                return (h != T(0) && !do_step(h)) ? 1 : 0;
             }
-            const T hl = step_limit();
-            const T h = sd > T(0) ? (sd < hl ? sd : hl) : (sd > -hl ? sd : -hl);
             if (!do_step(h)) {
                 return 1;
             }
         }
-        if (sd != T(0) && !do_step(sd)) {
-            return 1;
+    }
+
+    // The nearest candidate not yet tried (lower module index on ties), over
+    // the team; marks it tried.  false when none is left.
+    TAMD_HD bool next_candidate(const T* cs, const uint32_t* cm, uint32_t& used,
+                                uint32_t& bm) const {
+        const T inf = static_cast<T>(1e30);
+        T best = inf;
+        bm = 0xffffffffu;
+TAMD_UNROLL
+        for (int j = 0; j < kSlotsPerLane; ++j) {
+            const bool avail = ((used >> j) & 1u) == 0u && cs[j] < inf;
+            if (avail && (cs[j] < best || (cs[j] == best && cm[j] < bm))) {
+                best = cs[j];
+                bm = cm[j];
+            }
         }
-        return 0;
+        team_min(best, bm);
+        if (!(best < inf)) {
+            return false;
+        }
+TAMD_UNROLL
+        for (int j = 0; j < kSlotsPerLane; ++j) {
+            if (cs[j] < inf && cm[j] == bm) {
+                used |= (1u << j);
+            }
+        }
+        return true;
+    }
+
+    // Whether the straight line from the current state still meets the plane
+    // of `S` ahead (at least overstep_tol in front).
+    TAMD_HD bool plane_ahead(const ckf_surface& S) const {
+        const vec3<T> w = sf_w<T>(S);
+        const T tw = dot(st.dir, w);
+        return !(std::fabs(tw) < static_cast<T>(1e-8) ||
+                 dot(sf_center<T>(S) - st.pos, w) / tw < cfg.overstep_tol);
+    }
+
+    // On the plane of `S`: inside its exact mask, past the minimum path.
+    TAMD_HD bool accept(const ckf_surface& S) const {
+        const vec3<T> x = rel_to(st, sf_center<T>(S));
+        return inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
+               path > cfg.min_step_length;
+    }
+
+    // The unvisited layer whose (r, z) envelope the straight line from the
+    // current state enters first (lowest index on ties, over the team):
+    // best = -1 when none; bs = its entry distance.
+    TAMD_HD void pick_layer(uint64_t visited, int& best, T& bs) const {
+        best = -1;
+        bs = static_cast<T>(1e30);
+        const T inv_dz = T(1) / st.dir.z;
+        if constexpr (kT == 1) {
+            if (groups != nullptr && layer_boxes != nullptr && group_boxes != nullptr) {
+                pick_layer_grouped(visited, inv_dz, best, bs);
+                return;
+            }
+        }
+        for (uint32_t li = tl; li < n_layers; li += kT) {
+            if (visited & (1ull << li)) {
+                continue;
+            }
+            // a layer whose entry cannot come before the best one so far
+            // is not evaluated (10 um of slack covers the bound's rounding)
+            if (layer_entry_lower_bound(layers[li], st.pos, st.dir, cfg.mask_tol_max, inv_dz) >
+                bs + static_cast<T>(1e-2)) {
+                continue;
+            }
+            const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
+            if (s < bs) {
+                bs = s;
+                best = static_cast<int>(li);
+            }
+        }
+        uint32_t key = best < 0 ? 0xffffffffu : static_cast<uint32_t>(best);
+        team_min(bs, key);
+        best = key == 0xffffffffu ? -1 : static_cast<int>(key);
+    }
+
+    // Entry distance of layer li from the current state (layer_entry_distance;
+    // from the layer's box when there is one, the same arithmetic).
+    TAMD_HD T entry_distance(uint32_t li) const {
+        if (layer_boxes != nullptr) {
+            const nav_box B = layer_boxes[li];
+            if (B.kind == kBoxEnvelope) {
+                return envelope_entry_distance<T>(static_cast<T>(B.r0), static_cast<T>(B.r1),
+                                                  static_cast<T>(B.z0), static_cast<T>(B.z1),
+                                                  st.pos, st.dir, cfg.mask_tol_max);
+            }
+        }
+        return layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
+    }
+
+    // pick_layer over layer groups, best first: the bound of every group (four
+    // boxes loaded at a time, no branches), then rounds that take the group
+    // with the smallest bound left and evaluate its members -- each with its own
+    // box bound and, when that does not rule it out, its entry distance; the
+    // search ends when the smallest bound left is past the best distance.  A
+    // group or layer is only passed over when its bound exceeds the best
+    // distance (with the scan's 10 um of slack) or it cannot be reached, and
+    // equal distances go to the lower layer index, so the result is the
+    // scan's.
+    TAMD_HD void pick_layer_grouped(uint64_t visited, T inv_dz, int& best, T& bs) const {
+        const T big = static_cast<T>(1e29);
+        const T slack = static_cast<T>(1e-2);
+        const T mt = cfg.mask_tol_max;
+        const T rr = st.pos.x * st.pos.x + st.pos.y * st.pos.y;
+        const T hb0 = st.pos.x * st.dir.x + st.pos.y * st.dir.y;
+        // every group's bound, once (four boxes loaded at a time)
+        T gb[kMaxGroups];
+TAMD_UNROLL
+        for (uint32_t g0 = 0; g0 < kMaxGroups; g0 += 4u) {
+            nav_box bx[4];
+TAMD_UNROLL
+            for (uint32_t j = 0; j < 4u; ++j) {
+                bx[j] = group_boxes[g0 + j];  // slots past n_groups: kBoxEmpty
+            }
+TAMD_UNROLL
+            for (uint32_t j = 0; j < 4u; ++j) {
+                gb[g0 + j] = nav_box_bound<T>(bx[j], st.pos, st.dir, rr, hb0, mt, inv_dz);
+            }
+        }
+        for (uint32_t round = 0; round < n_groups; ++round) {
+            // the smallest and second smallest bound of the groups left
+            T m1 = big, m2 = big;
+            uint32_t g1 = kMaxGroups;
+TAMD_UNROLL
+            for (uint32_t g = 0; g < kMaxGroups; ++g) {
+                const T b = gb[g];
+                const bool lt1 = b < m1;
+                const bool lt2 = b < m2;
+                m2 = lt1 ? m1 : (lt2 ? b : m2);
+                g1 = lt1 ? g : g1;
+                m1 = lt1 ? b : m1;
+            }
+            if (g1 == kMaxGroups || m1 > bs + slack) {
+                return;
+            }
+TAMD_UNROLL
+            for (uint32_t g = 0; g < kMaxGroups; ++g) {
+                gb[g] = g == g1 ? static_cast<T>(1e30) : gb[g];
+            }
+            const layer_group& G = groups[g1];
+            const uint32_t gn = G.n;
+            uint32_t mem[kGroupSize];
+            nav_box mb[kGroupSize];
+TAMD_UNROLL
+            for (uint32_t k = 0; k < kGroupSize; ++k) {
+                mem[k] = G.member[k < gn ? k : 0u];
+                mb[k] = layer_boxes[mem[k]];
+            }
+            uint32_t cand = 0u;
+TAMD_UNROLL
+            for (uint32_t k = 0; k < kGroupSize; ++k) {
+                const T lb = nav_box_bound<T>(mb[k], st.pos, st.dir, rr, hb0, mt, inv_dz);
+                const bool ok = k < gn && ((visited >> mem[k]) & 1ull) == 0ull && lb < big &&
+                                !(lb > bs + slack);
+                cand |= ok ? (1u << k) : 0u;
+            }
+            while (cand != 0u) {
+                const uint32_t k = static_cast<uint32_t>(__builtin_ctz(cand));
+                cand &= cand - 1u;
+                const uint32_t li = G.member[k];
+                const T s = entry_distance(li);
+                if (s < bs || (s == bs && static_cast<int>(li) < best)) {
+                    bs = s;
+                    best = static_cast<int>(li);
+                }
+            }
+            // every group left is bounded by m2 or more
+            if (!(m2 <= bs + slack)) {
+                return;
+            }
+        }
+    }
+
+    // Below min_p or min_pT at the start: the propagation aborts at once.
+    TAMD_HD bool below_momentum_cut() const {
+        const T p = cfg.abs_q / std::fabs(st.qop);
+        const T pT = p * std::sqrt(st.dir.x * st.dir.x + st.dir.y * st.dir.y);
+        return p < cfg.min_p || pT < cfg.min_pT;
     }
 
     // Visit the candidates of layer `li` in increasing (distance, module).
@@ -521,38 +721,15 @@ TAMD_UNROLL
         T cs[kSlotsPerLane];
         uint32_t cm[kSlotsPerLane];
         collect(li, exclude, s_known, cs, cm);
-        const T inf = static_cast<T>(1e30);
         uint32_t used = 0u;
         for (int tries = 0; tries < kMaxCandidates; ++tries) {
-            // nearest unvisited candidate, lower module index on ties
-            T best = inf;
-            uint32_t bm = 0xffffffffu;
-TAMD_UNROLL
-            for (int j = 0; j < kSlotsPerLane; ++j) {
-                const bool avail = ((used >> j) & 1u) == 0u && cs[j] < inf;
-                if (avail && (cs[j] < best || (cs[j] == best && cm[j] < bm))) {
-                    best = cs[j];
-                    bm = cm[j];
-                }
-            }
-            team_min(best, bm);
-            if (!(best < inf)) {
+            uint32_t bm = 0u;
+            if (!next_candidate(cs, cm, used, bm)) {
                 return 1;
             }
-TAMD_UNROLL
-            for (int j = 0; j < kSlotsPerLane; ++j) {
-                if (cs[j] < inf && cm[j] == bm) {
-                    used |= (1u << j);
-                }
-            }
             const ckf_surface& S = sfs[bm];
-            {
-                const vec3<T> w = sf_w<T>(S);
-                const T tw = dot(st.dir, w);
-                if (std::fabs(tw) < static_cast<T>(1e-8) ||
-                    dot(sf_center<T>(S) - st.pos, w) / tw < cfg.overstep_tol) {
-                    continue;
-                }
+            if (!plane_ahead(S)) {
+                continue;
             }
             const int r = step_to_plane(S);
             if (r == 1) {
@@ -561,9 +738,7 @@ TAMD_UNROLL
             if (r == 2) {
                 continue;
             }
-            const vec3<T> x = rel_to(st, sf_center<T>(S));
-            if (inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
-                path > cfg.min_step_length) {
+            if (accept(S)) {
                 out_sf = bm;
                 return 0;
             }
@@ -576,12 +751,8 @@ TAMD_UNROLL
     TAMD_HD int propagate(int start_sf, uint32_t& out_sf) {
         path = T(0);
         n_steps = 0u;
-        {
-            const T p = cfg.abs_q / std::fabs(st.qop);
-            const T pT = p * std::sqrt(st.dir.x * st.dir.x + st.dir.y * st.dir.y);
-            if (p < cfg.min_p || pT < cfg.min_pT) {
-                return NAV_ABORT;
-            }
+        if (below_momentum_cut()) {
+            return NAV_ABORT;
         }
         uint64_t visited = 0ull;
         if (start_sf >= 0) {
@@ -597,30 +768,8 @@ TAMD_UNROLL
         }
         for (uint32_t guard = 0; guard <= n_layers; ++guard) {
             int best = -1;
-            T bs = static_cast<T>(1e30);
-            const T inv_dz = T(1) / st.dir.z;
-            for (uint32_t li = tl; li < n_layers; li += kT) {
-                if (visited & (1ull << li)) {
-                    continue;
-                }
-                // a layer whose entry cannot come before the best one so far
-                // is not evaluated (10 um of slack covers the bound's rounding)
-                if (layer_entry_lower_bound(layers[li], st.pos, st.dir, cfg.mask_tol_max, inv_dz) >
-                    bs + static_cast<T>(1e-2)) {
-                    continue;
-                }
-                const T s = layer_entry_distance(layers[li], st.pos, st.dir, cfg.mask_tol_max);
-                if (s < bs) {
-                    bs = s;
-                    best = static_cast<int>(li);
-                }
-            }
-            {
-                // nearest layer of the team, lowest index on ties
-                uint32_t key = best < 0 ? 0xffffffffu : static_cast<uint32_t>(best);
-                team_min(bs, key);
-                best = key == 0xffffffffu ? -1 : static_cast<int>(key);
-            }
+            T bs = T(0);
+            pick_layer(visited, best, bs);
             if (best < 0 || !(bs < static_cast<T>(1e29))) {
                 return NAV_EXIT;
             }
@@ -629,7 +778,7 @@ TAMD_UNROLL
                 if (!do_step(bs < hl ? bs : hl)) {
                     return NAV_ABORT;
                 }
-                bs = layer_entry_distance(layers[best], st.pos, st.dir, cfg.mask_tol_max);
+                bs = entry_distance(static_cast<uint32_t>(best));
             }
             if (bs < static_cast<T>(1e29)) {
                 // bs is the entry distance from the current state (recomputed

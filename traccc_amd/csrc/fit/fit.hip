@@ -74,6 +74,8 @@ struct fit_params {
     const ckf_surface* sfs;
     const ckf_layer* layers;
     uint32_t n_layers;
+    const layer_group* groups;
+    uint32_t n_groups;
     field_view<float> field;
     nav_config<float> nav;
     float inflation;
@@ -287,11 +289,8 @@ __device__ inline uint32_t smooth_state(const meas_data& m, float* xp, float* cp
 }
 
 __global__ __launch_bounds__(kFitBlock) void k_fit_forward(fit_params F, uint32_t iteration) {
-    __shared__ ckf_layer s_layers[64];
-    for (uint32_t k = threadIdx.x; k < F.n_layers; k += blockDim.x) {
-        s_layers[k] = F.layers[k];
-    }
-    __syncthreads();
+    __shared__ nav_tables s_nav;
+    stage_nav_tables(s_nav, F.layers, F.n_layers, F.groups, F.n_groups, F.nav.mask_tol_max);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= F.n_tracks) {
         return;
@@ -370,7 +369,8 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_forward(fit_params F, uint32_
         }
         uint32_t n_rk = 0u;
         float leg = 0.f;
-        const int nav = transport_bound<1>(G, s_layers, F.nav, x, c, sf, n_rk, leg, 0u);
+        const int nav = transport_bound<1>(G, s_nav.layers, F.nav, x, c, sf, n_rk, leg, 0u,
+                                           &s_nav, F.n_groups);
         if (nav == NAV_EXIT) {
             status = kIncomplete;
             break;
@@ -630,6 +630,8 @@ ckf_status ckf_fit_tracks(ckf_fitter* f, const ckf_device_detector* det, const c
                 F.sfs = det->d_sfs;
                 F.layers = det->d_layers;
                 F.n_layers = static_cast<uint32_t>(det->h_layers.size());
+                F.groups = det->d_groups;
+                F.n_groups = static_cast<uint32_t>(det->h_groups.size());
                 F.field = tamd::make_field_view<float>(det->field, det->d_map);
                 F.nav = fit_nav(f->cfg);
                 F.inflation = f->cfg.covariance_inflation_factor;
