@@ -201,10 +201,9 @@ enum nav_box_kind : uint32_t { kBoxEnvelope = 0u, kBoxNone = 1u, kBoxEmpty = 2u 
 
 struct alignas(16) nav_box {
     float r0, r1, z0, z1;  // the envelope (a group: the union of its members')
-    float rlo2;            // (r0 - margin)^2, 0 when r0 <= margin
-    float inv2rlo;         // 1 / (2 (r0 - margin)), 0 when r0 <= margin
     float r1m2;            // (r1 + margin)^2
     uint32_t kind;         // kBoxEnvelope; kBoxNone (plane layers: bound 0); kBoxEmpty
+    float pad[2];
 };
 
 TAMD_HD nav_box make_nav_box(float r0, float r1, float z0, float z1, uint32_t kind, float margin) {
@@ -213,36 +212,57 @@ TAMD_HD nav_box make_nav_box(float r0, float r1, float z0, float z1, uint32_t ki
     B.r1 = r1;
     B.z0 = z0;
     B.z1 = z1;
-    const float rlo = r0 - margin;
-    B.rlo2 = rlo > 0.f ? rlo * rlo : 0.f;
-    B.inv2rlo = rlo > 0.f ? 1.f / (2.f * rlo) : 0.f;
     B.r1m2 = (r1 + margin) * (r1 + margin);
     B.kind = kind;
+    B.pad[0] = 0.f;
+    B.pad[1] = 0.f;
     return B;
 }
 
-// A lower bound on envelope_entry_distance over every envelope inside box B
-// (the line from p along d, rr = |p_T|^2, hb0 = p_T . d_T, inv_dz = 1 / d.z),
-// without branches: 1e30 when the line cannot reach the widened box (beyond
-// its z range and not moving towards it -- the members' own distances are
-// +inf by the same comparisons -- or outside its outer radius, with a
-// relative safety margin, and not moving inwards); otherwise the larger of
-// the z-slab bound (the z distance to the widened range over |d.z|) and the
-// radial bound (a line covers at least (rlo^2 - r^2) / (2 rlo) of path to
-// reach radius rlo from r < rlo).  A larger box gives a smaller bound.
-// kBoxNone: 0; kBoxEmpty: 1e30.
+// A straight line from p along the unit vector d with the terms every box
+// bound shares: rr = |p_T|^2, hb0 = p_T . d_T, r = |p_T|, inv_dt = 1 / |d_T|
+// and inv_dz = 1 / d.z (inf when zero).
 template <typename T>
-TAMD_HD T nav_box_bound(const nav_box& B, const vec3<T>& p, const vec3<T>& d, T rr, T hb0, T margin,
-                        T inv_dz) {
+struct nav_ray {
+    vec3<T> p, d;
+    T rr, hb0, r, inv_dt, inv_dz;
+};
+
+template <typename T>
+TAMD_HD nav_ray<T> make_nav_ray(const vec3<T>& p, const vec3<T>& d) {
+    nav_ray<T> q;
+    q.p = p;
+    q.d = d;
+    q.rr = p.x * p.x + p.y * p.y;
+    q.hb0 = p.x * d.x + p.y * d.y;
+    q.r = std::sqrt(q.rr);
+    q.inv_dt = T(1) / std::sqrt(d.x * d.x + d.y * d.y);
+    q.inv_dz = T(1) / d.z;
+    return q;
+}
+
+// A lower bound on envelope_entry_distance over every envelope inside box B,
+// without branches.  An accepted crossing lies in the box widened by the
+// margin: z within [z0 - m, z1 + m] and radius at least r0 - m.  The bound is
+// the larger of the z-slab bound (the z distance to the widened range over
+// |d.z|) and the radial bound (the radius grows by at most |d_T| per unit
+// path, so reaching r0 - m from r takes at least (r0 - m - r) / |d_T|); it
+// is 1e30 when the line cannot reach the widened box: beyond its z range and
+// not moving towards it, or outside its outer radius (with a relative safety
+// margin) and not moving inwards -- the members' own distances are +inf by
+// the same comparisons.  A larger box gives a smaller bound.  kBoxNone: 0;
+// kBoxEmpty: 1e30.
+template <typename T>
+TAMD_HD T nav_box_bound(const nav_box& B, const nav_ray<T>& q, T margin) {
     const T zlo = static_cast<T>(B.z0) - margin;
     const T zhi = static_cast<T>(B.z1) + margin;
-    const bool below = p.z < zlo;
-    const bool above = p.z > zhi;
-    const bool gone = (above && d.z >= T(0)) || (below && d.z <= T(0)) ||
-                      (rr > static_cast<T>(B.r1m2) * static_cast<T>(1.000001f) && hb0 >= T(0));
-    const T lz = below ? (zlo - p.z) * inv_dz : (above ? (zhi - p.z) * inv_dz : T(0));
-    const T rl2 = static_cast<T>(B.rlo2);
-    const T lr = rr < rl2 ? (rl2 - rr) * static_cast<T>(B.inv2rlo) : T(0);
+    const bool below = q.p.z < zlo;
+    const bool above = q.p.z > zhi;
+    const bool gone = (above && q.d.z >= T(0)) || (below && q.d.z <= T(0)) ||
+                      (q.rr > static_cast<T>(B.r1m2) * static_cast<T>(1.000001f) && q.hb0 >= T(0));
+    const T lz = below ? (zlo - q.p.z) * q.inv_dz : (above ? (zhi - q.p.z) * q.inv_dz : T(0));
+    const T rlo = static_cast<T>(B.r0) - margin;
+    const T lr = rlo > q.r ? (rlo - q.r) * q.inv_dt : T(0);
     T lb = lz > lr ? lz : lr;
     lb = gone ? static_cast<T>(1e30) : lb;
     lb = B.kind == kBoxNone ? T(0) : lb;
@@ -269,10 +289,15 @@ TAMD_HD nav_box nav_box_of(const ckf_layer* layers, uint32_t n_layers, const lay
 }
 
 // Host: group the layers for the navigator.  (r, z) envelope layers are
-// ordered by (kind, z centre, r centre) and cut into runs of kGroupSize, so a
-// group holds neighbours (a barrel's consecutive radii, a disk's rings);
-// plane layers follow in runs of their own.  Returns the number of groups
-// (n <= 64 layers).
+// ordered by (kind, z centre, r centre); a group takes consecutive layers of
+// one kind while the next one's z range lies within kGroupGap of the
+// group's (a barrel's radii, a disk's rings) and it holds fewer than
+// kGroupSize.  Plane layers follow in runs of their own.  Should that give
+// more than kMaxGroups groups, the ordered layers are cut into plain runs of
+// kGroupSize instead (at most 16 for 64 layers).  Returns the number of
+// groups.
+constexpr float kGroupGap = 20.f;  // mm
+
 inline uint32_t build_layer_groups(const ckf_layer* L, uint32_t n, layer_group* out) {
     uint32_t order[64];
     uint32_t m = 0u;
@@ -301,30 +326,44 @@ inline uint32_t build_layer_groups(const ckf_layer* L, uint32_t n, layer_group* 
         }
         order[j] = v;
     }
-    uint32_t g = 0u;
-    for (uint32_t start = 0; start < m;) {
-        // a group never mixes envelope and plane layers
-        const uint32_t end_run = start < n_box ? n_box : m;
-        const uint32_t stop = start + kGroupSize < end_run ? start + kGroupSize : end_run;
-        layer_group G{};
-        G.bounded = start < n_box ? 1u : 0u;
-        G.n = stop - start;
-        G.env_r[0] = 1e30f;
-        G.env_r[1] = -1e30f;
-        G.env_z[0] = 1e30f;
-        G.env_z[1] = -1e30f;
-        for (uint32_t k = 0; k < G.n; ++k) {
-            const ckf_layer& l = L[order[start + k]];
-            G.member[k] = order[start + k];
-            G.env_r[0] = l.env_r[0] < G.env_r[0] ? l.env_r[0] : G.env_r[0];
-            G.env_r[1] = l.env_r[1] > G.env_r[1] ? l.env_r[1] : G.env_r[1];
-            G.env_z[0] = l.env_z[0] < G.env_z[0] ? l.env_z[0] : G.env_z[0];
-            G.env_z[1] = l.env_z[1] > G.env_z[1] ? l.env_z[1] : G.env_z[1];
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool clustered = pass == 0;
+        uint32_t g = 0u;
+        for (uint32_t start = 0; start < m;) {
+            // a group never mixes envelope and plane layers
+            const uint32_t end_run = start < n_box ? n_box : m;
+            layer_group G{};
+            G.bounded = start < n_box ? 1u : 0u;
+            G.env_r[0] = 1e30f;
+            G.env_r[1] = -1e30f;
+            G.env_z[0] = 1e30f;
+            G.env_z[1] = -1e30f;
+            uint32_t k = start;
+            while (k < end_run && G.n < kGroupSize) {
+                const ckf_layer& l = L[order[k]];
+                if (clustered && G.n > 0u && G.bounded != 0u &&
+                    (l.kind != L[order[start]].kind || l.env_z[0] > G.env_z[1] + kGroupGap ||
+                     l.env_z[1] < G.env_z[0] - kGroupGap)) {
+                    break;
+                }
+                G.member[G.n++] = order[k];
+                G.env_r[0] = l.env_r[0] < G.env_r[0] ? l.env_r[0] : G.env_r[0];
+                G.env_r[1] = l.env_r[1] > G.env_r[1] ? l.env_r[1] : G.env_r[1];
+                G.env_z[0] = l.env_z[0] < G.env_z[0] ? l.env_z[0] : G.env_z[0];
+                G.env_z[1] = l.env_z[1] > G.env_z[1] ? l.env_z[1] : G.env_z[1];
+                ++k;
+            }
+            if (g < kMaxGroups) {
+                out[g] = G;
+            }
+            ++g;
+            start = k;
         }
-        out[g++] = G;
-        start = stop;
+        if (g <= kMaxGroups) {
+            return g;
+        }
     }
-    return g;
+    return kMaxGroups;  // not reached: plain runs of four give at most 16 groups
 }
 
 // Module grid bin of a global point: phi bin (wrapped) and long bin (clamped).

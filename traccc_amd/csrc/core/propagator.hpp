@@ -638,8 +638,8 @@ TAMD_UNROLL
         const T big = static_cast<T>(1e29);
         const T slack = static_cast<T>(1e-2);
         const T mt = cfg.mask_tol_max;
-        const T rr = st.pos.x * st.pos.x + st.pos.y * st.pos.y;
-        const T hb0 = st.pos.x * st.dir.x + st.pos.y * st.dir.y;
+        (void)inv_dz;
+        const nav_ray<T> q = make_nav_ray(st.pos, st.dir);
         // every group's bound, once (four boxes loaded at a time)
         T gb[kMaxGroups];
 TAMD_UNROLL
@@ -651,7 +651,7 @@ TAMD_UNROLL
             }
 TAMD_UNROLL
             for (uint32_t j = 0; j < 4u; ++j) {
-                gb[g0 + j] = nav_box_bound<T>(bx[j], st.pos, st.dir, rr, hb0, mt, inv_dz);
+                gb[g0 + j] = nav_box_bound<T>(bx[j], q, mt);
             }
         }
         for (uint32_t round = 0; round < n_groups; ++round) {
@@ -686,7 +686,7 @@ TAMD_UNROLL
             uint32_t cand = 0u;
 TAMD_UNROLL
             for (uint32_t k = 0; k < kGroupSize; ++k) {
-                const T lb = nav_box_bound<T>(mb[k], st.pos, st.dir, rr, hb0, mt, inv_dz);
+                const T lb = nav_box_bound<T>(mb[k], q, mt);
                 const bool ok = k < gn && ((visited >> mem[k]) & 1ull) == 0ull && lb < big &&
                                 !(lb > bs + slack);
                 cand |= ok ? (1u << k) : 0u;
