@@ -351,6 +351,9 @@ struct propagation {
     mat<S, 8, 8> D;  // accumulated transport
     S path = S(0);
     uint32_t steps = 0;
+    // a surface is reached when the crossing lies within this distance of its
+    // mask (0: the exact mask, the CKF; the fit: min_mask_tolerance)
+    S accept_tol = S(0);
 
     S step_scale() const {
         return static_cast<S>(std::pow(120.0 * static_cast<double>(cfg.propagation.rk_error_tol), 0.2));
@@ -483,7 +486,7 @@ struct propagation {
             if (r == 1) return 2;
             if (r == 2) continue;
             const v3<S> rel = pos() - fl3<S>(sf.center);
-            if (in_mask<S>(sf, dot3(rel, fl3<S>(sf.axis_u)), dot3(rel, fl3<S>(sf.axis_v)), S(0)) &&
+            if (in_mask<S>(sf, dot3(rel, fl3<S>(sf.axis_u)), dot3(rel, fl3<S>(sf.axis_v)), accept_tol) &&
                 path > static_cast<S>(cfg.min_step_length_for_next_surface)) {
                 found = c.idx;
                 return 0;
@@ -651,10 +654,11 @@ void finish_bound(const propagation<S>& P, const ckf_bfield& field, const ckf_su
 template <typename S>
 int propagate_bound(const ckf_finding_config& cfg, const ckf_detector& det, const ckf_bfield& field,
                     const bound_state<S>& in, S mass, S abs_q, bound_state<S>& out, S* path_out,
-                    uint32_t* steps_out) {
+                    uint32_t* steps_out, S accept_tol = S(0)) {
     const ckf_surface& s0 = det.surfaces[in.surface];
     propagation<S> P{cfg, det, field, true, mass, abs_q, mat<S, 8, 1>::zero(), mat<S, 8, 8>::identity(),
                      S(0), 0u};
+    P.accept_tol = accept_tol;
     load_free<S>(P, s0, in);
     uint32_t found = 0u;
     const int r = P.run(static_cast<int>(in.surface), found);
@@ -1631,7 +1635,10 @@ fit_result<S> kalman_fit(const ckf_fitting_config& fc, const ckf_detector& det, 
                 }
             }
             bound_state<S> o;
-            const int r = propagate_bound<S>(nc, det, field, b, mass, abs_q, o, nullptr, nullptr);
+            // the fit reaches a module when its crossing lies within
+            // min_mask_tolerance of the mask
+            const int r = propagate_bound<S>(nc, det, field, b, mass, abs_q, o, nullptr, nullptr,
+                                             static_cast<S>(nc.propagation.min_mask_tolerance));
             if (r == 1) break;  // left the detector: states left unfiltered
             if (r != 0) {
                 fw = kFitPropagationFailure;

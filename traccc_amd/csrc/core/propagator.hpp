@@ -82,6 +82,9 @@ struct nav_config {
     T mass;   // particle hypothesis mass, GeV
     T abs_q;  // |charge| of the hypothesis
     T min_p, min_pT;
+    // a module is reached when the crossing lies within this distance of its
+    // mask (0, the CKF: the exact mask)
+    T accept_tol;
 };
 
 template <typename T>
@@ -105,6 +108,7 @@ inline nav_config<T> make_nav_config(const ckf_finding_config& c) {
     n.abs_q = static_cast<T>(std::fabs(c.ptc_charge));
     n.min_p = static_cast<T>(c.min_p);
     n.min_pT = static_cast<T>(c.min_pT);
+    n.accept_tol = T(0);
     return n;
 }
 
@@ -570,10 +574,11 @@ TAMD_UNROLL
                  dot(sf_center<T>(S) - st.pos, w) / tw < cfg.overstep_tol);
     }
 
-    // On the plane of `S`: inside its exact mask, past the minimum path.
+    // On the plane of `S`: inside its mask (within accept_tol), past the
+    // minimum path.
     TAMD_HD bool accept(const ckf_surface& S) const {
         const vec3<T> x = rel_to(st, sf_center<T>(S));
-        return inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), T(0)) &&
+        return inside_bounds<T>(S, dot(x, sf_u<T>(S)), dot(x, sf_v<T>(S)), cfg.accept_tol) &&
                path > cfg.min_step_length;
     }
 

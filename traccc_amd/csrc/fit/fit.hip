@@ -529,7 +529,14 @@ tamd::nav_config<float> fit_nav(const ckf_fitting_config& c) {
     f.ptc_pdg = c.ptc_pdg;
     f.ptc_mass = c.ptc_mass;
     f.ptc_charge = c.ptc_charge;
-    return tamd::make_nav_config<float>(f);
+    tamd::nav_config<float> n = tamd::make_nav_config<float>(f);
+    // the fit reaches a module when its crossing lies within
+    // min_mask_tolerance of the mask (the propagation configuration's mask
+    // tolerance at zero distance); measurements on the edge of an overlap
+    // module, which an exact mask test can miss by a few microns after the
+    // refit, are then still reached
+    n.accept_tol = static_cast<float>(c.propagation.min_mask_tolerance);
+    return n;
 }
 
 template <typename T>
