@@ -24,6 +24,7 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 
 #include "defs.hpp"
 #include "field.hpp"
@@ -397,13 +398,25 @@ TAMD_UNROLL
 #endif
     }
 
+    // kappa^(-4/5).  On the device in single precision from the hardware
+    // base-2 logarithm and exponential (a few ulp; the step size only needs
+    // to be a smooth function of the state), elsewhere std::pow.
+    TAMD_HD static T kappa_pow(T kappa) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (std::is_same<T, float>::value) {
+            return __builtin_amdgcn_exp2f(-0.8f * __builtin_amdgcn_logf(kappa));
+        }
+#endif
+        return std::pow(kappa, static_cast<T>(-0.8));
+    }
+
     TAMD_HD T step_limit() const {
         const vec3<T> B = field_at(field, st.pos);
         const vec3<T> tb = cross(st.dir, B);
         const T kappa = std::fabs(st.qop) * std::sqrt(dot(tb, tb));
         T h = cfg.step_constraint;
         if (kappa > static_cast<T>(1e-12)) {
-            const T ha = cfg.rk_step_scale * std::pow(kappa, static_cast<T>(-0.8));
+            const T ha = cfg.rk_step_scale * kappa_pow(kappa);
             h = ha < h ? ha : h;
         }
         return h;
