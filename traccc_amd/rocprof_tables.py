@@ -5,6 +5,11 @@
     python -m traccc_amd.rocprof_tables calls <results.db>
         kernel time per operator call of the last plain and the last
         scorer-pruned CKF call in the trace (calls end at k_build_write)
+    python -m traccc_amd.rocprof_tables counters <results.db> [<results.db> ...]
+        derived counters (VALUBusy, VALUUtilization, MeanOccupancyPerActiveCU,
+        MfmaUtil, ...) per kernel of the last plain and the last scorer-pruned
+        CKF call of each run, weighted by dispatch duration, with the kernel's
+        ms per call
     python -m traccc_amd.rocprof_tables traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_mib>
         HBM bytes per launch from one FETCH_SIZE and one WRITE_SIZE counter
         run, each scaled by what the same counter reads for a device copy of
@@ -120,6 +125,69 @@ def _call_sums(db: str) -> dict[str, dict[str, tuple[int, float]]]:
     return out
 
 
+def _call_counters(db: str) -> dict[str, dict[str, dict[str, list]]]:
+    """{kind: {kernel: {counter: [weighted sum, duration sum]}}} over the last
+    plain and the last pruned CKF call of one run (calls end at k_build_write)."""
+    con = sqlite3.connect(db)
+    rows = list(con.execute("select dispatch_id, kernel_name, counter_name, value, duration "
+                            "from counters_collection order by dispatch_id"))
+    disp: dict[int, tuple[str, float, dict[str, float]]] = {}
+    order: list[int] = []
+    for did, kn, cn, v, dur in rows:
+        if did not in disp:
+            disp[did] = (kn, float(dur), {})
+            order.append(did)
+        disp[did][2][cn] = float(v)
+    last: dict[str, list[int]] = {}
+    seg: list[int] = []
+    for did in order:
+        seg.append(did)
+        if "k_build_write" in disp[did][0]:
+            names = {_short(disp[d][0]) for d in seg}
+            kind = ("pruned" if "k_find_list" in names else
+                    "logged" if "k_find<1>" in names else "plain")
+            last[kind] = seg
+            seg = []
+    out: dict[str, dict[str, dict[str, list]]] = {}
+    for kind, sg in last.items():
+        acc: dict[str, dict[str, list]] = defaultdict(lambda: defaultdict(lambda: [0.0, 0.0]))
+        for d in sg:
+            kn, dur, cs = disp[d]
+            for cn, v in cs.items():
+                a = acc[_short(kn)][cn]
+                a[0] += v * dur
+                a[1] += dur
+        out[kind] = acc
+    return out
+
+
+def counters_table(dbs: list[str]) -> str:
+    merged: dict[str, dict[str, dict[str, list]]] = defaultdict(dict)
+    names: list[str] = []
+    for db in dbs:
+        for kind, ks in _call_counters(db).items():
+            for k, cs in ks.items():
+                merged[kind].setdefault(k, {}).update(cs)
+                for cn in cs:
+                    if cn not in names:
+                        names.append(cn)
+    out = ["| call | kernel | ms | " + " | ".join(names) + " |",
+           "|---|---|---:|" + "---:|" * len(names)]
+    for kind in ("pruned", "plain"):
+        ks = merged.get(kind, {})
+        rows = []
+        for k, cs in ks.items():
+            if not k.startswith("k_") or k.startswith(("k_fit", "k_amb", "k_sd", "k_mlp", "k_track")):
+                continue  # the CKF's own kernels only
+            ms = max(v[1] for v in cs.values()) / 1e6
+            vals = [f"{cs[cn][0] / cs[cn][1]:.2f}" if cn in cs and cs[cn][1] > 0 else "" for cn in names]
+            rows.append((ms, k, vals))
+        for ms, k, vals in sorted(rows, key=lambda r: -r[0]):
+            if ms >= 0.5:
+                out.append(f"| {kind} | `{k}` | {ms:.1f} | " + " | ".join(vals) + " |")
+    return "\n".join(out)
+
+
 def traffic(fetch_db: str, write_db: str, copy_fetch_db: str, copy_write_db: str,
             copy_mib: float) -> dict:
     kib = copy_mib * 1024.0
@@ -152,6 +220,8 @@ def main(argv: list[str]) -> int:
         print(kernels_table(argv[1]))
     elif len(argv) >= 2 and argv[0] == "calls":
         print(calls_table(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "counters":
+        print(counters_table(argv[1:]))
     elif len(argv) >= 6 and argv[0] == "traffic":
         print(json.dumps(traffic(*argv[1:5], float(argv[5])), indent=1))
     else:
