@@ -201,14 +201,14 @@ struct ckf_params {
     // (chi2, index) order with their features, scored before the updates
     uint32_t scored;              // 1: k_find takes its candidates from the list
     uint32_t list_len;
-    uint32_t* list_m;             // [in_cap][list_len] measurement
-    float* list_c;                // [in_cap][list_len] chi2
+    uint32_t* list_m;             // [list_len][in_cap] measurement
+    float* list_c;                // [list_len][in_cap] chi2
     uint32_t* find_count;         // [in_cap] branches each input keeps this step
     uint8_t* prop_flag;           // [state_cap] propagation outcome: bit 0 next input, bits 1-2 tips
     uint64_t* seed_run_min;       // [seeds] scorer feature 9: lowest rank-0 running chi2/ndf
     uint32_t* list_seed;          // [in_cap] seed of each listed input
     uint32_t* list_n;             // [in_cap] entries
-    short* list_row;              // [in_cap][list_len][16] each listed candidate's MLP row:
+    short* list_row;              // [list_len][in_cap][16] each listed candidate's MLP row:
                                   // features normalised to BF16, feature 9 raw FP32 in 12-13
     mlp_dev scorer;
     float score_threshold;        // logit below which a candidate is pruned

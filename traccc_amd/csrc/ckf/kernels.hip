@@ -371,6 +371,13 @@ __device__ inline uint32_t flt_slot(const ckf_params& P, uint32_t i, uint32_t k)
     return s < P.flt_cap ? static_cast<uint32_t>(s) : kInvalid;
 }
 
+// Slot of listed candidate q of input i: plane-major like the filtered
+// states, so a wave's lanes (consecutive inputs) read and write one plane's
+// consecutive entries.
+__device__ inline size_t list_slot(const ckf_params& P, uint32_t i, uint32_t q) {
+    return static_cast<size_t>(q) * P.in_cap + i;
+}
+
 // The link a staged slot becomes (a hole link carries the input's own state).
 __device__ inline dlink staged_link(const stage_head& h, const stage_cand& c, uint32_t step,
                                     uint32_t slot) {
@@ -439,7 +446,7 @@ __device__ inline void store_list_row(const mlp_dev& W, short* row, const float 
 
 // Scorer-pruned finding, first kernel: per input the list_len best candidates
 // in (chi2, index) order and their MLP rows, at the input's fixed slots
-// input * list_len + q; k_score_update scores and updates them.
+// list_slot(input, q); k_score_update scores and updates them.
 __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
     step_ctl* ctl = P.ctl;
     if (ctl->done) {
@@ -495,12 +502,12 @@ __global__ __launch_bounds__(kBlock) void k_find_list(ckf_params P) {
                 P.list_seed[i] = in.seed;
             }
             // the team's lanes write the list and the features of their
-            // candidates, at the input's fixed slots i * list_len + q
+            // candidates, at the input's fixed slots list_slot(i, q)
 #pragma unroll
             for (int j = 0; j < kListSlots; ++j) {
                 const uint32_t q = static_cast<uint32_t>(j * kListTeam) + tl;
                 if (q < k) {
-                    const size_t r = static_cast<size_t>(i) * Lc + q;
+                    const size_t r = list_slot(P, i, q);
                     P.list_m[r] = sm[j];
                     P.list_c[r] = sc[j];
                     const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, sm[j]);
@@ -574,7 +581,6 @@ __device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t
                                                    uint32_t step, uint32_t prev_base) {
     step_ctl* ctl = P.ctl;
     const uint32_t N = P.n_branch_surface;
-    const size_t slot0 = static_cast<size_t>(i) * P.list_len;
     uint32_t kept = 0u;
     for (uint32_t q = 0; q < k; ++q) {
         const uint32_t r = base + q;
@@ -584,7 +590,7 @@ __device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t
     }
     if (P.verdict_log != nullptr) {
         for (uint32_t q = 0; q < k; ++q) {
-            log_verdict(P, step, i, P.list_m[slot0 + q], ((kept >> q) & 1u) != 0u);
+            log_verdict(P, step, i, P.list_m[list_slot(P, i, q)], ((kept >> q) & 1u) != 0u);
         }
     }
     find_input in;
@@ -594,8 +600,8 @@ __device__ __forceinline__ void score_update_input(const ckf_params& P, uint32_t
     while (count < N && kept != 0u) {
         const uint32_t li = static_cast<uint32_t>(__builtin_ctz(kept));
         kept &= kept - 1u;
-        const uint32_t best_m = P.list_m[slot0 + li];
-        const float best_c = P.list_c[slot0 + li];
+        const uint32_t best_m = P.list_m[list_slot(P, i, li)];
+        const float best_c = P.list_c[list_slot(P, i, li)];
         const meas_data md = load_meas(P.meas_pos, P.meas_var, P.meas_info, best_m);
         float fx[6], fc[21];
 #pragma unroll
@@ -653,7 +659,6 @@ void k_score_update(ckf_params P) {
     const uint32_t n_in = ctl->n_in < P.in_cap ? ctl->n_in : P.in_cap;
     const uint32_t prev_base = ctl->prev_link_base;
     const uint32_t epoch = ctl->epoch;
-    const uint32_t Lc = P.list_len;
     const uint32_t l = lane();
     const uint32_t wave = threadIdx.x / kWave;
     short* rows = s_rows[wave];
@@ -672,7 +677,6 @@ void k_score_update(ckf_params P) {
         const uint32_t incl = wave_inclusive_scan(k);
         const uint32_t total = wave_broadcast(incl, kWave - 1);
         const uint32_t base = incl - k;
-        const size_t slot0 = static_cast<size_t>(i) * Lc;
         // rows [r0, r0 + kRows) per pass: pack, then score 16 at a time
         for (uint32_t r0 = 0u; r0 < total; r0 += kRows) {
             for (uint32_t q = 0; q < k; ++q) {
@@ -680,7 +684,8 @@ void k_score_update(ckf_params P) {
                 if (r < r0 || r >= r0 + kRows) {
                     continue;
                 }
-                const uint4* src = reinterpret_cast<const uint4*>(P.list_row + (slot0 + q) * kRowShorts);
+                const uint4* src =
+                    reinterpret_cast<const uint4*>(P.list_row + list_slot(P, i, q) * kRowShorts);
                 const uint4 a = src[0];
                 uint4 b = src[1];
                 // feature 9 (short 9, the high half of b.x) from the raw
