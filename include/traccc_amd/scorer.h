@@ -107,10 +107,11 @@ ckf_status ckf_scorer_gradient_noalloc(const ckf_scorer* s, const float* d_featu
 
 /* ---- in the CKF ----------------------------------------------------------- */
 /* Prune, before the Kalman update, every candidate whose logit is below
- * `threshold`: each input branch lists its max_num_branches_per_surface + 4
- * best candidates by (chi2, measurement index), the scorer scores the list,
- * and the update walks the surviving entries in order as the plain CKF walks
- * all candidates.  scorer == NULL restores the plain CKF.  The scorer must
+ * `threshold`: each input branch lists its max_num_branches_per_surface + 1
+ * best candidates by (chi2, measurement index) -- the plain CKF's selection
+ * and one more in place of a failed update -- the scorer scores the list,
+ * and the update walks the surviving entries in order as the plain CKF
+ * walks all candidates.  scorer == NULL restores the plain CKF.  The scorer must
  * outlive its use by the algorithm. */
 ckf_status ckf_algorithm_set_scorer(ckf_algorithm* alg, const ckf_scorer* scorer, float threshold);
 

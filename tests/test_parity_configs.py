@@ -133,7 +133,7 @@ def _pruned_vs_replay(name, det, fld, ev, cfg, quantile, capacity):
     kept = keys[(keys & kept_bit) != 0] & ~kept_bit
     frac = len(kept) / len(keys)
     t0 = time.perf_counter()
-    o = oracle.run_masked(cfg, det, fld, ev, min(cfg.max_num_branches_per_surface + 4, 16), kept)
+    o = oracle.run_masked(cfg, det, fld, ev, min(cfg.max_num_branches_per_surface + 1, 16), kept)
     print(f"\n{name}: {len(keys)} scored candidates, {frac:.3f} kept at threshold "
           f"{threshold:.3f}; plain {len(plain)} tracks; oracle replay {time.perf_counter() - t0:.1f} s",
           flush=True)
