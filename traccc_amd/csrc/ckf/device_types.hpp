@@ -84,6 +84,10 @@ static_assert(sizeof(line_u32) == 128 && sizeof(line_u64) == 128, "one counter p
 // without host round trips.  The fields every wave reads at kernel start
 // share the first line; each counter that waves hammer with atomics has its
 // own.
+// A deferred propagation: link, visited layers (2 words), RK steps, path,
+// the free state (11 words), the transport Jacobian (31 words), state ref.
+constexpr uint32_t kDeferWords = 48u;
+
 struct step_ctl {
     uint32_t step;        // current CKF step
     uint32_t epoch;       // tag of this step's look-back descriptors
@@ -107,6 +111,7 @@ struct step_ctl {
     line_u32 exit_count;  // blocks of k_prop_emit that have finished this step
     line_u32 n_rows;      // scorer rows scored this step (added up by k_score_update)
     line_u64 n_props;     // propagations so far
+    line_u32 n_defer;     // links whose propagation k_propagate_resume continues (this step)
     line_u32 xcd_ticket[4][8];  // per-XCD work queues (one counter per XCD): find, scorer
                                 // list, propagate, dedup mark
 };
@@ -118,6 +123,10 @@ struct ckf_params {
     uint32_t n_layers;
     const layer_group* groups;  // build_layer_groups of the layers
     uint32_t n_groups;
+    // propagations deferred at their second layer search: kDeferWords
+    // words per entry, word w of entry k at defer[w * defer_cap + k]
+    uint32_t* defer;
+    uint32_t defer_cap;
     uint32_t n_surfaces;
     field_view<float> field;
     nav_config<float> nav;

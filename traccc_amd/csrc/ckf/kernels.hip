@@ -1106,8 +1106,54 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl);
 // ---------------------------------------------------------------------------
 // propagate_to_next_surface
 // ---------------------------------------------------------------------------
+// The end of a link's propagation (status from the navigator): on NAV_FOUND
+// the covariance (read now; the navigation does not need it, so it is not
+// held in registers meanwhile) with the start surface's material at step 0,
+// the transport to the new surface, its material, the sanity checks and the
+// new state; the outcome flag (bit 0: next input, bits 1-2: tips).
+template <int kT>
+__device__ inline uint8_t finish_link(const ckf_params& P, const navigator<float, true, kT>& nav,
+                                      int status, uint32_t end_sf, uint32_t i, uint32_t ref,
+                                      uint32_t step, bool leader) {
+    const uint32_t in_par = (step + 1u) & 1u;
+    const uint32_t out_par = step & 1u;
+    const bool success = (status == NAV_FOUND);
+    bool ok = success;
+    bool succ = false;
+    uint32_t ntip = 0u;
+    if (success) {
+        float x[6], c[21];
+        uint32_t sf = 0u;
+        load_ref(P, ref, in_par, x, c, sf);
+        if (step == 0u) {
+            add_material(P, sf, x, c);
+        }
+        const geo_view G{P.sfs, P.n_layers, P.field};
+        finish_transport<kT>(G, P.nav, nav, end_sf, x, c);
+        sf = end_sf;
+        add_material(P, sf, x, c);
+        const float th = x[e_theta];
+        if (th <= 0.f || th >= 6.28318530717958647692f) ok = false;
+        if (!isfinite(x[e_phi])) ok = false;
+        if (fabsf(x[e_qop]) == 0.f) ok = false;
+        if (ok) {
+            if (leader) {
+                store_soa(P.state[out_par], P.state_cap, i, x, c, sf);
+            }
+            succ = true;
+        }
+    }
+    if (!ok && step + 1u >= P.min_cands) ++ntip;
+    if (success && step == P.max_cands - 1u) ++ntip;
+    return static_cast<uint8_t>((succ ? 1u : 0u) | (ntip << 1));
+}
+
 // kT lanes propagate one link together (team navigator); 64 / kT links per
-// wave partition.  The team leader writes the results.
+// wave partition.  The team leader writes the results.  With one lane per
+// link, a propagation that needs a second layer search (a few % of them, but
+// nearly every wave has one) is written to the defer buffer and continued by
+// k_propagate_resume, so the wave does not run a second search, approach and
+// window for one or two of its lanes.
 template <int kT>
 __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     step_ctl* ctl = P.ctl;
@@ -1123,10 +1169,10 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
     constexpr uint32_t kLinksPerWave = kWave / kT;
     const uint32_t n_parts = (n + kLinksPerWave - 1) / kLinksPerWave;
     const uint32_t in_par = (step + 1u) & 1u;
-    const uint32_t out_par = step & 1u;
     const bool dedup = step >= P.dup_min_len;
     const uint32_t tl = lane() % kT;
     const bool leader = (tl == 0u);
+    const bool defer = (kT == 1) && P.defer_cap > 0u;
 
     xcd_ticket_run queue(&ctl->xcd_ticket[2][0].v, n_parts, 1u);
     for (;;) {
@@ -1136,9 +1182,9 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
         }
         const uint32_t i = p * kLinksPerWave + lane() / kT;
         const bool valid = i < n;
-        bool succ = false;
         bool propagated = false;
-        uint32_t ntip = 0u;
+        bool deferred = false;
+        uint8_t flag = 0u;
         if (valid) {
             const dlink L = load_link(P, base + i);
             bool alive = true;
@@ -1152,10 +1198,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
             }
             if (alive && rank < P.n_branch_seed) {
                 if (link_skipped(L) > P.max_skip || link_consec(L) > P.max_consec) {
-                    ntip = 1u;
+                    flag = static_cast<uint8_t>(1u << 1);
                 } else {
-                    // the covariance is read after the navigation, which does
-                    // not need it: fewer registers live while navigating
                     float x[6];
                     uint32_t sf = 0u;
                     {
@@ -1168,47 +1212,113 @@ __global__ __launch_bounds__(kBlock) void k_propagate(ckf_params P) {
                         make_navigator<kT>(G, s_nav.layers, P.nav, tl, &s_nav, P.n_groups);
                     bound_to_free<float, true>(P.sfs[sf], x, nav.st, nav.J);
                     uint32_t end_sf = 0u;
-                    const int status = nav.propagate(static_cast<int>(sf), end_sf);
-                    const bool success = (status == NAV_FOUND);
-                    bool ok = success;
-                    float c[21];
-                    if (success) {
-                        {
-                            float x_again[6];
-                            uint32_t sf_again = 0u;
-                            load_ref(P, L.state_ref, in_par, x_again, c, sf_again);
-                        }
-                        if (step == 0u) {
-                            add_material(P, sf, x, c);
-                        }
-                        finish_transport<kT>(G, P.nav, nav, end_sf, x, c);
-                        sf = end_sf;
-                        add_material(P, sf, x, c);
-                        const float th = x[e_theta];
-                        if (th <= 0.f || th >= 6.28318530717958647692f) ok = false;
-                        if (!isfinite(x[e_phi])) ok = false;
-                        if (fabsf(x[e_qop]) == 0.f) ok = false;
-                        if (ok) {
-                            if (leader) {
-                                store_soa(P.state[out_par], P.state_cap, i, x, c, sf);
+                    int status = nav.propagate(static_cast<int>(sf), end_sf, defer);
+                    if (status == NAV_DEFER) {
+                        const uint32_t k = atomicAdd(&ctl->n_defer.v, 1u);
+                        if (k < P.defer_cap) {
+                            uint32_t w[kDeferWords];
+                            w[0] = i;
+                            w[1] = static_cast<uint32_t>(nav.deferred_visited);
+                            w[2] = static_cast<uint32_t>(nav.deferred_visited >> 32);
+                            w[3] = nav.n_steps;
+                            w[4] = __float_as_uint(nav.path);
+                            const float fs[11] = {nav.st.pos.x, nav.st.pos.y, nav.st.pos.z,
+                                                  nav.st.dir.x, nav.st.dir.y, nav.st.dir.z,
+                                                  nav.st.qop,   nav.st.time,  nav.st.lo.x,
+                                                  nav.st.lo.y,  nav.st.lo.z};
+#pragma unroll
+                            for (int e = 0; e < 11; ++e) w[5 + e] = __float_as_uint(fs[e]);
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                                for (int j = 0; j < 5; ++j) {
+                                    w[16 + 5 * r + j] = __float_as_uint(nav.J.p[r][j]);
+                                    w[31 + 5 * r + j] = __float_as_uint(nav.J.d[r][j]);
+                                }
                             }
-                            succ = true;
+                            w[46] = __float_as_uint(nav.J.t_qop);
+                            w[47] = L.state_ref;
+#pragma unroll
+                            for (uint32_t e = 0; e < kDeferWords; ++e) {
+                                P.defer[static_cast<size_t>(e) * P.defer_cap + k] = w[e];
+                            }
+                            deferred = true;
+                        } else {
+                            // no room: continue here
+                            status = nav.layer_loop(nav.deferred_visited, 1u, end_sf, false);
                         }
                     }
-                    if (!ok && step + 1u >= P.min_cands) ++ntip;
-                    if (success && step == P.max_cands - 1u) ++ntip;
+                    if (!deferred) {
+                        flag = finish_link<kT>(P, nav, status, end_sf, i, L.state_ref, step, leader);
+                    }
                 }
             }
         }
-        // the link's outcome for k_prop_emit; the propagation count here
-        if (valid && leader) {
-            P.prop_flag[i] = static_cast<uint8_t>((succ ? 1u : 0u) | (ntip << 1));
+        // the link's outcome for k_prop_emit (a deferred one's comes from
+        // k_propagate_resume); the propagation count here
+        if (valid && leader && !deferred) {
+            P.prop_flag[i] = flag;
         }
         const uint64_t mp = wave_ballot(leader && propagated);
         if (lane() == 0u && mp != 0ull) {
             atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->n_props.v),
                       static_cast<unsigned long long>(__popcll(mp)));
         }
+    }
+}
+
+// The deferred propagations of k_propagate<1>, one lane each: the layer
+// loop continues from its second iteration with the saved state, then the
+// same end as in k_propagate.
+__global__ __launch_bounds__(kBlock) void k_propagate_resume(ckf_params P) {
+    step_ctl* ctl = P.ctl;
+    if (ctl->done) {
+        return;
+    }
+    __shared__ nav_tables s_nav;
+    stage_nav_tables(s_nav, P.layers, P.n_layers, P.groups, P.n_groups, P.nav.mask_tol_max);
+    const uint32_t step = ctl->step;
+    const uint32_t nd = ctl->n_defer.v;
+    const uint32_t n = nd < P.defer_cap ? nd : P.defer_cap;
+    const uint32_t n_parts = (n + kWave - 1u) / kWave;
+    const bool takes = wave_takes_tickets(n_parts);
+    for (;;) {
+        const uint32_t p = takes ? wave_ticket(&ctl->ticket[7].v) : n_parts;
+        if (p >= n_parts) {
+            break;
+        }
+        const uint32_t k = p * kWave + lane();
+        if (k >= n) {
+            continue;
+        }
+        uint32_t w[kDeferWords];
+#pragma unroll
+        for (uint32_t e = 0; e < kDeferWords; ++e) {
+            w[e] = P.defer[static_cast<size_t>(e) * P.defer_cap + k];
+        }
+        const geo_view G{P.sfs, P.n_layers, P.field};
+        navigator<float, true, 1> nav = make_navigator<1>(G, s_nav.layers, P.nav, 0u, &s_nav, P.n_groups);
+        const uint64_t visited = static_cast<uint64_t>(w[1]) | (static_cast<uint64_t>(w[2]) << 32);
+        nav.n_steps = w[3];
+        nav.path = __uint_as_float(w[4]);
+        nav.st.pos = {__uint_as_float(w[5]), __uint_as_float(w[6]), __uint_as_float(w[7])};
+        nav.st.dir = {__uint_as_float(w[8]), __uint_as_float(w[9]), __uint_as_float(w[10])};
+        nav.st.qop = __uint_as_float(w[11]);
+        nav.st.time = __uint_as_float(w[12]);
+        nav.st.lo = {__uint_as_float(w[13]), __uint_as_float(w[14]), __uint_as_float(w[15])};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                nav.J.p[r][j] = __uint_as_float(w[16 + 5 * r + j]);
+                nav.J.d[r][j] = __uint_as_float(w[31 + 5 * r + j]);
+            }
+        }
+        nav.J.t_qop = __uint_as_float(w[46]);
+        uint32_t end_sf = 0u;
+        const int status = nav.layer_loop(visited, 1u, end_sf, false);
+        const uint32_t i = w[0];
+        P.prop_flag[i] = finish_link<1>(P, nav, status, end_sf, i, w[47], step, true);
     }
 }
 
@@ -1346,6 +1456,7 @@ __device__ void advance_step(const ckf_params& P, step_ctl* ctl) {
     ctl->n_steps_run = ctl->step;
     ctl->epoch += 1u;
     for (int k = 0; k < kTickets; ++k) ctl->ticket[k].v = 0u;
+    ctl->n_defer.v = 0u;
     for (int q = 0; q < 4; ++q) {
         for (uint32_t x = 0; x < kXcds; ++x) ctl->xcd_ticket[q][x].v = 0u;
     }
@@ -1850,6 +1961,7 @@ hipError_t launch_propagate_work(const ckf_params& P, const launch_shape& s, hip
             break;
         default:
             hipLaunchKernelGGL(k_propagate<1>, dim3(s.prop_blocks), dim3(kBlock), 0, stream, P);
+            hipLaunchKernelGGL(k_propagate_resume, dim3(s.resume_blocks), dim3(kBlock), 0, stream, P);
             break;
     }
     return hipGetLastError();

@@ -16,6 +16,7 @@ struct launch_shape {
     uint32_t list_blocks;   // scorer-pruned finding: candidate lists
     uint32_t score_blocks;  // and the MLP + updates over them (64 inputs per wave)
     uint32_t emit_blocks;   // the compaction kernels (k_*_emit)
+    uint32_t resume_blocks; // deferred propagations (k_propagate_resume)
 };
 
 // One CKF step: find_tracks, duplicate removal (active from

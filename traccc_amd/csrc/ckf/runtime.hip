@@ -98,6 +98,9 @@ struct ckf_algorithm {
     // scorer training data (branch log) and scorer-pruned finding
     dbuf feat_slot, log_feat, log_label, seed_best, list_m, list_c, list_n, list_row,
         find_count, seed_run_min, list_seed, prop_flag;
+    // propagations deferred to k_propagate_resume (one lane per link)
+    dbuf defer;
+    uint32_t defer_cap = 0;
     bool logging = false;
     // candidate-verdict log of scored calls (diagnostics)
     dbuf verdict_log, verdict_n;
@@ -133,7 +136,7 @@ struct ckf_algorithm {
                        &pkf_states, &pkf_n, &pkf_chi2, &pkf_ndf, &pkf_holes, &pkf_cpos,
                        &pkf_lb_lo, &feat_slot, &log_feat,
                        &log_label, &seed_best, &list_m, &list_c, &list_n, &list_row, &find_count,
-                       &seed_run_min, &list_seed, &prop_flag, &verdict_log, &verdict_n};
+                       &seed_run_min, &list_seed, &prop_flag, &verdict_log, &verdict_n, &defer};
         for (dbuf* b : all) b->release();
         for (int k = 0; k < tamd::kLookbacks; ++k) {
             lb_lo[k].release();
@@ -168,6 +171,10 @@ void ensure_buffers(ckf_algorithm& A, const ckf_device_event& ev) {
     A.live.ensure(4u * size_t(A.in_cap));
     A.find_count.ensure(4u * size_t(A.in_cap));
     A.prop_flag.ensure(size_t(A.state_cap));
+    // a propagation that needs a second layer search (a few % of them)
+    // continues in its own kernel; beyond the buffer it continues in place
+    A.defer_cap = std::max<uint32_t>(A.state_cap / 8u, 1024u);
+    A.defer.ensure(4u * size_t(tamd::kDeferWords) * A.defer_cap);
     A.tips.ensure(4u * size_t(A.tip_cap));
     A.seed_first.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
     A.seed_base.ensure(4u * size_t(std::max(ev.n_seeds, 1u)));
@@ -251,6 +258,8 @@ tamd::ckf_params make_params(ckf_algorithm& A, const ckf_device_detector& D,
     P.n_layers = static_cast<uint32_t>(D.h_layers.size());
     P.groups = D.d_groups;
     P.n_groups = static_cast<uint32_t>(D.h_groups.size());
+    P.defer = A.defer.as<uint32_t>();
+    P.defer_cap = A.defer_cap;
     P.n_surfaces = static_cast<uint32_t>(D.h_sfs.size());
     P.field = tamd::make_field_view<float>(D.field, D.d_map);
     P.nav = tamd::make_nav_config<float>(c);
@@ -357,6 +366,7 @@ tamd::launch_shape make_shape(const ckf_algorithm& A) {
     s.prop_team = A.prop_team;
     const uint32_t prop_parts = (A.state_cap + 64u / s.prop_team - 1u) / (64u / s.prop_team);
     s.prop_blocks = std::max(1u, std::min((prop_parts + 3u) / 4u, max_blocks));
+    s.resume_blocks = std::max(1u, std::min((parts(A.defer_cap) + 3u) / 4u, max_blocks));
     // scorer-pruned finding: 16 inputs per wave listing, 64 per scoring wave
     s.list_blocks = std::max(1u, std::min((((A.in_cap + 15u) / 16u) + 3u) / 4u, max_blocks));
     s.score_blocks = std::max(1u, std::min((((A.in_cap + 63u) / 64u) + 3u) / 4u, max_blocks));

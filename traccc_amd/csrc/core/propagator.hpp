@@ -323,7 +323,9 @@ TAMD_HD T highland_theta0(T x_over_x0, T qop, T mass, T abs_q) {
     return T(0.0136) / (beta * p) * abs_q * std::sqrt(x_over_x0) * corr;
 }
 
-enum nav_status : int { NAV_FOUND = 0, NAV_EXIT = 1, NAV_ABORT = 2 };
+// NAV_DEFER: propagate(..., defer) stopped before its second layer search;
+// layer_loop(deferred_visited, 1, ...) continues it.
+enum nav_status : int { NAV_FOUND = 0, NAV_EXIT = 1, NAV_ABORT = 2, NAV_DEFER = 3 };
 
 constexpr int kMaxCandidates = 9;
 
@@ -376,6 +378,7 @@ struct navigator {
     transport_jac<T> J;
     T path;
     uint32_t n_steps;
+    uint64_t deferred_visited = 0ull;
 
     // Team-wide minimum of (v, key), the lowest key winning ties; every lane
     // of the team ends with the winner.
@@ -765,8 +768,10 @@ TAMD_UNROLL
     }
 
     // Propagate from the current state (on surface start_sf, or -1 for a
-    // free start) to the next sensitive surface.
-    TAMD_HD int propagate(int start_sf, uint32_t& out_sf) {
+    // free start) to the next sensitive surface.  With defer, a propagation
+    // that needs a second layer search stops there with NAV_DEFER (state,
+    // path and step count kept, the visited layers in deferred_visited).
+    TAMD_HD int propagate(int start_sf, uint32_t& out_sf, bool defer = false) {
         path = T(0);
         n_steps = 0u;
         if (below_momentum_cut()) {
@@ -784,7 +789,17 @@ TAMD_UNROLL
             }
             visited |= (1ull << l0);
         }
-        for (uint32_t guard = 0; guard <= n_layers; ++guard) {
+        return layer_loop(visited, 0u, out_sf, defer);
+    }
+
+    // The layer loop of propagate from iteration guard0 with the layers in
+    // `visited` done: pick the next layer, approach it, visit its window.
+    TAMD_HD int layer_loop(uint64_t visited, uint32_t guard0, uint32_t& out_sf, bool defer) {
+        for (uint32_t guard = guard0; guard <= n_layers; ++guard) {
+            if (defer && guard > guard0) {
+                deferred_visited = visited;
+                return NAV_DEFER;
+            }
             int best = -1;
             T bs = T(0);
             pick_layer(visited, best, bs);
