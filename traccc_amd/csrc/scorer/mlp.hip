@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
     float w3[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) w3[nt] = W.w3[16 * nt + r16];
-    // dW3[c] = sum_r H2[r][c] dS[r]; db3 = sum_r dS[r]
+    // db3 = sum_r dS[r]
     {
         float db3 = 0.f;
 #pragma unroll
@@ -182,34 +182,31 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) db3 += __shfl_xor(db3, d, 64);
         if (l == 0u) gp[kOffB3] = db3;
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            float acc = 0.f;
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc += gelu(z2[mt][nt][e]) * ds[mt][e];
-            }
-            acc += __shfl_xor(acc, 16, 64);
-            acc += __shfl_xor(acc, 32, 64);
-            if (l < 16u) gp[kOffW3 + 16 * nt + r16] = acc;
-        }
     }
-    // dZ2 (in place of z2) and db2
+    // dW3[c] = sum_r H2[r][c] dS[r]; dZ2 = dS W3^T * GELU'(Z2) (in place of
+    // z2) and db2, from one GELU evaluation per entry
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        float acc = 0.f;
+        float accw = 0.f, accb = 0.f;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                z2[mt][nt][e] = ds[mt][e] * w3[nt] * gelu_grad(z2[mt][nt][e]);
-                acc += z2[mt][nt][e];
+                float h, gg;
+                gelu_and_grad(z2[mt][nt][e], h, gg);
+                accw += h * ds[mt][e];
+                z2[mt][nt][e] = ds[mt][e] * w3[nt] * gg;
+                accb += z2[mt][nt][e];
             }
         }
-        acc += __shfl_xor(acc, 16, 64);
-        acc += __shfl_xor(acc, 32, 64);
-        if (l < 16u) gp[kOffB2 + 16 * nt + r16] = acc;
+        accw += __shfl_xor(accw, 16, 64);
+        accw += __shfl_xor(accw, 32, 64);
+        accb += __shfl_xor(accb, 16, 64);
+        accb += __shfl_xor(accb, 32, 64);
+        if (l < 16u) {
+            gp[kOffW3 + 16 * nt + r16] = accw;
+            gp[kOffB2 + 16 * nt + r16] = accb;
+        }
     }
     // dW2 = H1^T dZ2: A = H1^T (from the H1 tile in LDS), B = dZ2 in D layout
 #pragma unroll
@@ -305,6 +302,7 @@ struct adam_hyper {
 // views (the k_mlp_pack layouts).  Block 0 also sums the tile losses into
 // losses[batch_idx].  The last block to finish advances the training state.
 constexpr int kStepWaves = 16;
+constexpr int kStepLoads = 32;
 
 __device__ inline void pack_param(uint32_t p, float w, short* w1t, short* w2, short* w2t,
                                   float* b1, float* b2, float* w3, float* b3) {
@@ -339,14 +337,36 @@ __global__ __launch_bounds__(64 * kStepWaves) void k_mlp_step(
     const uint32_t l = __lane_id();
     const uint32_t p = blockIdx.x * 64u + l;
     const bool real = p < static_cast<uint32_t>(kMlpParams);
+    // the wave's tiles in batches of kStepLoads loads in flight, added in
+    // tile order (the same sum as one load at a time)
     float acc = 0.f;
-    for (uint32_t t = wave; t < tiles; t += kStepWaves) {
-        acc += real ? grad_part[static_cast<size_t>(t) * kMlpParams + p] : 0.f;
+    for (uint32_t t0 = wave; t0 < tiles; t0 += kStepWaves * kStepLoads) {
+        float g[kStepLoads];
+#pragma unroll
+        for (int k = 0; k < kStepLoads; ++k) {
+            const uint32_t t = t0 + static_cast<uint32_t>(k) * kStepWaves;
+            g[k] = (real && t < tiles) ? grad_part[static_cast<size_t>(t) * kMlpParams + p] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kStepLoads; ++k) {
+            if (t0 + static_cast<uint32_t>(k) * kStepWaves < tiles) acc += g[k];
+        }
     }
     s_part[wave][l] = acc;
     if (blockIdx.x == 0u && wave == 1u) {
         float lo = 0.f;
-        for (uint32_t t = l; t < tiles; t += 64u) lo += loss_part[t];
+        for (uint32_t t0 = l; t0 < tiles; t0 += 64u * kStepLoads) {
+            float x[kStepLoads];
+#pragma unroll
+            for (int k = 0; k < kStepLoads; ++k) {
+                const uint32_t t = t0 + 64u * static_cast<uint32_t>(k);
+                x[k] = t < tiles ? loss_part[t] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < kStepLoads; ++k) {
+                if (t0 + 64u * static_cast<uint32_t>(k) < tiles) lo += x[k];
+            }
+        }
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) lo += __shfl_xor(lo, d, 64);
         if (l == 0u) losses[ts.batch_idx] = lo;

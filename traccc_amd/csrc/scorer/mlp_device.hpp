@@ -40,10 +40,8 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline short f2bf(float f) {
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
+    // round to nearest even: gfx950's v_cvt_pk_bf16_f32
+    return __builtin_bit_cast(short, static_cast<__bf16>(f));
 }
 __device__ inline float bf2f(short h) {
     return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(h)) << 16);
@@ -70,9 +68,26 @@ __device__ inline float erf_as(float x) {
 }
 
 __device__ inline float gelu(float z) { return 0.5f * z * (1.f + erf_as(z * 0.70710678118654752f)); }
+
+// GELU and its derivative 0.5 (1 + erf(z / sqrt 2)) + z phi(z) from one
+// exponential: erf_as's exp(-x^2) at x = z / sqrt 2 is the exp(-z^2 / 2) of
+// the normal density phi.
+__device__ inline void gelu_and_grad(float z, float& g, float& gg) {
+    const float x = z * 0.70710678118654752f;
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.f + 0.3275911f * ax);
+    const float poly =
+        ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+         0.254829592f) * t;
+    const float ex = __expf(-ax * ax);
+    const float one_erf = 1.f + copysignf(1.f - poly * ex, x);
+    g = 0.5f * z * one_erf;
+    gg = 0.5f * one_erf + z * 0.39894228040143268f * ex;
+}
 __device__ inline float gelu_grad(float z) {
-    return 0.5f * (1.f + erf_as(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * __expf(-0.5f * z * z);
+    float g, gg;
+    gelu_and_grad(z, g, gg);
+    return gg;
 }
 
 // Wave-internal LDS hand-over: every lane's earlier LDS writes are visible to
