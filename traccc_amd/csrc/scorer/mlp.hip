@@ -32,6 +32,9 @@ namespace tamd {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+// LDS per wave of the gradient kernel: the forward pass's X and H1 tiles,
+// then H1^T [64][kHStride], X^T [16][kHStride] and the tile's 64 labels (FP32)
+constexpr int kGradLdsPerWave = kMlpLdsPerWave + 64 * kHStride + 16 * kHStride + 128;
 constexpr int kBlock = 64 * kWavesPerBlock;
 
 // Pseudo-random bijection of [0, n): i -> (a i + b) mod n, gcd(a, n) = 1.
@@ -112,15 +115,18 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
                                                       const train_state* state, uint32_t n,
                                                       uint32_t batch, float pos_weight,
                                                       float* grad_part, float* loss_part) {
-    __shared__ short lds[kWavesPerBlock * kMlpLdsPerWave];
+    __shared__ short lds[kWavesPerBlock * kGradLdsPerWave];
     const train_state ts = *state;
     const perm_map perm = ts.perm;
     const uint32_t row0 = ts.batch_idx * batch;
     const uint32_t rows = batch_rows(ts, n, batch);
     const float inv_norm = rows > 0u ? 1.f / static_cast<float>(rows) : 0.f;
     const uint32_t wave = threadIdx.x / 64u;
-    short* lds_x = lds + wave * kMlpLdsPerWave;
+    short* lds_x = lds + wave * kGradLdsPerWave;
     short* lds_h = lds_x + 64 * kMlpInPad;
+    short* lds_ht = lds_h + 64 * kHStride;
+    short* lds_xt = lds_ht + 64 * kHStride;
+    float* lds_y = reinterpret_cast<float*>(lds_xt + 16 * kHStride);
     const uint32_t tile = blockIdx.x * kWavesPerBlock + wave;
     const uint32_t l = __lane_id();
     const uint32_t r16 = l % 16u;
@@ -132,14 +138,18 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
 
     const uint32_t i = tile * 64u + l;
     const bool valid = i < rows;
+    // the lane's row: its features, and its label into the wave's LDS for
+    // the lanes that hold the row's logit below (the permutation is
+    // evaluated once per row)
     float f[kMlpIn];
     {
         const uint32_t row = valid ? perm(row0 + i) : 0u;
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) f[k] = valid ? feats[static_cast<size_t>(row) * kMlpIn + k] : 0.f;
+        lds_y[l] = (valid && labels[row]) ? 1.f : 0.f;
     }
     float z1[4][4][4], z2[4][4][4], s[4][4];
-    mlp_tile_layers(W, f, valid, lds_x, lds_h, z1, z2);
+    mlp_tile_layers(W, f, valid, lds_x, lds_h, z1, z2, lds_xt, lds_ht);
     mlp_tile_output(W, z2, s);
 
     // loss and dL/ds for the rows this lane holds
@@ -150,9 +160,9 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t pos = tile * 64u + 16u * mt + g4 + e;
+            const float y = lds_y[16u * mt + g4 + e];
             ds[mt][e] = 0.f;
             if (pos < rows) {
-                const float y = labels[perm(row0 + pos)] ? 1.f : 0.f;
                 const float w = (y > 0.5f ? pos_weight : 1.f) * inv_norm;
                 const float x = s[mt][e];
                 const float sig = 1.f / (1.f + expf(-x));
@@ -208,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
             gp[kOffB2 + 16 * nt + r16] = accb;
         }
     }
-    // dW2 = H1^T dZ2: A = H1^T (from the H1 tile in LDS), B = dZ2 in D layout
+    // dW2 = H1^T dZ2: A = H1^T (the transposed tile in LDS), B = dZ2 in D layout
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         bf16x4 b[4];
@@ -222,9 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt) {
-                bf16x4 a;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) a[e] = lds_h[(16 * kt + g4 + e) * kHStride + 16 * mi + r16];
+                const bf16x4 a = load4(lds_ht + (16 * mi + r16) * kHStride + 16 * kt + g4);
                 acc = mfma16(a, b[kt], acc);
             }
 #pragma unroll
@@ -269,18 +277,16 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(mlp_dev W, const float* fea
         accb += __shfl_xor(accb, 32, 64);
         if (l < 16u) gp[kOffB1 + 16 * nt + r16] = accb;
     }
-    // dW1 = X^T dZ1: A = X^T (from the X tile in LDS), B = dZ1 in D layout
+    // dW1 = X^T dZ1: A = X^T (the transposed tile in LDS), B = dZ1 in D layout
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
-            bf16x4 a, b;
+            const bf16x4 a = load4(lds_xt + r16 * kHStride + 16 * kt + g4);
+            bf16x4 b;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                a[e] = lds_x[(16 * kt + g4 + e) * kMlpInPad + r16];
-                b[e] = f2bf(z1[kt][nt][e]);
-            }
+            for (int e = 0; e < 4; ++e) b[e] = f2bf(z1[kt][nt][e]);
             acc = mfma16(a, b, acc);
         }
 #pragma unroll

@@ -104,10 +104,12 @@ __device__ inline bf16x4 load4(const short* p) { return *reinterpret_cast<const 
 // holds its row's 12 raw features (lane = row; valid = false gives a zero
 // row).  z1 / z2 are in D layout: z[mt][nt][e] is row 16 mt + 4 (l / 16) + e,
 // column 16 nt + l % 16.  Leaves the normalised X tile in lds_x and H1 =
-// GELU(z1) in lds_h.
+// GELU(z1) in lds_h; with lds_xt / lds_ht also X^T [16][kHStride] and H1^T
+// [64][kHStride] (the A operands of the backward pass's weight gradients).
 __device__ inline void mlp_tile_layers(const mlp_dev& W, const float* feat, bool valid,
                                        short* lds_x, short* lds_h, float (&z1)[4][4][4],
-                                       float (&z2)[4][4][4]) {
+                                       float (&z2)[4][4][4], short* lds_xt = nullptr,
+                                       short* lds_ht = nullptr) {
     const uint32_t l = __lane_id();
     const uint32_t r16 = l % 16u;
     const uint32_t g4 = 4u * (l / 16u);
@@ -118,6 +120,7 @@ __device__ inline void mlp_tile_layers(const mlp_dev& W, const float* feat, bool
             v = (feat[f] - W.mean[f]) * W.inv_std[f];
         }
         lds_x[l * kMlpInPad + f] = f2bf(v);
+        if (lds_xt != nullptr) lds_xt[f * kHStride + l] = f2bf(v);
     }
     wave_lds_sync();
     // layer 1: Z1 = X W1 + b1 (K = 16, one MFMA per tile)
@@ -130,10 +133,15 @@ __device__ inline void mlp_tile_layers(const mlp_dev& W, const float* feat, bool
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = mfma16(a, b, acc);
             const float bias = W.b1[16 * nt + r16];
+            bf16x4 hb;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 z1[mt][nt][e] = acc[e] + bias;
-                lds_h[(16 * mt + g4 + e) * kHStride + 16 * nt + r16] = f2bf(gelu(z1[mt][nt][e]));
+                hb[e] = f2bf(gelu(z1[mt][nt][e]));
+                lds_h[(16 * mt + g4 + e) * kHStride + 16 * nt + r16] = hb[e];
+            }
+            if (lds_ht != nullptr) {
+                *reinterpret_cast<bf16x4*>(lds_ht + (16 * nt + r16) * kHStride + 16 * mt + g4) = hb;
             }
         }
     }
