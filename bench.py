@@ -339,8 +339,9 @@ def ambiguity_line(wl, dev, cands, fitted, fd, stream, torch):
 
 
 def propagate_roofline(ctx, kt, workload_tag):
-    """Roofline of k_propagate (the dominant kernel of the pruned call):
-    algorithmic bytes per launch / its average launch duration (HIP events)."""
+    """Roofline of the propagation step (k_propagate and k_propagate_resume,
+    the dominant kernels of the pruned call): algorithmic bytes per step /
+    the step's average duration (HIP events)."""
     st = ctx["stats_pruned"]
     launches = max(kt["steps"], 1)
     lanes = kt["lanes"]
@@ -357,7 +358,11 @@ def propagate_roofline(ctx, kt, workload_tag):
     avg_ms = kt["propagate_kernel"] / launches
     per_launch = alg_bytes / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    kernel = f"k_propagate<{lanes}>"
+    # one "launch" is the propagation step: k_propagate and, with one lane
+    # per link, the k_propagate_resume that continues its deferred links
+    # (the HIP events bracket both)
+    kernels = [f"k_propagate<{lanes}>"] + (["k_propagate_resume"] if lanes == 1 else [])
+    kernel = " + ".join(kernels)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_tag}.json")
     if os.path.exists(tpath):
@@ -365,8 +370,8 @@ def propagate_roofline(ctx, kt, workload_tag):
         # (FETCH_SIZE + WRITE_SIZE, copy-calibrated), per working launch
         with open(tpath) as fh:
             tk = json.load(fh).get("calls", {}).get("pruned", {})
-        if kernel in tk:
-            traffic = int(tk[kernel]["hbm_bytes"] / launches)
+        if kernels[0] in tk:
+            traffic = int(sum(tk[k]["hbm_bytes"] for k in kernels if k in tk) / launches)
     return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
